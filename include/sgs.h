@@ -1,0 +1,142 @@
+/*
+ * C ABI of the MI355X-native stochastic Gaussian splatting renderer (libsgs.so).
+ *
+ * Plain pointers and sizes only. Every pointer named "device" is HIP device memory of
+ * the current device; the library never allocates: the caller passes outputs, persistent
+ * state (k-buffer, history) and a workspace of at least the size the matching
+ * *_workspace_size function reports. All work is enqueued on the caller's stream
+ * (a hipStream_t passed as void*; NULL is the null stream) and no entry point
+ * synchronises with the host.
+ *
+ * Every entry point returns SGS_SUCCESS (0) or a negative sgs_status; sgs_last_error()
+ * then describes the failure (thread-local).
+ *
+ * Reference interface cited as file:line of fhahlbohm/stochastic-gaussian-splatting,
+ * StochasticGSCudaBackend/StochasticGSCudaBackend/.
+ */
+#ifndef SGS_H
+#define SGS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* k-buffer entries per (sub)pixel: resolve.h:9, bound as _C.k_buffer_size (render.cu:164). */
+#define SGS_K_BUFFER_SIZE 4
+
+typedef enum sgs_status {
+    SGS_SUCCESS = 0,
+    SGS_ERROR_INVALID_ARGUMENT = -1,
+    SGS_ERROR_WORKSPACE_TOO_SMALL = -2,
+    SGS_ERROR_HIP = -3,
+} sgs_status;
+
+/* TemporalMode (render.py:44-47, resolve.h:11-15). */
+typedef enum sgs_temporal_mode {
+    SGS_TEMPORAL_RESET = 0,
+    SGS_TEMPORAL_STATIC = 1,
+    SGS_TEMPORAL_REPROJECT = 2,
+} sgs_temporal_mode;
+
+/* Gaussian parameters (render.py:73-77, render.cu:17-21): device pointers to contiguous
+ * float32 arrays. rotations must be 16-byte aligned. */
+typedef struct sgs_gaussians {
+    const float* means;           /* (N,3) world positions */
+    const float* scales;          /* (N,3) log scales */
+    const float* rotations;       /* (N,4) quaternion (r,i,j,k), unnormalized */
+    const float* opacities;       /* (N,) logits */
+    const float* sh_coefficients; /* (N,B,3) */
+    uint32_t n_gaussians;
+    uint32_t total_sh_bases;      /* B in {1,4,9,16} (render.cu:150) */
+} sgs_gaussians;
+
+/* Settings (render.py:8-42) in its as_tuple() order; w2c (4,4 row-major) and
+ * cam_position (3,) are device pointers like the reference's CUDA tensors. */
+typedef struct sgs_settings {
+    const float* w2c;
+    const float* cam_position;
+    uint32_t width;
+    uint32_t height;
+    float focal_x;
+    float focal_y;
+    float center_x;
+    float center_y;
+    float near_plane;
+    float far_plane;
+    int32_t proper_antialiasing;
+    int32_t supersampling_factor; /* values < 1 mean 1 (render.cu:52) */
+    int32_t use_gps;
+    int32_t hybrid_scheduling;
+    int32_t spatial_reuse;
+} sgs_settings;
+
+/* History (render.py:49-53): device pointers, colour/depth premultiplied by alpha. */
+typedef struct sgs_history {
+    float* image;       /* (H,W,3) */
+    float* depth;       /* (H,W,1) */
+    float* alpha;       /* (H,W,1) */
+    float* weight_mass; /* (H,W,1) */
+} sgs_history;
+
+/* Scheduling knobs, compile-time constants in the reference (scatter_utils.cuh:11,
+ * accumulate_hybrid.cu:19-29, accumulate_hybrid_gps.cu:18-22). A zero field takes its
+ * default; a NULL sgs_tuning* takes all defaults. None of them changes a result bit. */
+typedef struct sgs_tuning {
+    uint32_t sequential_threshold;    /* fragments/points one lane handles alone (default 8) */
+    uint32_t binning_pixel_threshold; /* hybrid ST: AABB area above which a Gaussian is binned (512) */
+    uint32_t binning_point_threshold; /* hybrid GPS: points above which a Gaussian is chunked (512) */
+    uint32_t max_binned_gaussians;    /* binning capacity (min(N, 2^20)); overflow is scattered */
+    uint32_t tile_size;               /* hybrid ST LDS tile edge: 16, 32 or 64 (32) */
+    uint32_t max_tile_instances;      /* tile-instance capacity (2^24); overflow is scattered */
+    uint32_t max_gps_points;          /* GPS points per Gaussian; 0 = rn(W*H/(2 ss^2)), the
+                                         reference's cap (accumulate_scatter.cu:128). This one
+                                         does change results when it binds. */
+} sgs_tuning;
+
+/* Replaces _C.k_buffer_size (render.cu:164). */
+int sgs_k_buffer_size(void);
+
+/* Thread-local description of the last failure of an sgs_* call on this thread. */
+const char* sgs_last_error(void);
+
+/* Bytes of device workspace sgs_render needs for these settings. */
+size_t sgs_render_workspace_size(const sgs_settings* settings, uint32_t n_gaussians, const sgs_tuning* tuning);
+
+/* Replaces _C.render (render.cu:16-45, 162-165; Python side render.py:72-95).
+ * One stochastic sample of one view: clear the packed framebuffer, accumulate the
+ * closest accepted fragment per (sub)pixel, shade it, resolve with the history.
+ *   blue_noise        device (64,128,128) uint8 volume or NULL/0 elements for white noise
+ *   k_buffer_*        device (K,P,2) int32 / (K,P,4) float32, updated in place; may be
+ *                     NULL unless settings->spatial_reuse
+ *   history           read in STATIC/REPROJECT modes (may be NULL in RESET)
+ *   view_to_previous_view device (4,4) float32, read in REPROJECT mode
+ *   out               fresh (H,W,*) outputs; must not alias history
+ *   frame_idx         the sample's RNG frame (the reference's process-global counter,
+ *                     render.cu:56-57, made explicit)
+ * On return (stream order) the first 8*P bytes of the workspace hold the packed
+ * framebuffer (float_bits(depth) << 32 | index, all ones = empty) of this sample. */
+int sgs_render(const sgs_gaussians* gaussians, const sgs_settings* settings,
+               const uint8_t* blue_noise, uint64_t blue_noise_numel,
+               int32_t* k_buffer_primitives, float* k_buffer_occlusion,
+               const sgs_history* history, const float* view_to_previous_view, int32_t temporal_mode,
+               const sgs_history* out, uint32_t frame_idx, const sgs_tuning* tuning,
+               void* workspace, size_t workspace_bytes, void* stream);
+
+/* No reference counterpart (the reference only binds render): the accumulation stage
+ * of sgs_render alone, into a caller framebuffer of (H*ss)*(W*ss) uint64 words that it
+ * clears first. counters, if not NULL, is a device uint64[4] that receives
+ * {fragments tested, atomics issued, Gaussians projected, Gaussians binned}; counting
+ * adds a few instructions per fragment, so timed runs pass NULL. */
+size_t sgs_accumulate_workspace_size(const sgs_settings* settings, uint32_t n_gaussians, const sgs_tuning* tuning);
+int sgs_accumulate(const sgs_gaussians* gaussians, const sgs_settings* settings,
+                   const uint8_t* blue_noise, uint64_t blue_noise_numel,
+                   uint64_t* framebuffer, uint32_t frame_idx, const sgs_tuning* tuning,
+                   uint64_t* counters, void* workspace, size_t workspace_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SGS_H */

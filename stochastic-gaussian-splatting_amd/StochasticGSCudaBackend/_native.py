@@ -1,0 +1,272 @@
+"""ctypes binding of libsgs.so (C ABI in include/sgs.h).
+
+Plays the role of the reference's pybind11 module ``_C`` (render.cu:162-165): ``render``
+with the same 29 positional arguments and ``k_buffer_size``. Torch only provides device
+memory and the current stream; all compute is in the HIP kernels of libsgs.so. Importing
+this module raises ImportError when the library has not been built, so nothing can run on
+a silent fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import itertools
+import os
+import threading
+from pathlib import Path
+from typing import NamedTuple
+
+import torch  # loads the HIP runtime that libsgs.so then binds to
+
+LIB_PATH = Path(os.environ.get("SGS_LIBRARY", Path(__file__).resolve().parent / "libsgs.so"))
+if not LIB_PATH.is_file():
+    raise ImportError(f"{LIB_PATH} is not built: run `python -c 'import __graft_entry__ as g; g.build()'` "
+                      "or `python stochastic-gaussian-splatting_amd/csrc/build.py`")
+_lib = C.CDLL(str(LIB_PATH))
+
+SGS_SUCCESS = 0
+SGS_ERROR_INVALID_ARGUMENT = -1
+SGS_ERROR_WORKSPACE_TOO_SMALL = -2
+SGS_ERROR_HIP = -3
+
+
+class sgs_gaussians(C.Structure):
+    _fields_ = [("means", C.c_void_p), ("scales", C.c_void_p), ("rotations", C.c_void_p),
+                ("opacities", C.c_void_p), ("sh_coefficients", C.c_void_p),
+                ("n_gaussians", C.c_uint32), ("total_sh_bases", C.c_uint32)]
+
+
+class sgs_settings(C.Structure):
+    _fields_ = [("w2c", C.c_void_p), ("cam_position", C.c_void_p), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("focal_x", C.c_float), ("focal_y", C.c_float), ("center_x", C.c_float), ("center_y", C.c_float),
+                ("near_plane", C.c_float), ("far_plane", C.c_float), ("proper_antialiasing", C.c_int32),
+                ("supersampling_factor", C.c_int32), ("use_gps", C.c_int32), ("hybrid_scheduling", C.c_int32),
+                ("spatial_reuse", C.c_int32)]
+
+
+class sgs_history(C.Structure):
+    _fields_ = [("image", C.c_void_p), ("depth", C.c_void_p), ("alpha", C.c_void_p), ("weight_mass", C.c_void_p)]
+
+
+class sgs_tuning(C.Structure):
+    _fields_ = [("sequential_threshold", C.c_uint32), ("binning_pixel_threshold", C.c_uint32),
+                ("binning_point_threshold", C.c_uint32), ("max_binned_gaussians", C.c_uint32),
+                ("tile_size", C.c_uint32), ("max_tile_instances", C.c_uint32), ("max_gps_points", C.c_uint32)]
+
+
+_P = C.POINTER
+_lib.sgs_k_buffer_size.restype = C.c_int
+_lib.sgs_k_buffer_size.argtypes = []
+_lib.sgs_last_error.restype = C.c_char_p
+_lib.sgs_last_error.argtypes = []
+_lib.sgs_render_workspace_size.restype = C.c_size_t
+_lib.sgs_render_workspace_size.argtypes = [_P(sgs_settings), C.c_uint32, _P(sgs_tuning)]
+_lib.sgs_render.restype = C.c_int
+_lib.sgs_render.argtypes = [_P(sgs_gaussians), _P(sgs_settings), C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p,
+                            _P(sgs_history), C.c_void_p, C.c_int32, _P(sgs_history), C.c_uint32, _P(sgs_tuning),
+                            C.c_void_p, C.c_size_t, C.c_void_p]
+_lib.sgs_accumulate_workspace_size.restype = C.c_size_t
+_lib.sgs_accumulate_workspace_size.argtypes = [_P(sgs_settings), C.c_uint32, _P(sgs_tuning)]
+_lib.sgs_accumulate.restype = C.c_int
+_lib.sgs_accumulate.argtypes = [_P(sgs_gaussians), _P(sgs_settings), C.c_void_p, C.c_uint64, C.c_void_p,
+                                C.c_uint32, _P(sgs_tuning), C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]
+
+k_buffer_size: int = _lib.sgs_k_buffer_size()
+
+
+class Tuning(NamedTuple):
+    """Scheduling knobs (sgs_tuning); 0 keeps the library default. Only max_gps_points changes results."""
+    sequential_threshold: int = 0
+    binning_pixel_threshold: int = 0
+    binning_point_threshold: int = 0
+    max_binned_gaussians: int = 0
+    tile_size: int = 0
+    max_tile_instances: int = 0
+    max_gps_points: int = 0
+
+    def c(self) -> sgs_tuning:
+        return sgs_tuning(*[int(v) for v in self])
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != SGS_SUCCESS:
+        msg = _lib.sgs_last_error().decode(errors="replace")
+        if rc in (SGS_ERROR_INVALID_ARGUMENT, SGS_ERROR_WORKSPACE_TOO_SMALL):
+            raise ValueError(f"{what}: {msg}")
+        raise RuntimeError(f"{what}: {msg}")
+
+
+# frame counter of the reference's process-global static (render.cu:56-57), made explicit
+_frame_counter = itertools.count()
+_frame_lock = threading.Lock()
+
+
+def next_frame_idx() -> int:
+    with _frame_lock:
+        return next(_frame_counter) & 0xFFFFFFFF
+
+
+def _f32(t: torch.Tensor, name: str, device: torch.device) -> torch.Tensor:
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name} must be a torch.Tensor")
+    if t.dtype != torch.float32:
+        raise TypeError(f"{name} must be float32, got {t.dtype}")
+    if t.device != device:
+        raise ValueError(f"{name} is on {t.device}, expected {device}")
+    return t.contiguous()
+
+
+def gaussians_struct(means, scales, rotations, opacities, sh_coefficients, keep: list) -> sgs_gaussians:
+    device = means.device
+    if device.type != "cuda":
+        raise ValueError("renderer not implemented in CPU mode: tensors must be on a ROCm device")
+    n = means.shape[0]
+    if means.dim() != 2 or means.shape[1] != 3:
+        raise ValueError(f"means must be (N,3), got {tuple(means.shape)}")
+    arrays = [_f32(means, "means", device), _f32(scales, "scales", device), _f32(rotations, "rotations", device),
+              _f32(opacities, "opacities", device), _f32(sh_coefficients, "sh_coefficients", device)]
+    m, s, r, o, sh = arrays
+    if s.shape != (n, 3) or r.shape != (n, 4) or o.numel() != n:
+        raise ValueError("scales (N,3), rotations (N,4) and opacities (N,) must match means (N,3)")
+    if sh.dim() != 3 or sh.shape[0] != n or sh.shape[2] != 3:
+        raise ValueError(f"sh_coefficients must be (N,B,3), got {tuple(sh.shape)}")
+    keep.extend(arrays)
+    return sgs_gaussians(m.data_ptr(), s.data_ptr(), r.data_ptr(), o.data_ptr(), sh.data_ptr(), n, sh.shape[1])
+
+
+def settings_struct(w2c, cam_position, width, height, focal_x, focal_y, center_x, center_y, near_plane, far_plane,
+                    proper_antialiasing, supersampling_factor, use_gps, hybrid_scheduling, spatial_reuse,
+                    device: torch.device, keep: list) -> sgs_settings:
+    w = torch.as_tensor(w2c).to(device=device, dtype=torch.float32).reshape(4, 4).contiguous()
+    cp = torch.as_tensor(cam_position).to(device=device, dtype=torch.float32).reshape(3).contiguous()
+    keep.extend([w, cp])
+    return sgs_settings(w.data_ptr(), cp.data_ptr(), int(width), int(height), float(focal_x), float(focal_y),
+                        float(center_x), float(center_y), float(near_plane), float(far_plane),
+                        int(bool(proper_antialiasing)), int(supersampling_factor), int(bool(use_gps)),
+                        int(bool(hybrid_scheduling)), int(bool(spatial_reuse)))
+
+
+def _ptr_or_null(t: torch.Tensor | None) -> int | None:
+    return None if t is None or t.numel() == 0 else t.data_ptr()
+
+
+def _workspace(nbytes: int, device: torch.device) -> torch.Tensor:
+    ws = torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=device)
+    if ws.data_ptr() % 256:
+        ws = torch.empty(int(nbytes) + 256, dtype=torch.uint8, device=device)
+        off = (-ws.data_ptr()) % 256
+        ws = ws[off:off + int(nbytes)]
+    return ws
+
+
+def render(means, scales, rotations, opacities, sh_coefficients, w2c, cam_position, width, height, focal_x, focal_y,
+           center_x, center_y, near_plane, far_plane, proper_antialiasing, supersampling_factor, use_gps,
+           hybrid_scheduling, spatial_reuse, blue_noise, k_buffer_primitives, k_buffer_occlusion, history_image,
+           history_depth, history_alpha, history_weight_mass, view_to_previous_view, temporal_mode, *,
+           frame_idx: int | None = None, tuning: Tuning | None = None, return_framebuffer: bool = False):
+    """Same positional arguments and result as _C.render (render.cu:16-45): fresh
+    (image (H,W,3), depth, alpha, weight_mass (H,W,1)) tensors. ``frame_idx`` defaults to the
+    process-global counter; ``return_framebuffer`` appends the packed framebuffer (P,) int64."""
+    device = means.device
+    keep: list = []
+    g = gaussians_struct(means, scales, rotations, opacities, sh_coefficients, keep)
+    s = settings_struct(w2c, cam_position, width, height, focal_x, focal_y, center_x, center_y, near_plane, far_plane,
+                        proper_antialiasing, supersampling_factor, use_gps, hybrid_scheduling, spatial_reuse, device, keep)
+    mode = int(temporal_mode)
+    H, W = int(height), int(width)
+    with torch.cuda.device(device):
+        opts = dict(dtype=torch.float32, device=device)
+        image = torch.empty((H, W, 3), **opts)
+        depth = torch.empty((H, W, 1), **opts)
+        alpha = torch.empty((H, W, 1), **opts)
+        weight_mass = torch.empty((H, W, 1), **opts)
+        hist_t = [history_image, history_depth, history_alpha, history_weight_mass]
+        hist = None
+        if mode != 0:
+            hist_t = [_f32(h, n, device) for h, n in zip(hist_t, ("history.image", "history.depth", "history.alpha",
+                                                                   "history.weight_mass"))]
+            if hist_t[0].shape != (H, W, 3) or any(h.numel() != H * W for h in hist_t[1:]):
+                raise ValueError("history tensors do not match the image size")
+            keep.extend(hist_t)
+            hist = sgs_history(*[h.data_ptr() for h in hist_t])
+        v2p = None
+        if mode == 2:
+            v2p_t = torch.as_tensor(view_to_previous_view).to(device=device, dtype=torch.float32).reshape(4, 4)
+            v2p_t = v2p_t.contiguous()
+            keep.append(v2p_t)
+            v2p = v2p_t.data_ptr()
+        kp = ko = None
+        if spatial_reuse:
+            P = W * H * max(1, int(supersampling_factor)) ** 2
+            if k_buffer_primitives.shape != (k_buffer_size, P, 2) or k_buffer_occlusion.shape != (k_buffer_size, P, 4):
+                raise ValueError("k-buffer does not match the settings (use create_k_buffer)")
+            if not (k_buffer_primitives.is_contiguous() and k_buffer_occlusion.is_contiguous()):
+                raise ValueError("k-buffer tensors must be contiguous (they are updated in place)")
+            kp, ko = k_buffer_primitives.data_ptr(), k_buffer_occlusion.data_ptr()
+        bn = None
+        bn_numel = 0
+        if blue_noise is not None and blue_noise.numel() > 0:
+            bn_t = blue_noise.to(device=device, dtype=torch.uint8).contiguous()
+            keep.append(bn_t)
+            bn, bn_numel = bn_t.data_ptr(), bn_t.numel()
+        tc = tuning.c() if tuning is not None else None
+        tp = C.byref(tc) if tc is not None else None
+        nbytes = _lib.sgs_render_workspace_size(C.byref(s), g.n_gaussians, tp)
+        ws = _workspace(nbytes, device)
+        fidx = next_frame_idx() if frame_idx is None else int(frame_idx) & 0xFFFFFFFF
+        out = sgs_history(image.data_ptr(), depth.data_ptr(), alpha.data_ptr(), weight_mass.data_ptr())
+        stream = torch.cuda.current_stream(device).cuda_stream
+        rc = _lib.sgs_render(C.byref(g), C.byref(s), bn, bn_numel, kp, ko, C.byref(hist) if hist else None, v2p, mode,
+                             C.byref(out), fidx, tp, ws.data_ptr(), ws.numel(), stream)
+        _check(rc, "sgs_render")
+        if return_framebuffer:
+            P = W * H * max(1, int(supersampling_factor)) ** 2
+            fb = ws[:8 * P].view(torch.int64).clone()
+            return image, depth, alpha, weight_mass, fb
+        return image, depth, alpha, weight_mass
+
+
+def accumulate(means, scales, rotations, opacities, sh_coefficients, w2c, cam_position, width, height, focal_x,
+               focal_y, center_x, center_y, near_plane, far_plane, proper_antialiasing, supersampling_factor, use_gps,
+               hybrid_scheduling, spatial_reuse=False, blue_noise=None, *, frame_idx: int, tuning: Tuning | None = None,
+               framebuffer: torch.Tensor | None = None, counters: torch.Tensor | None = None,
+               workspace: torch.Tensor | None = None) -> torch.Tensor:
+    """Accumulation stage only (sgs_accumulate): returns the packed framebuffer as
+    (H*ss, W*ss) int64 (uint64 bits). ``counters`` (4,) int64 receives the work counts."""
+    device = means.device
+    keep: list = []
+    g = gaussians_struct(means, scales, rotations, opacities, sh_coefficients, keep)
+    s = settings_struct(w2c, cam_position, width, height, focal_x, focal_y, center_x, center_y, near_plane, far_plane,
+                        proper_antialiasing, supersampling_factor, use_gps, hybrid_scheduling, spatial_reuse, device, keep)
+    ss = max(1, int(supersampling_factor))
+    FH, FW = int(height) * ss, int(width) * ss
+    with torch.cuda.device(device):
+        if framebuffer is None:
+            framebuffer = torch.empty((FH, FW), dtype=torch.int64, device=device)
+        elif framebuffer.numel() != FH * FW or framebuffer.dtype != torch.int64 or not framebuffer.is_contiguous():
+            raise ValueError("framebuffer must be a contiguous int64 tensor of (H*ss)*(W*ss) elements")
+        tc = tuning.c() if tuning is not None else None
+        tp = C.byref(tc) if tc is not None else None
+        nbytes = _lib.sgs_accumulate_workspace_size(C.byref(s), g.n_gaussians, tp)
+        if nbytes and (workspace is None or workspace.numel() < nbytes or workspace.data_ptr() % 256):
+            workspace = _workspace(nbytes, device)
+        bn = None
+        bn_numel = 0
+        if blue_noise is not None and blue_noise.numel() > 0:
+            bn_t = blue_noise.to(device=device, dtype=torch.uint8).contiguous()
+            keep.append(bn_t)
+            bn, bn_numel = bn_t.data_ptr(), bn_t.numel()
+        cptr = None
+        if counters is not None:
+            if counters.dtype != torch.int64 or counters.numel() < 4 or not counters.is_contiguous():
+                raise ValueError("counters must be a contiguous int64 tensor of 4 elements")
+            cptr = counters.data_ptr()
+        stream = torch.cuda.current_stream(device).cuda_stream
+        rc = _lib.sgs_accumulate(C.byref(g), C.byref(s), bn, bn_numel, framebuffer.data_ptr(), int(frame_idx) & 0xFFFFFFFF,
+                                 tp, cptr, workspace.data_ptr() if nbytes else None, nbytes, stream)
+        _check(rc, "sgs_accumulate")
+    return framebuffer.view(FH, FW)
+
+
+def library():
+    """The loaded ctypes.CDLL (for symbol checks)."""
+    return _lib

@@ -1,0 +1,134 @@
+// Host-side declarations shared by the translation units of libsgs.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "sgs.h"
+#include "sgs_device.h"
+
+namespace sgs {
+
+// Gaussian arrays as the kernels read them.
+struct SceneArgs {
+    const float* means;
+    const float* scales;
+    const float* rotations;
+    const float* opacities;
+    const float* sh;
+    uint32_t n;
+    uint32_t sh_bases;
+};
+
+// Resolved scheduling knobs (zero fields of sgs_tuning replaced by defaults).
+struct Tuning {
+    uint32_t sequential_threshold;
+    uint32_t binning_pixel_threshold;
+    uint32_t binning_point_threshold;
+    uint32_t max_binned;
+    uint32_t tile_size;
+    uint32_t max_tile_instances;
+    uint32_t max_gps_points;  // 0 = reference heuristic
+};
+
+// One binned large Gaussian of the hybrid ST path (64 bytes, 16-byte aligned).
+struct alignas(16) BinnedST {
+    uint64_t fragment;
+    float mean_x, mean_y;          // mean2d - 0.5 (scatter pixel convention)
+    float ce_a, ce_b, ce_c, opacity;
+    float cov_x, cov_y, cov_z, power_threshold;
+    uint32_t x0x1;                 // x0 | x1 << 16 (AABB, framebuffer pixels)
+    uint32_t y0y1;                 // y0 | y1 << 16
+};
+static_assert(sizeof(BinnedST) == 64, "BinnedST layout");
+
+// One binned Gaussian of the hybrid GPS path (64 bytes).
+struct alignas(16) BinnedGPS {
+    uint64_t fragment;
+    float mean_x, mean_y;
+    float L0, L1, L2, power_threshold;
+    float conic_x, conic_y, conic_z, opacity_rcp;
+    float opacity_dilog;
+    uint32_t n_points;
+    uint32_t seed_x, seed_y;       // make_seed(index, frame) before the count draw
+};
+static_assert(sizeof(BinnedGPS) == 64, "BinnedGPS layout");
+
+// Device counters at the start of the hybrid scratch area.
+struct HybridCounters {
+    uint32_t n_binned;             // may exceed the capacity; min() is taken by readers
+    uint32_t n_instances;          // hybrid ST tile instances (total, may exceed capacity)
+    uint32_t overflow;             // non-zero: instance list overflowed, binned set is scattered
+    uint32_t pad[61];
+};
+
+// Binning outputs of an accumulation kernel (all null when not binning).
+struct BinArgs {
+    HybridCounters* counters;
+    void* binned;                  // BinnedST* or BinnedGPS*
+    uint32_t capacity;
+    uint32_t threshold;
+};
+
+// Workspace carve-up. Every sub-buffer is 256-byte aligned.
+struct Arena {
+    size_t offset = 0;
+    size_t take(size_t bytes)
+    {
+        const size_t at = offset;
+        offset += (bytes + 255u) & ~size_t(255u);
+        return at;
+    }
+};
+
+FbCamera make_fb_camera(uint32_t width, uint32_t height, float focal_x, float focal_y, float center_x, float center_y,
+                        float near_plane, float far_plane, int32_t proper_antialiasing, int32_t supersampling_factor);
+uint32_t gps_max_points(const FbCamera& cam, uint32_t override_points);
+
+// ---- launchers (sgs_accumulate.hip) ----
+hipError_t launch_fill_u64(uint64_t* p, size_t n, uint64_t value, hipStream_t stream);
+hipError_t launch_zero_u32(uint32_t* p, size_t n, hipStream_t stream);
+hipError_t launch_st_scatter(const SceneArgs& sc, const float* w2c, const FbCamera& cam, uint64_t* fb, uint32_t frame,
+                             const uint8_t* blue_noise_slice, uint32_t blue_noise_rotation, uint32_t seq_threshold,
+                             const BinArgs& bin, uint64_t* counters, hipStream_t stream);
+hipError_t launch_gps_scatter(const SceneArgs& sc, const float* w2c, const FbCamera& cam, uint64_t* fb, uint32_t frame,
+                              uint32_t max_points, uint32_t seq_threshold, const BinArgs& bin, uint64_t* counters,
+                              hipStream_t stream);
+
+// ---- launchers (sgs_hybrid.hip) ----
+size_t hybrid_st_scratch_bytes(uint32_t fb_width, uint32_t fb_height, const Tuning& t);
+size_t hybrid_gps_scratch_bytes(const Tuning& t);
+hipError_t run_hybrid_st(const SceneArgs& sc, const float* w2c, const FbCamera& cam, uint64_t* fb, uint32_t frame,
+                         const uint8_t* bn_slice, uint32_t bn_rotation, const Tuning& t, void* scratch,
+                         uint64_t* counters, hipStream_t stream);
+hipError_t run_hybrid_gps(const SceneArgs& sc, const float* w2c, const FbCamera& cam, uint64_t* fb, uint32_t frame,
+                          uint32_t max_points, const Tuning& t, void* scratch, uint64_t* counters,
+                          hipStream_t stream);
+
+// ---- launchers (sgs_resolve.hip) ----
+struct HistoryPtrs {
+    const float* image;
+    const float* depth;
+    const float* alpha;
+    const float* weight_mass;
+};
+struct OutputPtrs {
+    float* image;
+    float* depth;
+    float* alpha;
+    float* weight_mass;
+};
+hipError_t launch_sample(const SceneArgs& sc, const float* w2c, const float* cam_position, const FbCamera& cam,
+                         const uint64_t* fb, float4* samples, float4* reuse_weights, uint2* k_prim, float4* k_occ,
+                         bool spatial_reuse, hipStream_t stream);
+hipError_t launch_resolve(const FbCamera& cam, uint32_t width, uint32_t height, float focal_x, float focal_y,
+                          float center_x, float center_y, const float4* samples, const float4* reuse_weights,
+                          const HistoryPtrs& hist, const float* view_to_previous_view, OutputPtrs out,
+                          bool spatial_reuse, int32_t temporal_mode, hipStream_t stream);
+// sample + resolve fused (no spatial reuse, RESET/STATIC): reads the framebuffer directly
+hipError_t launch_shade_resolve(const SceneArgs& sc, const float* cam_position, const FbCamera& cam, uint32_t width,
+                                uint32_t height, const uint64_t* fb, const HistoryPtrs& hist, OutputPtrs out,
+                                int32_t temporal_mode, hipStream_t stream);
+
+}  // namespace sgs

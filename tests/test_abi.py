@@ -1,0 +1,102 @@
+"""CPU checks of the C ABI: libsgs.so loads, exports every function include/sgs.h
+declares, and rejects bad arguments on the host before touching a GPU. Also checks the
+Python API mirrors the reference's render.py (names, fields, enum values)."""
+from __future__ import annotations
+
+import ctypes as C
+import re
+from pathlib import Path
+
+import pytest
+
+REPO = Path(__file__).resolve().parents[1]
+HEADER = REPO / "include" / "sgs.h"
+
+
+def declared_functions() -> list[str]:
+    text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
+    return sorted(set(re.findall(r"\b(sgs_[a-z0-9_]+)\s*\(", text)))
+
+
+@pytest.fixture(scope="module")
+def native():
+    from StochasticGSCudaBackend import _native
+    return _native
+
+
+def test_library_exports_every_declared_symbol(native):
+    names = declared_functions()
+    assert {"sgs_render", "sgs_k_buffer_size", "sgs_last_error", "sgs_render_workspace_size"} <= set(names)
+    lib = native.library()
+    for name in names:
+        assert hasattr(lib, name), name
+
+
+def test_k_buffer_size(native):
+    assert native.k_buffer_size == 4 == native.library().sgs_k_buffer_size()
+
+
+def _settings(native, **kw):
+    s = native.sgs_settings(w2c=0x1000, cam_position=0x2000, width=64, height=48, focal_x=50.0, focal_y=50.0,
+                            center_x=32.0, center_y=24.0, near_plane=0.2, far_plane=100.0, proper_antialiasing=0,
+                            supersampling_factor=1, use_gps=0, hybrid_scheduling=0, spatial_reuse=0)
+    for k, v in kw.items():
+        setattr(s, k, v)
+    return s
+
+
+def _gaussians(native, **kw):
+    g = native.sgs_gaussians(means=0x100, scales=0x200, rotations=0x300, opacities=0x400, sh_coefficients=0x500,
+                             n_gaussians=10, total_sh_bases=16)
+    for k, v in kw.items():
+        setattr(g, k, v)
+    return g
+
+
+def _render(native, g, s, mode=0, history=None, out=True, ws_bytes=1 << 30, kp=None):
+    o = native.sgs_history(0x10000, 0x20000, 0x30000, 0x40000) if out else None
+    return native.library().sgs_render(C.byref(g), C.byref(s), None, 0, kp, kp, C.byref(history) if history else None,
+                                       None, mode, C.byref(o) if o else None, 0, None, 0x100000, ws_bytes, None)
+
+
+def test_host_side_validation(native):
+    lib = native.library()
+    g, s = _gaussians(native), _settings(native)
+    cases = [
+        (dict(mode=3), "invalid temporal mode"),
+        (dict(mode=1), "need a history"),
+        (dict(out=False), "output"),
+        (dict(ws_bytes=16), "workspace"),
+    ]
+    for kw, msg in cases:
+        rc = _render(native, g, s, **kw)
+        assert rc < 0, kw
+        assert msg in lib.sgs_last_error().decode()
+    assert _render(native, _gaussians(native, total_sh_bases=5), s) == native.SGS_ERROR_INVALID_ARGUMENT
+    assert _render(native, _gaussians(native, rotations=0x304), s) == native.SGS_ERROR_INVALID_ARGUMENT
+    assert _render(native, g, _settings(native, width=0)) == native.SGS_ERROR_INVALID_ARGUMENT
+    assert _render(native, g, _settings(native, width=70000)) == native.SGS_ERROR_INVALID_ARGUMENT
+    assert _render(native, g, _settings(native, spatial_reuse=1)) == native.SGS_ERROR_INVALID_ARGUMENT
+    assert _render(native, g, _settings(native, w2c=None)) == native.SGS_ERROR_INVALID_ARGUMENT
+    assert _render(native, g, s, ws_bytes=16) == native.SGS_ERROR_WORKSPACE_TOO_SMALL
+
+
+def test_workspace_sizes(native):
+    lib = native.library()
+    s = _settings(native)
+    base = lib.sgs_render_workspace_size(C.byref(s), 10, None)
+    assert base >= 64 * 48 * (8 + 16)
+    s2 = _settings(native, supersampling_factor=2, spatial_reuse=1)
+    assert lib.sgs_render_workspace_size(C.byref(s2), 10, None) >= 4 * 64 * 48 * (8 + 16 + 16)
+    assert lib.sgs_accumulate_workspace_size(C.byref(s), 10, None) == 0
+
+
+def test_python_api_mirrors_reference():
+    from StochasticGSCudaBackend import History, Settings, TemporalMode, create_k_buffer, render  # noqa: F401
+    assert Settings._fields == ("w2c", "cam_position", "width", "height", "focal_x", "focal_y", "center_x",
+                                "center_y", "near_plane", "far_plane", "proper_antialiasing", "supersampling_factor",
+                                "use_gps", "hybrid_scheduling", "spatial_reuse")
+    assert History._fields == ("image", "depth", "alpha", "weight_mass")
+    assert [(m.name, int(m)) for m in TemporalMode] == [("RESET", 0), ("STATIC", 1), ("REPROJECT", 2)]
+    s = Settings(*range(15))
+    assert s.as_tuple() == tuple(range(15))
