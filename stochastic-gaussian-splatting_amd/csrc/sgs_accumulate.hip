@@ -126,10 +126,9 @@ __global__ __launch_bounds__(kBlock) void st_scatter_kernel(SceneArgs sc, const 
                 float4* dst = reinterpret_cast<float4*>(static_cast<BinnedST*>(bin.binned) + slot);
                 const Aabb b = pixel_aabb(p, cam);
                 dst[0] = make_float4(__uint_as_float((uint32_t)frag), __uint_as_float((uint32_t)(frag >> 32)), msx, msy);
-                dst[1] = make_float4(ce.a, ce.b, ce.c, o);
-                dst[2] = make_float4(p.cov_x, p.cov_y, p.cov_z, p.power_threshold);
-                dst[3] = make_float4(__uint_as_float(b.x0 | (b.x1 << 16)), __uint_as_float(b.y0 | (b.y1 << 16)), 0.0f,
-                                     0.0f);
+                dst[1] = make_float4(p.conic_x, p.conic_y, p.conic_z, o);
+                dst[2] = make_float4(p.power_threshold, __uint_as_float(b.x0 | (b.x1 << 16)),
+                                     __uint_as_float(b.y0 | (b.y1 << 16)), 0.0f);
                 active = false;
                 n_binned = 1;
             }
@@ -240,26 +239,6 @@ hipError_t launch_st_scatter(const SceneArgs& sc, const float* w2c, const FbCame
 // ------------------------------------------------------------------------------------------
 // Gaussian point splatting
 // ------------------------------------------------------------------------------------------
-// Point k of a Gaussian: per-point counter seed, radius-corrected Box-Muller offset through
-// the Cholesky factor L of cov2d, power test at the pixel centre (scatter_utils.cuh:115-130).
-__device__ __forceinline__ bool gps_point(uint32_t k, uint32_t seed_x, uint32_t seed_y, float mx, float my, float L0,
-                                          float L1, float L2, float cx, float cy, float cz, float pt, float o_rcp,
-                                          float o_dilog, const FbCamera& cam, uint32_t& pixel)
-{
-    uint2 st = make_uint2(seed_x ^ ((k + 1u) * 0x9E3779B9u), seed_y);
-    const float2 u = pcg2d(st);
-    const float2 xy = corrected_box_muller(u.x, u.y, o_rcp, o_dilog);
-    const float sx = mx + L0 * xy.x;
-    const float sy = my + (L1 * xy.x + L2 * xy.y);
-    if (!(sx < cam.width && sx >= 0.0f && sy < cam.height && sy >= 0.0f)) return false;
-    const float dcx = truncf(sx) + 0.5f - mx;
-    const float dcy = truncf(sy) + 0.5f - my;
-    const float power = 0.5f * (cx * dcx * dcx + cz * dcy * dcy) + cy * dcx * dcy;
-    if (power > pt) return false;
-    pixel = (uint32_t)sy * cam.width_i + (uint32_t)sx;
-    return true;
-}
-
 template <bool kBin, bool kCount>
 __global__ __launch_bounds__(kBlock) void gps_scatter_kernel(SceneArgs sc, const float4* __restrict__ w2c, FbCamera cam,
                                                              uint64_t* __restrict__ fb, uint32_t frame,
@@ -325,7 +304,7 @@ __global__ __launch_bounds__(kBlock) void gps_scatter_kernel(SceneArgs sc, const
     if (active && count <= seq) {
         for (uint32_t k = 0; k < count; k++) {
             uint32_t pix;
-            if (gps_point(k, seed_x, seed_y, mx, my, L0, L1, L2, cx, cy, cz, pt, o_rcp, o_dilog, cam, pix)) {
+            if (gps_point_at(k, seed_x, seed_y, mx, my, L0, L1, L2, cx, cy, cz, pt, o_rcp, o_dilog, cam, pix)) {
                 atomicMin(reinterpret_cast<unsigned long long*>(&fb[pix]), (unsigned long long)frag);
                 if constexpr (kCount) n_accepted++;
             }
@@ -345,7 +324,7 @@ __global__ __launch_bounds__(kBlock) void gps_scatter_kernel(SceneArgs sc, const
         const uint64_t cfrag = bcast(frag, l);
         for (uint32_t k = lane; k < cn; k += 64u) {
             uint32_t pix;
-            if (gps_point(k, csx, csy, cmx, cmy, cL0, cL1, cL2, ccx, ccy, ccz, cpt, corc, codl, cam, pix)) {
+            if (gps_point_at(k, csx, csy, cmx, cmy, cL0, cL1, cL2, ccx, ccy, ccz, cpt, corc, codl, cam, pix)) {
                 atomicMin(reinterpret_cast<unsigned long long*>(&fb[pix]), (unsigned long long)cfrag);
                 if constexpr (kCount) n_accepted++;
             }

@@ -435,6 +435,27 @@ __device__ __forceinline__ Aabb pixel_aabb(const Projected& p, const FbCamera& c
     return b;
 }
 
+// ------------------------------------------------------------------------------------------
+// GPS point k of a Gaussian: per-point counter seed, radius-corrected Box-Muller offset through
+// the Cholesky factor L of cov2d, power test at the pixel centre (scatter_utils.cuh:115-130).
+__device__ __forceinline__ bool gps_point_at(uint32_t k, uint32_t seed_x, uint32_t seed_y, float mx, float my, float L0,
+                                          float L1, float L2, float cx, float cy, float cz, float pt, float o_rcp,
+                                          float o_dilog, const FbCamera& cam, uint32_t& pixel)
+{
+    uint2 st = make_uint2(seed_x ^ ((k + 1u) * 0x9E3779B9u), seed_y);
+    const float2 u = pcg2d(st);
+    const float2 xy = corrected_box_muller(u.x, u.y, o_rcp, o_dilog);
+    const float sx = mx + L0 * xy.x;
+    const float sy = my + (L1 * xy.x + L2 * xy.y);
+    if (!(sx < cam.width && sx >= 0.0f && sy < cam.height && sy >= 0.0f)) return false;
+    const float dcx = truncf(sx) + 0.5f - mx;
+    const float dcy = truncf(sy) + 0.5f - my;
+    const float power = 0.5f * (cx * dcx * dcx + cz * dcy * dcy) + cy * dcx * dcy;
+    if (power > pt) return false;
+    pixel = (uint32_t)sy * cam.width_i + (uint32_t)sx;
+    return true;
+}
+
 // wave-uniform broadcast of a lane's value through an SGPR (v_readlane_b32)
 __device__ __forceinline__ uint32_t bcast(uint32_t v, int lane)
 {

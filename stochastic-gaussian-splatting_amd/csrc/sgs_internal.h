@@ -36,10 +36,11 @@ struct Tuning {
 struct alignas(16) BinnedST {
     uint64_t fragment;
     float mean_x, mean_y;          // mean2d - 0.5 (scatter pixel convention)
-    float ce_a, ce_b, ce_c, opacity;
-    float cov_x, cov_y, cov_z, power_threshold;
+    float conic_x, conic_y, conic_z, opacity;
+    float power_threshold;
     uint32_t x0x1;                 // x0 | x1 << 16 (AABB, framebuffer pixels)
     uint32_t y0y1;                 // y0 | y1 << 16
+    uint32_t pad[5];
 };
 static_assert(sizeof(BinnedST) == 64, "BinnedST layout");
 

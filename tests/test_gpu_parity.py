@@ -113,7 +113,9 @@ def _gpu_render(gpu, t, cam, frame, hist=None, mode=0, v2p=None, kb=None):
     out = gpu.render(*gpu_args(t, cam), torch.empty(0, dtype=torch.uint8, device="cuda"), kb[0], kb[1], *hist, v2p_t,
                      mode, frame_idx=frame, return_framebuffer=True)
     torch.cuda.synchronize()
-    return [o.cpu().numpy() for o in out[:4]], out[4].cpu().numpy().view(np.uint64)
+    ss = max(1, int(cam.get("supersampling_factor", 1)))
+    fb = out[4].cpu().numpy().view(np.uint64).reshape(H * ss, W * ss)
+    return [o.cpu().numpy() for o in out[:4]], fb
 
 
 def _close(got, ref):
@@ -218,3 +220,62 @@ def test_public_render_api(gpu):
     torch.cuda.synchronize()
     assert history.image.shape == (48, 64, 3) and history.alpha.shape == (48, 64, 1)
     assert float(history.alpha.max()) <= 1.0 + 1e-5 and float(history.alpha.max()) > 0.1
+
+
+# ---------------------------------------------------------------------------------------------
+# hybrid scheduling: must equal the scatter path (and hence the oracle) bit for bit
+# ---------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("tile,threshold,max_instances,max_binned,blue", [
+    (32, 64, 0, 0, False), (16, 32, 0, 0, False), (64, 128, 0, 0, True),
+    (32, 64, 1, 0, False),      # instance list overflow: binned Gaussians go to the fallback scatter
+    (32, 16, 0, 7, False),      # binning capacity overflow: the rest is scattered directly
+])
+def test_hybrid_st_equals_scatter(gpu, oracle_mod, tile, threshold, max_instances, max_binned, blue):
+    import torch
+    arrays = small_scene(3000, seed=31, log_scale_mean=-3.2, log_scale_std=0.9)
+    sc, t = oracle_scene(arrays), to_device(arrays)
+    cam = small_camera(200, 150, hybrid_scheduling=True)
+    vol = np.random.default_rng(4).integers(0, 256, (64, 128, 128), dtype=np.uint8) if blue else None
+    tuning = gpu.Tuning(tile_size=tile, binning_pixel_threshold=threshold, max_tile_instances=max_instances,
+                        max_binned_gaussians=max_binned)
+    counters = torch.zeros(4, dtype=torch.int64, device="cuda")
+    for frame in (2, 70):
+        print_case(test="hybrid_st", tile=tile, threshold=threshold, max_instances=max_instances,
+                   max_binned=max_binned, blue=blue, frame=frame)
+        if blue:
+            ref, _ = oracle_mod.accumulate(sc, cam, frame, blue_noise_slice=vol[frame % 64].reshape(-1),
+                                           blue_noise_rotation=((frame // 64) * 0x9E3779B9) & 0xFFFFFFFF)
+        else:
+            ref, _ = oracle_mod.accumulate(sc, cam, frame)
+        got = gpu_fb(gpu, t, cam, frame, tuning=tuning, counters=counters,
+                     blue=torch.from_numpy(vol).cuda() if blue else None)
+        assert int(counters[3]) > (0 if max_binned == 0 else 6), "nothing was binned"
+        assert np.array_equal(got, ref), f"{np.count_nonzero(got != ref)} of {ref.size} pixels differ"
+
+
+@pytest.mark.parametrize("ss,threshold,max_binned", [(1, 32, 0), (2, 64, 0), (1, 16, 5)])
+def test_hybrid_gps_equals_scatter(gpu, oracle_mod, ss, threshold, max_binned):
+    import torch
+    arrays = small_scene(2000, seed=32, log_scale_mean=-3.3, log_scale_std=0.8)
+    sc, t = oracle_scene(arrays), to_device(arrays)
+    cam = small_camera(160, 120, supersampling_factor=ss, use_gps=True, hybrid_scheduling=True)
+    tuning = gpu.Tuning(binning_point_threshold=threshold, max_binned_gaussians=max_binned)
+    counters = torch.zeros(4, dtype=torch.int64, device="cuda")
+    for frame in (1, 9):
+        print_case(test="hybrid_gps", ss=ss, threshold=threshold, max_binned=max_binned, frame=frame)
+        ref, _ = oracle_mod.accumulate(sc, cam, frame)
+        got = gpu_fb(gpu, t, cam, frame, tuning=tuning, counters=counters)
+        assert int(counters[3]) > 0, "nothing was binned"
+        assert np.array_equal(got, ref), f"{np.count_nonzero(got != ref)} of {ref.size} pixels differ"
+
+
+def test_hybrid_render_matches_oracle(gpu, oracle_mod):
+    arrays = small_scene(2000, seed=33, log_scale_mean=-3.3)
+    sc, t = oracle_scene(arrays), to_device(arrays)
+    for gps in (False, True):
+        cam = small_camera(96, 72, hybrid_scheduling=True, use_gps=gps)
+        print_case(test="hybrid_render", gps=gps)
+        ref = oracle_mod.render(sc, cam, 5)
+        got, fb = _gpu_render(gpu, t, cam, 5)
+        assert np.array_equal(fb, ref[4])
+        _close(got, ref[:4])
