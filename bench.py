@@ -40,7 +40,12 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--gaussians", type=int, default=6_000_000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU oracle sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["plain", "batched", "all"], default="all")
+    ap.add_argument("--only", choices=["plain", "batched", "all"], default="all",
+                    help="run only one path (profiling); the JSON line then reports that path")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="copy 1 GiB once first, to calibrate HBM byte counters against a known size")
+    ap.add_argument("--no-morton", action="store_true",
+                    help="keep generation order instead of the Morton order scenes are stored in")
     return ap.parse_args()
 
 
@@ -50,6 +55,23 @@ def view_settings(cam: dict, device):
     return Settings(torch.from_numpy(cam["w2c"]).to(device), torch.from_numpy(cam["cam_position"]).to(device),
                     cam["width"], cam["height"], cam["focal_x"], cam["focal_y"], cam["center_x"], cam["center_y"],
                     cam["near_plane"], cam["far_plane"], False, 1, False, False, False)
+
+
+def host_settings(cam: dict):
+    import torch
+    from StochasticGSCudaBackend import Settings
+    return Settings(torch.from_numpy(cam["w2c"]), torch.from_numpy(cam["cam_position"]), cam["width"], cam["height"],
+                    cam["focal_x"], cam["focal_y"], cam["center_x"], cam["center_y"], cam["near_plane"],
+                    cam["far_plane"], False, 1, False, False, False)
+
+
+def frames_for_step(views, frame_base: int) -> list[int]:
+    """First RNG frame of every view, in the order run_plain assigns them."""
+    out, f = [], frame_base
+    for cam in views:
+        out.append(f)
+        f += cam["spp"]
+    return out
 
 
 def samples_per_step(views) -> int:
@@ -143,6 +165,31 @@ def roofline_scatter(scene_n: int, views, ms, work):
     }
 
 
+def roofline_batch(scene_n: int, views, acc_ms: list, work):
+    """Algorithmic bytes of one persistent accumulation launch: per view 44 B of parameters
+    for every Gaussian (12 B means for the depth cull + 32 B scales/rotation/opacity) and 8 B
+    per atomicMin issued (sample s of view v: the per-sample count of the counting pass
+    times spp)."""
+    atomics = sum(work[v][1] * cam["spp"] for v, cam in enumerate(views))
+    tested = sum(work[v][0] for v in range(len(views)))
+    b = 44.0 * scene_n * len(views) + 8.0 * atomics
+    t = sum(acc_ms) / len(acc_ms) * 1e-3
+    achieved = b / t / 1e9
+    return {
+        "kernel": "sgs::batch_accumulate_kernel",
+        "bound": "hbm",
+        "achieved": round(achieved, 2),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "avg_launch_ms": round(t * 1e3, 3),
+        "algorithmic_bytes_per_launch": int(b),
+        "atomics_per_s": round(atomics / t, 1),
+        "fragment_alphas_per_s": round(tested / t, 1),
+    }
+
+
 def cpu_baseline(scene_np, views, budget_s: float):
     """The oracle (scalar C, OpenMP over Gaussians) rendering views of the same workload,
     one sample each, until the time budget is spent."""
@@ -174,13 +221,16 @@ def main() -> int:
     import numpy as np
     import torch
     from StochasticGSCudaBackend import create_k_buffer
-    from StochasticGSCudaBackend.synthetic import make_scene, scaled, scene_c4, views_c4
+    from StochasticGSCudaBackend.synthetic import make_scene, morton_order, reorder, scaled, scene_c4, views_c4
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     t0 = time.perf_counter()
     spec = scaled(scene_c4(), args.gaussians)
     scene_np = make_scene(spec)
+    if not args.no_morton:
+        # scenes are stored Morton-ordered, as the reference's model loader does (Model.py:32-34)
+        scene_np = reorder(scene_np, morton_order(scene_np["means"]))
     scene = {k: torch.from_numpy(v).to(dev) for k, v in scene_np.items()}
     views = views_c4(args.views)
     settings = [view_settings(v, dev) for v in views]
@@ -189,62 +239,154 @@ def main() -> int:
     log(f"scene {spec.n} Gaussians, {len(views)} views, {samples_per_step(views) / 1e6:.1f} M pixel-samples/step "
         f"(setup {time.perf_counter() - t0:.1f} s)")
 
+    if args.copy_probe:
+        src = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     S = samples_per_step(views)
     result = {}
-    # ---- plain: per-view sequential render() calls -------------------------------------
-    for w in range(args.warmup):
-        run_plain(scene, views, settings, k_buffers, frame_base=1_000_000 * (w + 1))
-        torch.cuda.synchronize()
-        log(f"plain warmup {w + 1}/{args.warmup} done")
-    step_ms = []
-    torch.cuda.synchronize()
-    t_all = time.perf_counter()
-    for k in range(args.steps):
-        ts = time.perf_counter()
-        run_plain(scene, views, settings, k_buffers, frame_base=10_000 * k)
-        torch.cuda.synchronize()
-        step_ms.append((time.perf_counter() - ts) * 1e3)
-        log(f"plain step {k + 1}/{args.steps}: {step_ms[-1]:.1f} ms")
-    plain_s = time.perf_counter() - t_all
-    plain_value = S * args.steps / plain_s / 1e6
-    result["plain"] = {"value": round(plain_value, 2), "ms_per_step": round(plain_s * 1e3 / args.steps, 2),
-                       "p50_step_ms": round(float(np.percentile(step_ms, 50)), 2),
-                       "p99_step_ms": round(float(np.percentile(step_ms, 99)), 2)}
+    plain_out0 = None
+    if args.only in ("plain", "all"):
+        result["plain"], plain_out0 = bench_plain(args, scene, views, settings, k_buffers, S)
+    if args.only in ("batched", "all"):
+        result["batched"], acc_ms = bench_batched(args, scene, views, S, plain_out0, dev)
+        if "plain" in result:
+            result["batched"]["speedup_vs_plain"] = round(result["batched"]["value"] / result["plain"]["value"], 3)
 
-    # ---- roofline of the dominant kernel ----------------------------------------------------
+    # ---- roofline of the dominant kernel (and of the plain scatter) ----------------------------
     ms, work = time_scatter_launches(scene, views)
-    roof = roofline_scatter(spec.n, views, ms, work)
-    log(f"scatter launches: avg {roof['avg_launch_ms']} ms, {roof['achieved']} GB/s algorithmic")
+    roof_plain = roofline_scatter(spec.n, views, ms, work)
+    log(f"plain scatter launches: avg {roof_plain['avg_launch_ms']} ms, {roof_plain['achieved']} GB/s algorithmic")
+    roof = roof_plain
+    if "batched" in result:
+        roof = roofline_batch(spec.n, views, acc_ms, work)
+        log(f"batched accumulate: {roof['avg_launch_ms']} ms, {roof['achieved']} GB/s algorithmic")
 
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(scene_np, views, args.cpu_seconds)
         log(f"cpu oracle: {cpu['value']} Msamples/s on {cpu['cores']} threads")
 
-    value = plain_value
-    ms_per_step = plain_s * 1e3 / args.steps
+    head = result.get("batched", result.get("plain"))
     line = {
         "metric": "pixel-samples/s (Msamples/s) batched vs plain, 6M Gaussians; p50/p99 frame ms",
-        "value": round(value, 2),
+        "value": head["value"],
         "unit": "Msamples/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 2),
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"{spec.n} Gaussians (8-component mixture, SH3), {len(views)} views on a ring, "
-                               f"resolutions 256x256..2048x1152, spp in {{1,4,16}}, ST scatter, ss=1",
-                   "path": "plain per-view render() calls", "pixel_samples_per_step": S},
-        "plain": result["plain"],
+        "config": {"workload": f"{spec.n} Gaussians (8-component mixture, SH3, Morton order), {len(views)} views "
+                               f"on a ring, resolutions 256x256..2048x1152, spp in {{1,4,16}}, ST scatter, ss=1",
+                   "path": ("batched persistent launch (all views and samples)" if "batched" in result
+                            else "plain per-view render() calls"),
+                   "pixel_samples_per_step": S},
+        "batched": result.get("batched"),
+        "plain": result.get("plain"),
+        "p50_frame_ms": result["plain"]["p50_frame_ms"] if "plain" in result else None,
+        "p99_frame_ms": result["plain"]["p99_frame_ms"] if "plain" in result else None,
         "roofline": roof,
+        "roofline_plain": roof_plain,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
     return 0
+
+
+def bench_plain(args, scene, views, settings, k_buffers, S):
+    """Per-view sequential render() calls of the drop-in API: throughput and frame latency."""
+    import numpy as np
+    import torch
+    for w in range(args.warmup):
+        run_plain(scene, views, settings, k_buffers, frame_base=1_000_000 * (w + 1))
+        torch.cuda.synchronize()
+        log(f"plain warmup {w + 1}/{args.warmup} done")
+    out0 = None
+    torch.cuda.synchronize()
+    t_all = time.perf_counter()
+    for k in range(args.steps):
+        ts = time.perf_counter()
+        out = run_plain(scene, views, settings, k_buffers, frame_base=10_000 * k)
+        torch.cuda.synchronize()
+        if k == 0:
+            out0 = out
+        log(f"plain step {k + 1}/{args.steps}: {(time.perf_counter() - ts) * 1e3:.1f} ms")
+    plain_s = time.perf_counter() - t_all
+    # per-frame latency: one view with all its samples, synchronised (outside the timed loop)
+    frame_ms = []
+    for v in range(len(views)):
+        ts = time.perf_counter()
+        run_plain(scene, views[v:v + 1], settings[v:v + 1], k_buffers[v:v + 1], frame_base=500_000 + 100 * v)
+        torch.cuda.synchronize()
+        frame_ms.append((time.perf_counter() - ts) * 1e3)
+    return {"value": round(S * args.steps / plain_s / 1e6, 2), "ms_per_step": round(plain_s * 1e3 / args.steps, 2),
+            "p50_frame_ms": round(float(np.percentile(frame_ms, 50)), 3),
+            "p99_frame_ms": round(float(np.percentile(frame_ms, 99)), 3)}, out0
+
+
+def bench_batched(args, scene, views, S, plain_out0, dev):
+    """Every view and sample in one persistent launch; stage timing with HIP events on the
+    library's stream; graph replay; bit comparison with the plain outputs of the same frames."""
+    import numpy as np
+    import torch
+    from StochasticGSCudaBackend.batch import BatchRenderer, BatchView
+    bviews = [BatchView(host_settings(cam), spp=cam["spp"], frame_idx=0) for cam in views]
+    br = BatchRenderer(scene["means"], scene["scales"], scene["rotations"], scene["opacities"], scene["sh"], bviews)
+    max_diff = None
+    if plain_out0 is not None:
+        br.set_frames(frames_for_step(views, 0))
+        br.upload_table()
+        br.render()
+        torch.cuda.synchronize()
+        max_diff = 0.0
+        for a, b in zip(br.outputs, plain_out0):
+            for x, y in zip(a, b):
+                max_diff = max(max_diff, float((x - y).abs().max()))
+        log(f"batched vs plain (same frames): max abs difference {max_diff}")
+    stream = torch.cuda.current_stream(dev)
+    for w in range(args.warmup):
+        br.set_frames(frames_for_step(views, 2_000_000 + 10_000 * w))
+        br.upload_table()
+        br.render()
+    torch.cuda.synchronize()
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
+    for e4 in ev:
+        for e in e4:
+            e.record(stream)
+    torch.cuda.synchronize()
+    t_all = time.perf_counter()
+    for k in range(args.steps):
+        ts = time.perf_counter()
+        br.set_frames(frames_for_step(views, 10_000 * k))
+        br.upload_table()
+        br.render(stage_events=ev[k])
+        torch.cuda.synchronize()
+        log(f"batched step {k + 1}/{args.steps}: {(time.perf_counter() - ts) * 1e3:.1f} ms")
+    batch_s = time.perf_counter() - t_all
+    acc_ms = [e4[1].elapsed_time(e4[2]) for e4 in ev]
+    stage = {"clear_ms": round(float(np.mean([e4[0].elapsed_time(e4[1]) for e4 in ev])), 3),
+             "accumulate_ms": round(float(np.mean(acc_ms)), 3),
+             "resolve_ms": round(float(np.mean([e4[2].elapsed_time(e4[3]) for e4 in ev])), 3)}
+    graph = br.capture()
+    torch.cuda.synchronize()
+    g_ms = []
+    for k in range(args.steps):
+        ts = time.perf_counter()
+        br.set_frames(frames_for_step(views, 3_000_000 + 10_000 * k))
+        br.upload_table()
+        graph.replay()
+        torch.cuda.synchronize()
+        g_ms.append((time.perf_counter() - ts) * 1e3)
+    return {"value": round(S * args.steps / batch_s / 1e6, 2), "ms_per_step": round(batch_s * 1e3 / args.steps, 2),
+            "graph_ms_per_step": round(float(np.mean(g_ms)), 2), "stages": stage,
+            "max_abs_diff_vs_plain": max_diff}, acc_ms
 
 
 if __name__ == "__main__":

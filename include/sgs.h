@@ -136,6 +136,65 @@ int sgs_accumulate(const sgs_gaussians* gaussians, const sgs_settings* settings,
                    uint64_t* framebuffer, uint32_t frame_idx, const sgs_tuning* tuning,
                    uint64_t* counters, void* workspace, size_t workspace_bytes, void* stream);
 
+
+/* ---------------------------------------------------------------------------------------
+ * Batched multi-view rendering (no reference counterpart: the reference renders one view
+ * per call from Python). Every view renders spp samples in one persistent launch; view v's
+ * result equals spp successive sgs_render calls with frames frame_idx .. frame_idx+spp-1,
+ * the first with temporal_mode and the rest STATIC (identical framebuffer winners).
+ * Spatial reuse, GPS, hybrid scheduling and blue noise are single-view features.
+ * --------------------------------------------------------------------------------------- */
+typedef struct sgs_view {
+    float w2c[16];                    /* host copy, row-major world->view */
+    float cam_position[3];            /* host copy */
+    uint32_t width;
+    uint32_t height;
+    float focal_x;
+    float focal_y;
+    float center_x;
+    float center_y;
+    float near_plane;
+    float far_plane;
+    int32_t proper_antialiasing;
+    int32_t supersampling_factor;
+    uint32_t spp;                     /* samples of this view, >= 1 */
+    uint32_t frame_idx;               /* RNG frame of sample 0 */
+    int32_t temporal_mode;            /* mode of sample 0 */
+    float view_to_previous_view[16];  /* host copy, read in REPROJECT mode */
+    sgs_history history;              /* device, read unless RESET */
+    sgs_history out;                  /* device outputs (H,W,*) */
+} sgs_view;
+
+/* Scheduling of the persistent batched accumulation; zero fields take defaults.
+ * None of them changes a result bit. */
+typedef struct sgs_batch_tuning {
+    uint32_t sequential_threshold;    /* fragments one lane handles alone (8) */
+    uint32_t big_threshold;           /* fragments above which a Gaussian is split into
+                                         work-queue entries (4096) */
+    uint32_t entry_fragments;         /* fragments per queue entry (2048) */
+    uint32_t queue_capacity;          /* queue entries (2^20); overflow is scattered in place */
+    uint32_t view_major;              /* 1: items ordered view by view; 0: Gaussian chunk by
+                                         chunk across all views (default) */
+    uint32_t blocks_per_cu;           /* persistent workgroups per CU (0: occupancy limit) */
+} sgs_batch_tuning;
+
+/* Device workspace needed by sgs_render_batch for these views. */
+size_t sgs_render_batch_workspace_size(const sgs_view* views, uint32_t n_views, const sgs_batch_tuning* tuning);
+
+/* Bytes of the encoded view table, and the encoding itself into host memory (for callers
+ * that upload it once, e.g. before capturing the render into a graph). */
+size_t sgs_batch_view_table_bytes(uint32_t n_views);
+int sgs_batch_encode_views(const sgs_view* views, uint32_t n_views, void* host_table);
+
+/* Renders all views. view_table_device: the encoded table already in device memory, or
+ * NULL to let the library copy it into the workspace (a pageable host-to-device copy,
+ * not capturable into a graph). stage_events: NULL, or 4 hipEvent_t (as void*) recorded
+ * on the stream before the framebuffer clear, before and after the accumulation launch,
+ * and after the resolve, for timing the stages. */
+int sgs_render_batch(const sgs_gaussians* gaussians, const sgs_view* views, uint32_t n_views,
+                     const void* view_table_device, const sgs_batch_tuning* tuning,
+                     void* workspace, size_t workspace_bytes, void* const* stage_events, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

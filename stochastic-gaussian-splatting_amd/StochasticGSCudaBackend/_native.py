@@ -53,6 +53,21 @@ class sgs_tuning(C.Structure):
                 ("tile_size", C.c_uint32), ("max_tile_instances", C.c_uint32), ("max_gps_points", C.c_uint32)]
 
 
+class sgs_view(C.Structure):
+    _fields_ = [("w2c", C.c_float * 16), ("cam_position", C.c_float * 3), ("width", C.c_uint32),
+                ("height", C.c_uint32), ("focal_x", C.c_float), ("focal_y", C.c_float), ("center_x", C.c_float),
+                ("center_y", C.c_float), ("near_plane", C.c_float), ("far_plane", C.c_float),
+                ("proper_antialiasing", C.c_int32), ("supersampling_factor", C.c_int32), ("spp", C.c_uint32),
+                ("frame_idx", C.c_uint32), ("temporal_mode", C.c_int32), ("view_to_previous_view", C.c_float * 16),
+                ("history", sgs_history), ("out", sgs_history)]
+
+
+class sgs_batch_tuning(C.Structure):
+    _fields_ = [("sequential_threshold", C.c_uint32), ("big_threshold", C.c_uint32),
+                ("entry_fragments", C.c_uint32), ("queue_capacity", C.c_uint32), ("view_major", C.c_uint32),
+                ("blocks_per_cu", C.c_uint32)]
+
+
 _P = C.POINTER
 _lib.sgs_k_buffer_size.restype = C.c_int
 _lib.sgs_k_buffer_size.argtypes = []
@@ -69,6 +84,16 @@ _lib.sgs_accumulate_workspace_size.argtypes = [_P(sgs_settings), C.c_uint32, _P(
 _lib.sgs_accumulate.restype = C.c_int
 _lib.sgs_accumulate.argtypes = [_P(sgs_gaussians), _P(sgs_settings), C.c_void_p, C.c_uint64, C.c_void_p,
                                 C.c_uint32, _P(sgs_tuning), C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]
+
+_lib.sgs_render_batch_workspace_size.restype = C.c_size_t
+_lib.sgs_render_batch_workspace_size.argtypes = [C.c_void_p, C.c_uint32, _P(sgs_batch_tuning)]
+_lib.sgs_batch_view_table_bytes.restype = C.c_size_t
+_lib.sgs_batch_view_table_bytes.argtypes = [C.c_uint32]
+_lib.sgs_batch_encode_views.restype = C.c_int
+_lib.sgs_batch_encode_views.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p]
+_lib.sgs_render_batch.restype = C.c_int
+_lib.sgs_render_batch.argtypes = [_P(sgs_gaussians), C.c_void_p, C.c_uint32, C.c_void_p, _P(sgs_batch_tuning),
+                                  C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p]
 
 k_buffer_size: int = _lib.sgs_k_buffer_size()
 

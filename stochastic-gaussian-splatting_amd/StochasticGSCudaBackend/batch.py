@@ -1,0 +1,164 @@
+"""Batched multi-view rendering: many cameras of uneven resolution and spp in one
+persistent launch (sgs_render_batch).
+
+View v's result equals ``spp`` successive ``render()`` calls with frames
+``frame_idx .. frame_idx + spp - 1``, the first with ``temporal_mode`` and the rest STATIC
+(the SPP loop of the reference's Renderer.py:85-87), with identical framebuffer winners.
+``BatchRenderer`` keeps the workspace, outputs and the encoded view table resident so a
+render can be repeated or captured into a graph and replayed.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import NamedTuple, Sequence
+
+import numpy as np
+import torch
+
+from . import _native
+from .render import History, Settings, TemporalMode
+
+
+class BatchTuning(NamedTuple):
+    """sgs_batch_tuning; 0 keeps the library default. No field changes a result bit."""
+    sequential_threshold: int = 0
+    big_threshold: int = 0
+    entry_fragments: int = 0
+    queue_capacity: int = 0
+    view_major: int = 0
+    blocks_per_cu: int = 0
+
+    def c(self) -> _native.sgs_batch_tuning:
+        return _native.sgs_batch_tuning(*[int(v) for v in self])
+
+
+@dataclass
+class BatchView:
+    settings: Settings
+    spp: int = 1
+    frame_idx: int | None = None  # None: the next spp frames of the process-global counter
+    temporal_mode: TemporalMode = TemporalMode.RESET
+    history: History | None = None
+    view_to_previous_view: torch.Tensor | np.ndarray | None = None
+
+
+def _host(t, shape) -> np.ndarray:
+    if isinstance(t, torch.Tensor):
+        t = t.detach().to("cpu")
+    return np.ascontiguousarray(np.asarray(t, np.float32).reshape(shape))
+
+
+def _check_settings(s: Settings) -> None:
+    if s.use_gps or s.hybrid_scheduling or s.spatial_reuse:
+        raise ValueError("batched rendering covers stochastic transparency without spatial reuse; "
+                         "render GPS, hybrid or spatial-reuse views with render()")
+
+
+class BatchRenderer:
+    """Renders a fixed list of views repeatedly; outputs are ``self.outputs`` (one History
+    per view, overwritten by every render)."""
+
+    def __init__(self, means, scales, rotations, opacities, sh_coefficients, views: Sequence[BatchView],
+                 tuning: BatchTuning | None = None):
+        self.device = means.device
+        self._keep: list = []
+        self.gaussians = _native.gaussians_struct(means, scales, rotations, opacities, sh_coefficients, self._keep)
+        self.n_views = len(views)
+        self.views = list(views)
+        self.tuning = tuning.c() if tuning is not None else None
+        self._tp = C.byref(self.tuning) if self.tuning is not None else None
+        self.c_views = (_native.sgs_view * max(1, self.n_views))()
+        self.outputs: list[History] = []
+        opts = dict(dtype=torch.float32, device=self.device)
+        for i, v in enumerate(self.views):
+            s = v.settings
+            _check_settings(s)
+            H, W = int(s.height), int(s.width)
+            out = History(torch.empty((H, W, 3), **opts), torch.empty((H, W, 1), **opts),
+                          torch.empty((H, W, 1), **opts), torch.empty((H, W, 1), **opts))
+            self.outputs.append(out)
+            cv = self.c_views[i]
+            cv.w2c[:] = _host(s.w2c, 16).tolist()
+            cv.cam_position[:] = _host(s.cam_position, 3).tolist()
+            cv.width, cv.height = W, H
+            cv.focal_x, cv.focal_y = float(s.focal_x), float(s.focal_y)
+            cv.center_x, cv.center_y = float(s.center_x), float(s.center_y)
+            cv.near_plane, cv.far_plane = float(s.near_plane), float(s.far_plane)
+            cv.proper_antialiasing = int(bool(s.proper_antialiasing))
+            cv.supersampling_factor = int(s.supersampling_factor)
+            cv.spp = int(v.spp)
+            cv.temporal_mode = int(v.temporal_mode)
+            v2p = np.eye(4, dtype=np.float32) if v.view_to_previous_view is None else _host(v.view_to_previous_view, 16)
+            cv.view_to_previous_view[:] = np.asarray(v2p, np.float32).reshape(16).tolist()
+            if int(v.temporal_mode) != 0:
+                if v.history is None:
+                    raise ValueError(f"view {i}: STATIC and REPROJECT need a history")
+                hist = [_native._f32(h, n, self.device) for h, n in zip(v.history, History._fields)]
+                self._keep.extend(hist)
+                cv.history = _native.sgs_history(*[h.data_ptr() for h in hist])
+            cv.out = _native.sgs_history(*[o.data_ptr() for o in out])
+        self.set_frames([v.frame_idx for v in self.views])
+        nbytes = _native._lib.sgs_render_batch_workspace_size(self.c_views, self.n_views, self._tp)
+        self.workspace = _native._workspace(nbytes, self.device)
+        tb = _native._lib.sgs_batch_view_table_bytes(self.n_views)
+        self._table_host = torch.empty(tb, dtype=torch.uint8, pin_memory=True)
+        self.table = torch.empty(tb, dtype=torch.uint8, device=self.device)
+        self._copied = None
+        self.upload_table()
+
+    def set_frames(self, frames: Sequence[int | None]) -> None:
+        """Sets each view's first RNG frame (None: take spp frames from the global counter).
+        Call upload_table() afterwards to make a captured graph see them."""
+        for i, f in enumerate(frames):
+            if f is None:
+                f = _native.next_frame_idx()
+                for _ in range(self.c_views[i].spp - 1):
+                    _native.next_frame_idx()
+            self.c_views[i].frame_idx = int(f) & 0xFFFFFFFF
+
+    def upload_table(self) -> None:
+        """Encodes the views into the pinned staging buffer and copies it to the device table
+        on the current stream (after the previous copy has read the staging buffer)."""
+        if self._copied is not None:
+            self._copied.synchronize()
+        rc = _native._lib.sgs_batch_encode_views(self.c_views, self.n_views, self._table_host.data_ptr())
+        _native._check(rc, "sgs_batch_encode_views")
+        with torch.cuda.device(self.device):
+            self.table.copy_(self._table_host, non_blocking=True)
+            self._copied = torch.cuda.Event()
+            self._copied.record()
+
+    def render(self, stage_events: Sequence["torch.cuda.Event"] | None = None) -> list[History]:
+        """stage_events: 4 recorded-once events marked before the clear, before and after the
+        accumulation launch and after the resolve (for timing)."""
+        ev = None
+        if stage_events is not None:
+            ev = (C.c_void_p * 4)(*[e.cuda_event for e in stage_events])
+        with torch.cuda.device(self.device):
+            stream = torch.cuda.current_stream(self.device).cuda_stream
+            rc = _native._lib.sgs_render_batch(C.byref(self.gaussians), self.c_views, self.n_views,
+                                               self.table.data_ptr(), self._tp, self.workspace.data_ptr(),
+                                               self.workspace.numel(), ev, stream)
+        _native._check(rc, "sgs_render_batch")
+        return self.outputs
+
+    def capture(self) -> "torch.cuda.CUDAGraph":
+        """Captures render() into a graph (HIP graph on ROCm); replay() re-renders with the
+        view table's current contents."""
+        graph = torch.cuda.CUDAGraph()
+        side = torch.cuda.Stream(device=self.device)
+        side.wait_stream(torch.cuda.current_stream(self.device))
+        with torch.cuda.stream(side):
+            self.render()  # warm-up on the capture stream
+        torch.cuda.current_stream(self.device).wait_stream(side)
+        with torch.cuda.graph(graph):
+            self.render()
+        return graph
+
+
+def render_batch(means, scales, rotations, opacities, sh_coefficients, views: Sequence[BatchView],
+                 tuning: BatchTuning | None = None) -> list[History]:
+    """Renders every view (all of its samples) in one persistent launch; fresh outputs."""
+    r = BatchRenderer(means, scales, rotations, opacities, sh_coefficients, views, tuning)
+    return r.render()

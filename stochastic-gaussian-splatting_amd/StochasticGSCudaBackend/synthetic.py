@@ -62,6 +62,24 @@ def make_scene(spec: SceneSpec) -> dict[str, np.ndarray]:
     }
 
 
+def morton_order(means: np.ndarray, bits: int = 10) -> np.ndarray:
+    """Permutation sorting Gaussians by the Morton code of their means quantised to
+    2^bits cells per axis of the bounding box, the order the reference's model loader
+    stores scenes in (Model.py:32-34), which keeps screen-space neighbours together."""
+    lo = means.min(axis=0)
+    span = np.maximum(means.max(axis=0) - lo, 1e-12)
+    q = np.clip(((means - lo) / span * (1 << bits)).astype(np.int64), 0, (1 << bits) - 1)
+    code = np.zeros(len(means), np.int64)
+    for b in range(bits):
+        for axis in range(3):
+            code |= ((q[:, axis] >> b) & 1) << (3 * b + axis)
+    return np.argsort(code, kind="stable")
+
+
+def reorder(arrays: dict[str, np.ndarray], order: np.ndarray) -> dict[str, np.ndarray]:
+    return {k: np.ascontiguousarray(v[order]) for k, v in arrays.items()}
+
+
 def look_at(eye, target=(0.0, 0.0, 0.0), up=(0.0, -1.0, 0.0)) -> np.ndarray:
     """4x4 world-to-view matrix, +z towards ``target``, image +y along -``up``."""
     eye = np.asarray(eye, np.float64)

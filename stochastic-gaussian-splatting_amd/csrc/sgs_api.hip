@@ -306,3 +306,235 @@ int sgs_accumulate(const sgs_gaussians* gaussians, const sgs_settings* s, const 
 }
 
 }  // extern "C"
+
+// ============================================================================================
+// batched multi-view rendering
+// ============================================================================================
+namespace {
+
+BatchTuning resolve_batch_tuning(const sgs_batch_tuning* t)
+{
+    BatchTuning r;
+    r.sequential_threshold = (t && t->sequential_threshold) ? t->sequential_threshold : 8u;
+    r.big_threshold = (t && t->big_threshold) ? t->big_threshold : 4096u;
+    r.entry_fragments = (t && t->entry_fragments) ? t->entry_fragments : 2048u;
+    r.queue_capacity = (t && t->queue_capacity) ? t->queue_capacity : (1u << 20);
+    r.view_major = (t && t->view_major) ? 1u : 0u;
+    r.blocks_per_cu = t ? t->blocks_per_cu : 0u;
+    return r;
+}
+
+constexpr uint32_t kResolveBlock = 256;
+
+size_t table_views_bytes(uint32_t n) { return ((size_t)n * sizeof(ViewDesc) + 255u) & ~size_t(255u); }
+size_t table_bytes(uint32_t n) { return table_views_bytes(n) + 4u * ((size_t)n + 1u); }
+
+uint32_t view_ss(const sgs_view& v) { return v.supersampling_factor < 1 ? 1u : (uint32_t)v.supersampling_factor; }
+uint64_t view_fb_pixels(const sgs_view& v)
+{
+    const uint64_t ss = view_ss(v);
+    return (uint64_t)v.width * ss * v.height * ss;
+}
+
+struct BatchLayout {
+    size_t counters, ready, queue, table, fb, samples, total;
+    uint64_t fb_words, n_samples;
+    uint32_t n_blocks;
+};
+
+BatchLayout batch_layout(const sgs_view* views, uint32_t n, const BatchTuning& t)
+{
+    BatchLayout L;
+    L.fb_words = 0;
+    L.n_samples = 0;
+    uint64_t blocks = 0;
+    for (uint32_t v = 0; v < n; v++) {
+        const uint64_t P = view_fb_pixels(views[v]);
+        L.fb_words += P * views[v].spp;
+        if (views[v].temporal_mode == SGS_TEMPORAL_REPROJECT) L.n_samples += P;
+        blocks += ((uint64_t)views[v].width * views[v].height + kResolveBlock - 1u) / kResolveBlock;
+    }
+    L.n_blocks = (uint32_t)blocks;
+    Arena a;
+    L.counters = a.take(sizeof(BatchCounters));
+    L.ready = a.offset;  // contiguous with the counters: one zero-fill clears both
+    a.offset += 4u * (size_t)t.queue_capacity;
+    a.offset = (a.offset + 255u) & ~size_t(255u);
+    L.queue = a.take(sizeof(BigEntry) * (size_t)t.queue_capacity);
+    L.table = a.take(table_bytes(n));
+    L.fb = a.take(8u * L.fb_words);
+    L.samples = a.take(16u * (L.n_samples ? L.n_samples : 1u));
+    L.total = a.offset;
+    return L;
+}
+
+int check_views(const sgs_view* views, uint32_t n)
+{
+    if (!views && n) return fail(SGS_ERROR_INVALID_ARGUMENT, "views is NULL");
+    uint64_t blocks = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        const sgs_view& v = views[i];
+        const std::string at = "view " + std::to_string(i) + ": ";
+        if (v.width == 0 || v.height == 0) return fail(SGS_ERROR_INVALID_ARGUMENT, at + "width and height must be > 0");
+        const uint64_t ss = view_ss(v);
+        if ((uint64_t)v.width * ss > 65535u || (uint64_t)v.height * ss > 65535u || view_fb_pixels(v) >= (1ull << 32))
+            return fail(SGS_ERROR_INVALID_ARGUMENT, at + "framebuffer too large");
+        if (v.spp < 1 || v.spp > 4096) return fail(SGS_ERROR_INVALID_ARGUMENT, at + "spp must be in [1, 4096]");
+        if (!(v.focal_x != 0.0f) || !(v.focal_y != 0.0f)) return fail(SGS_ERROR_INVALID_ARGUMENT, at + "focal length is 0");
+        if (v.temporal_mode < SGS_TEMPORAL_RESET || v.temporal_mode > SGS_TEMPORAL_REPROJECT)
+            return fail(SGS_ERROR_INVALID_ARGUMENT, at + "invalid temporal mode " + std::to_string(v.temporal_mode));
+        if (!v.out.image || !v.out.depth || !v.out.alpha || !v.out.weight_mass)
+            return fail(SGS_ERROR_INVALID_ARGUMENT, at + "output pointers are NULL");
+        if (v.temporal_mode != SGS_TEMPORAL_RESET &&
+            (!v.history.image || !v.history.depth || !v.history.alpha || !v.history.weight_mass))
+            return fail(SGS_ERROR_INVALID_ARGUMENT, at + "STATIC and REPROJECT need a history");
+        if (v.temporal_mode != SGS_TEMPORAL_RESET &&
+            (v.history.image == v.out.image || v.history.depth == v.out.depth || v.history.alpha == v.out.alpha ||
+             v.history.weight_mass == v.out.weight_mass))
+            return fail(SGS_ERROR_INVALID_ARGUMENT, at + "outputs must not alias the history");
+        blocks += ((uint64_t)v.width * v.height + kResolveBlock - 1u) / kResolveBlock;
+    }
+    if (blocks >= (1ull << 31)) return fail(SGS_ERROR_INVALID_ARGUMENT, "too many output pixels in one batch");
+    return SGS_SUCCESS;
+}
+
+void encode_views(const sgs_view* views, uint32_t n, void* host)
+{
+    ViewDesc* d = static_cast<ViewDesc*>(host);
+    uint32_t* prefix = reinterpret_cast<uint32_t*>(static_cast<char*>(host) + table_views_bytes(n));
+    uint64_t fb_off = 0, smp_off = 0;
+    uint32_t block = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        const sgs_view& v = views[i];
+        ViewDesc x;
+        std::memset(&x, 0, sizeof x);
+        x.r1 = make_float4(v.w2c[0], v.w2c[1], v.w2c[2], v.w2c[3]);
+        x.r2 = make_float4(v.w2c[4], v.w2c[5], v.w2c[6], v.w2c[7]);
+        x.r3 = make_float4(v.w2c[8], v.w2c[9], v.w2c[10], v.w2c[11]);
+        x.cam = make_fb_camera(v.width, v.height, v.focal_x, v.focal_y, v.center_x, v.center_y, v.near_plane,
+                               v.far_plane, v.proper_antialiasing, v.supersampling_factor);
+        x.cam_x = v.cam_position[0];
+        x.cam_y = v.cam_position[1];
+        x.cam_z = v.cam_position[2];
+        x.focal_x = v.focal_x;
+        x.focal_y = v.focal_y;
+        x.center_x = v.center_x;
+        x.center_y = v.center_y;
+        x.width = v.width;
+        x.height = v.height;
+        x.spp = v.spp;
+        x.frame0 = v.frame_idx;
+        x.mode = v.temporal_mode;
+        x.block_begin = block;
+        x.fb_offset = fb_off;
+        x.samples_offset = smp_off;
+        x.hist = { v.history.image, v.history.depth, v.history.alpha, v.history.weight_mass };
+        x.out = { v.out.image, v.out.depth, v.out.alpha, v.out.weight_mass };
+        std::memcpy(x.v2p, v.view_to_previous_view, sizeof x.v2p);
+        d[i] = x;
+        prefix[i] = block;
+        const uint64_t P = view_fb_pixels(v);
+        fb_off += P * v.spp;
+        if (v.temporal_mode == SGS_TEMPORAL_REPROJECT) smp_off += P;
+        block += (uint32_t)(((uint64_t)v.width * v.height + kResolveBlock - 1u) / kResolveBlock);
+    }
+    prefix[n] = block;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t sgs_batch_view_table_bytes(uint32_t n_views) { return table_bytes(n_views); }
+
+int sgs_batch_encode_views(const sgs_view* views, uint32_t n_views, void* host_table)
+{
+    int rc;
+    if ((rc = check_views(views, n_views))) return rc;
+    if (!host_table) return fail(SGS_ERROR_INVALID_ARGUMENT, "host_table is NULL");
+    encode_views(views, n_views, host_table);
+    return SGS_SUCCESS;
+}
+
+size_t sgs_render_batch_workspace_size(const sgs_view* views, uint32_t n_views, const sgs_batch_tuning* tuning)
+{
+    if (!views && n_views) return 0;
+    return batch_layout(views, n_views, resolve_batch_tuning(tuning)).total;
+}
+
+int sgs_render_batch(const sgs_gaussians* gaussians, const sgs_view* views, uint32_t n_views,
+                     const void* view_table_device, const sgs_batch_tuning* tuning, void* workspace,
+                     size_t workspace_bytes, void* const* stage_events, void* stream_ptr)
+{
+    int rc;
+    if ((rc = check_gaussians(gaussians)) || (rc = check_views(views, n_views))) return rc;
+    if (n_views == 0) return SGS_SUCCESS;
+    const BatchTuning t = resolve_batch_tuning(tuning);
+    if (t.entry_fragments < 64u) return fail(SGS_ERROR_INVALID_ARGUMENT, "entry_fragments must be >= 64");
+    const BatchLayout L = batch_layout(views, n_views, t);
+    if (!workspace || workspace_bytes < L.total)
+        return fail(SGS_ERROR_WORKSPACE_TOO_SMALL,
+                    "workspace of " + std::to_string(L.total) + " bytes needed, got " + std::to_string(workspace_bytes));
+    if (reinterpret_cast<uintptr_t>(workspace) % 256u != 0)
+        return fail(SGS_ERROR_INVALID_ARGUMENT, "workspace must be 256-byte aligned");
+    if (view_table_device && reinterpret_cast<uintptr_t>(view_table_device) % 16u != 0)
+        return fail(SGS_ERROR_INVALID_ARGUMENT, "view_table_device must be 16-byte aligned");
+
+    hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
+    char* ws = static_cast<char*>(workspace);
+    const char* table = static_cast<const char*>(view_table_device);
+    hipError_t e;
+    if (!table) {
+        std::string host(table_bytes(n_views), '\0');
+        encode_views(views, n_views, host.data());
+        e = hipMemcpyAsync(ws + L.table, host.data(), host.size(), hipMemcpyHostToDevice, stream);
+        if (e != hipSuccess) return hip_fail(e, "view table upload");
+        table = ws + L.table;
+    }
+    const ViewDesc* desc = reinterpret_cast<const ViewDesc*>(table);
+    const uint32_t* prefix = reinterpret_cast<const uint32_t*>(table + table_views_bytes(n_views));
+    BatchCounters* counters = reinterpret_cast<BatchCounters*>(ws + L.counters);
+    uint32_t* ready = reinterpret_cast<uint32_t*>(ws + L.ready);
+    BigEntry* queue = reinterpret_cast<BigEntry*>(ws + L.queue);
+    uint64_t* fb = reinterpret_cast<uint64_t*>(ws + L.fb);
+    float4* samples = reinterpret_cast<float4*>(ws + L.samples);
+
+    auto mark = [&](int i) -> hipError_t {
+        return stage_events ? hipEventRecord(static_cast<hipEvent_t>(stage_events[i]), stream) : hipSuccess;
+    };
+    if ((e = mark(0)) != hipSuccess) return hip_fail(e, "stage event");
+    e = launch_zero_u32(reinterpret_cast<uint32_t*>(counters), (L.ready + 4u * (size_t)t.queue_capacity - L.counters) / 4u,
+                        stream);
+    if (e != hipSuccess) return hip_fail(e, "queue clear");
+    e = launch_fill_u64(fb, L.fb_words, kEmpty, stream);
+    if (e != hipSuccess) return hip_fail(e, "framebuffer clear");
+    const SceneArgs sc = scene_args(gaussians);
+    if ((e = mark(1)) != hipSuccess) return hip_fail(e, "stage event");
+    e = launch_batch_accumulate(sc, desc, n_views, fb, counters, queue, ready, t, stream);
+    if (e != hipSuccess) return hip_fail(e, "batched accumulation");
+    if ((e = mark(2)) != hipSuccess) return hip_fail(e, "stage event");
+    // sample 0 of REPROJECT views needs the full sample buffer (3x3 neighbourhood moments)
+    uint64_t fb_off = 0, smp_off = 0;
+    for (uint32_t i = 0; i < n_views; i++) {
+        const sgs_view& v = views[i];
+        const uint64_t P = view_fb_pixels(v);
+        if (v.temporal_mode == SGS_TEMPORAL_REPROJECT) {
+            const FbCamera cam = make_fb_camera(v.width, v.height, v.focal_x, v.focal_y, v.center_x, v.center_y,
+                                                v.near_plane, v.far_plane, v.proper_antialiasing,
+                                                v.supersampling_factor);
+            const char* d = table + (size_t)i * sizeof(ViewDesc);
+            e = launch_sample(sc, reinterpret_cast<const float*>(d + offsetof(ViewDesc, r1)),
+                              reinterpret_cast<const float*>(d + offsetof(ViewDesc, cam_x)), cam, fb + fb_off,
+                              samples + smp_off, nullptr, nullptr, nullptr, false, stream);
+            if (e != hipSuccess) return hip_fail(e, "batched sample");
+            smp_off += P;
+        }
+        fb_off += P * v.spp;
+    }
+    e = launch_batch_resolve(sc, desc, n_views, prefix, L.n_blocks, fb, samples, stream);
+    if (e != hipSuccess) return hip_fail(e, "batched resolve");
+    if ((e = mark(3)) != hipSuccess) return hip_fail(e, "stage event");
+    return SGS_SUCCESS;
+}
+
+}  // extern "C"

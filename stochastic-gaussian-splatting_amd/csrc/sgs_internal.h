@@ -72,6 +72,65 @@ struct BinArgs {
     uint32_t threshold;
 };
 
+struct HistoryPtrs {
+    const float* image;
+    const float* depth;
+    const float* alpha;
+    const float* weight_mass;
+};
+struct OutputPtrs {
+    float* image;
+    float* depth;
+    float* alpha;
+    float* weight_mass;
+};
+
+// One view of a batched render as the kernels read it (device table, 16-byte aligned).
+struct alignas(16) ViewDesc {
+    float4 r1, r2, r3;          // world-to-view rows
+    FbCamera cam;               // framebuffer (subpixel) camera
+    float cam_x, cam_y, cam_z;  // camera position
+    float focal_x, focal_y, center_x, center_y;  // image pixels
+    uint32_t width, height;     // image pixels
+    uint32_t spp;               // samples rendered for this view
+    uint32_t frame0;            // RNG frame of sample 0; sample s uses frame0 + s
+    int32_t mode;               // temporal mode of sample 0; later samples are STATIC
+    uint32_t block_begin;       // first resolve block of this view
+    uint64_t fb_offset;         // first framebuffer word of sample 0 (samples are P words apart)
+    uint64_t samples_offset;    // REPROJECT views: first float4 of the sample-0 buffer
+    HistoryPtrs hist;
+    OutputPtrs out;
+    float v2p[16];
+};
+
+// Work-queue state of the persistent batched accumulation.
+struct BatchCounters {
+    uint32_t next_item;         // phase A: next (Gaussian chunk, view) item
+    uint32_t done_items;        // phase A items finished (release after their queue writes)
+    uint32_t produced;          // queue slots reserved
+    uint32_t consumed;          // queue slots claimed by phase B waves
+    uint32_t pad[60];
+};
+
+// A fragment range of a large Gaussian in one view, deferred to phase B (64 bytes).
+struct alignas(16) BigEntry {
+    uint64_t fragment;
+    float mean_x, mean_y;       // mean2d - 0.5
+    float ce_a, ce_b, ce_c, opacity;
+    uint32_t x0, y0, w, view;   // AABB origin and width, view index
+    uint32_t f0, f1, pad0, pad1;  // fragment range [f0, f1) in row-major AABB order
+};
+static_assert(sizeof(BigEntry) == 64, "BigEntry layout");
+
+struct BatchTuning {
+    uint32_t sequential_threshold;
+    uint32_t big_threshold;     // fragments above which a Gaussian goes to the queue
+    uint32_t entry_fragments;   // fragments per queue entry
+    uint32_t queue_capacity;
+    uint32_t view_major;        // 0: chunk-major items (Gaussian parameters reused across views)
+    uint32_t blocks_per_cu;     // persistent grid = CUs x blocks_per_cu (0: occupancy limit)
+};
+
 // Workspace carve-up. Every sub-buffer is 256-byte aligned.
 struct Arena {
     size_t offset = 0;
@@ -97,6 +156,11 @@ hipError_t launch_gps_scatter(const SceneArgs& sc, const float* w2c, const FbCam
                               uint32_t max_points, uint32_t seq_threshold, const BinArgs& bin, uint64_t* counters,
                               hipStream_t stream);
 
+// ---- launchers (sgs_batch.hip) ----
+hipError_t launch_batch_accumulate(const SceneArgs& sc, const ViewDesc* views, uint32_t n_views, uint64_t* fb,
+                                   BatchCounters* counters, BigEntry* queue, uint32_t* ready, const BatchTuning& t,
+                                   hipStream_t stream);
+
 // ---- launchers (sgs_hybrid.hip) ----
 size_t hybrid_st_scratch_bytes(uint32_t fb_width, uint32_t fb_height, const Tuning& t);
 size_t hybrid_gps_scratch_bytes(const Tuning& t);
@@ -108,18 +172,9 @@ hipError_t run_hybrid_gps(const SceneArgs& sc, const float* w2c, const FbCamera&
                           hipStream_t stream);
 
 // ---- launchers (sgs_resolve.hip) ----
-struct HistoryPtrs {
-    const float* image;
-    const float* depth;
-    const float* alpha;
-    const float* weight_mass;
-};
-struct OutputPtrs {
-    float* image;
-    float* depth;
-    float* alpha;
-    float* weight_mass;
-};
+hipError_t launch_batch_resolve(const SceneArgs& sc, const ViewDesc* views, uint32_t n_views,
+                                const uint32_t* block_prefix, uint32_t n_blocks, const uint64_t* fb,
+                                const float4* samples, hipStream_t stream);
 hipError_t launch_sample(const SceneArgs& sc, const float* w2c, const float* cam_position, const FbCamera& cam,
                          const uint64_t* fb, float4* samples, float4* reuse_weights, uint2* k_prim, float4* k_occ,
                          bool spatial_reuse, hipStream_t stream);

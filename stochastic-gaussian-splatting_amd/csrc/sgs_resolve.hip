@@ -260,16 +260,14 @@ __device__ __forceinline__ TemporalOut temporal_static(const SpatialSums& s, con
     return o;
 }
 
+// one output pixel from the sample buffer: box filter (+ spatial reuse) and temporal mode
 template <bool kReuse>
-__global__ __launch_bounds__(kBlockResolve) void resolve_kernel(FbCamera cam, uint32_t width, uint32_t height,
-                                                                float focal_x, float focal_y, float center_x,
-                                                                float center_y, const float4* __restrict__ samples,
-                                                                const float4* __restrict__ rw, HistoryPtrs hist,
-                                                                const float* __restrict__ m, OutputPtrs out,
-                                                                int32_t mode)
+__device__ __forceinline__ TemporalOut resolve_from_samples(const FbCamera& cam, uint32_t width, uint32_t height,
+                                                            float focal_x, float focal_y, float center_x,
+                                                            float center_y, const float4* __restrict__ samples,
+                                                            const float4* __restrict__ rw, const HistoryPtrs& hist,
+                                                            const float* __restrict__ m, int32_t mode, uint32_t pix)
 {
-    const uint32_t pix = blockIdx.x * kBlockResolve + threadIdx.x;
-    if (pix >= width * height) return;
     const uint32_t ss = (uint32_t)cam.ss;
     const uint32_t bx = (pix % width) * ss, by = (pix / width) * ss;
     const uint32_t FW = cam.width_i, FH = cam.height_i;
@@ -402,12 +400,31 @@ __global__ __launch_bounds__(kBlockResolve) void resolve_kernel(FbCamera cam, ui
             }
         }
     }
+    return o;
+}
+
+__device__ __forceinline__ void store_output(const OutputPtrs& out, uint32_t pix, const TemporalOut& o)
+{
     out.image[3u * pix + 0u] = o.c0;
     out.image[3u * pix + 1u] = o.c1;
     out.image[3u * pix + 2u] = o.c2;
     out.depth[pix] = o.d;
     out.alpha[pix] = o.a;
     out.weight_mass[pix] = o.w;
+}
+
+template <bool kReuse>
+__global__ __launch_bounds__(kBlockResolve) void resolve_kernel(FbCamera cam, uint32_t width, uint32_t height,
+                                                                float focal_x, float focal_y, float center_x,
+                                                                float center_y, const float4* __restrict__ samples,
+                                                                const float4* __restrict__ rw, HistoryPtrs hist,
+                                                                const float* __restrict__ m, OutputPtrs out,
+                                                                int32_t mode)
+{
+    const uint32_t pix = blockIdx.x * kBlockResolve + threadIdx.x;
+    if (pix >= width * height) return;
+    store_output(out, pix, resolve_from_samples<kReuse>(cam, width, height, focal_x, focal_y, center_x, center_y,
+                                                        samples, rw, hist, m, mode, pix));
 }
 
 hipError_t launch_resolve(const FbCamera& cam, uint32_t width, uint32_t height, float focal_x, float focal_y,
@@ -432,22 +449,20 @@ hipError_t launch_resolve(const FbCamera& cam, uint32_t width, uint32_t height, 
 // output pixel shades its ss x ss winners directly, so the (P,4) sample buffer is never
 // written. Same operations in the same order as sample_kernel + resolve_kernel.
 // ------------------------------------------------------------------------------------------
+// box filter of one output pixel shading its ss x ss framebuffer winners directly
+// (identical operations to sample_kernel followed by resolve_from_samples without reuse)
 template <uint32_t B>
-__global__ __launch_bounds__(kBlockResolve) void shade_resolve_kernel(SceneArgs sc, const float* __restrict__ cam_position,
-                                                                      FbCamera cam, uint32_t width, uint32_t height,
-                                                                      const uint64_t* __restrict__ fb, HistoryPtrs hist,
-                                                                      OutputPtrs out, int32_t mode)
+__device__ __forceinline__ SpatialSums shade_box(const SceneArgs& sc, float cpx, float cpy, float cpz,
+                                                 const FbCamera& cam, uint32_t width, const uint64_t* __restrict__ fb,
+                                                 uint32_t pix)
 {
-    const uint32_t pix = blockIdx.x * kBlockResolve + threadIdx.x;
-    if (pix >= width * height) return;
     const uint32_t ss = (uint32_t)cam.ss;
     const uint32_t bx = (pix % width) * ss, by = (pix / width) * ss;
     const float nf = 1.0f / (float)(ss * ss);
-    const float cpx = cam_position[0], cpy = cam_position[1], cpz = cam_position[2];
     SpatialSums s = { 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0u };
     for (uint32_t oy = 0; oy < ss; oy++) {
         for (uint32_t ox = 0; ox < ss; ox++) {
-            const uint64_t data = fb[(by + oy) * cam.width_i + bx + ox];
+            const uint64_t data = fb[(size_t)(by + oy) * cam.width_i + bx + ox];
             const uint32_t dk = (uint32_t)(data >> 32);
             float4 sm = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             if (dk != 0xffffffffu) {
@@ -463,14 +478,21 @@ __global__ __launch_bounds__(kBlockResolve) void shade_resolve_kernel(SceneArgs 
     s.d = s.d * nf;
     s.a = s.a * nf;
     s.w = s.w * nf;
+    return s;
+}
+
+template <uint32_t B>
+__global__ __launch_bounds__(kBlockResolve) void shade_resolve_kernel(SceneArgs sc, const float* __restrict__ cam_position,
+                                                                      FbCamera cam, uint32_t width, uint32_t height,
+                                                                      const uint64_t* __restrict__ fb, HistoryPtrs hist,
+                                                                      OutputPtrs out, int32_t mode)
+{
+    const uint32_t pix = blockIdx.x * kBlockResolve + threadIdx.x;
+    if (pix >= width * height) return;
+    const SpatialSums s = shade_box<B>(sc, cam_position[0], cam_position[1], cam_position[2], cam, width, fb, pix);
     TemporalOut o = { s.c0, s.c1, s.c2, s.d, s.a, s.w };
     if (mode == SGS_TEMPORAL_STATIC) o = temporal_static(s, hist, pix);
-    out.image[3u * pix + 0u] = o.c0;
-    out.image[3u * pix + 1u] = o.c1;
-    out.image[3u * pix + 2u] = o.c2;
-    out.depth[pix] = o.d;
-    out.alpha[pix] = o.a;
-    out.weight_mass[pix] = o.w;
+    store_output(out, pix, o);
 }
 
 hipError_t launch_shade_resolve(const SceneArgs& sc, const float* cam_position, const FbCamera& cam, uint32_t width,
@@ -485,6 +507,104 @@ hipError_t launch_shade_resolve(const SceneArgs& sc, const float* cam_position, 
         case 4: hipLaunchKernelGGL((shade_resolve_kernel<4>), dim3(grid), dim3(kBlockResolve), 0, stream, sc, cam_position, cam, width, height, fb, hist, out, temporal_mode); break;
         case 9: hipLaunchKernelGGL((shade_resolve_kernel<9>), dim3(grid), dim3(kBlockResolve), 0, stream, sc, cam_position, cam, width, height, fb, hist, out, temporal_mode); break;
         default: hipLaunchKernelGGL((shade_resolve_kernel<16>), dim3(grid), dim3(kBlockResolve), 0, stream, sc, cam_position, cam, width, height, fb, hist, out, temporal_mode); break;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace sgs
+
+// ------------------------------------------------------------------------------------------
+// batched resolve: every output pixel of every view folds its view's spp samples in order,
+// sample 0 with the view's temporal mode and the rest STATIC, the operations of spp
+// successive single-sample calls (the history of sample s is the output of sample s - 1).
+// A pixel's winners often repeat across samples, so the last shaded winner is reused.
+// ------------------------------------------------------------------------------------------
+namespace sgs {
+
+template <uint32_t B>
+__global__ __launch_bounds__(kBlockResolve) void batch_resolve_kernel(SceneArgs sc, const ViewDesc* __restrict__ views,
+                                                                      uint32_t n_views,
+                                                                      const uint32_t* __restrict__ block_prefix,
+                                                                      const uint64_t* __restrict__ fb,
+                                                                      const float4* __restrict__ samples)
+{
+    // view of this block: last v with block_prefix[v] <= blockIdx.x
+    uint32_t lo = 0, hi = n_views - 1u;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1u) >> 1;
+        if (block_prefix[mid] <= blockIdx.x) lo = mid;
+        else hi = mid - 1u;
+    }
+    const ViewDesc& vd = views[lo];
+    const uint32_t pix = (blockIdx.x - block_prefix[lo]) * kBlockResolve + threadIdx.x;
+    if (pix >= vd.width * vd.height) return;
+    const size_t P = (size_t)vd.cam.width_i * vd.cam.height_i;
+    const uint64_t* fbv = fb + vd.fb_offset;
+    TemporalOut o;
+    uint32_t s0 = 0;
+    if (vd.mode == SGS_TEMPORAL_REPROJECT) {
+        o = resolve_from_samples<false>(vd.cam, vd.width, vd.height, vd.focal_x, vd.focal_y, vd.center_x, vd.center_y,
+                                        samples + vd.samples_offset, nullptr, vd.hist, vd.v2p, vd.mode, pix);
+        s0 = 1;
+    }
+    const uint32_t ss = (uint32_t)vd.cam.ss;
+    const uint32_t bx = (pix % vd.width) * ss, by = (pix / vd.width) * ss;
+    const float nf = 1.0f / (float)(ss * ss);
+    uint32_t last_g = 0xffffffffu;
+    float3 last_col = make_float3(0.0f, 0.0f, 0.0f);
+    for (uint32_t s = s0; s < vd.spp; s++) {
+        SpatialSums sp = { 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0u };
+        for (uint32_t oy = 0; oy < ss; oy++) {
+            for (uint32_t ox = 0; ox < ss; ox++) {
+                const uint64_t data = fbv[s * P + (size_t)(by + oy) * vd.cam.width_i + bx + ox];
+                const uint32_t dk = (uint32_t)(data >> 32);
+                float4 sm = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                if (dk != 0xffffffffu) {
+                    const uint32_t g = (uint32_t)data;
+                    if (g != last_g) {
+                        last_col = shade<B>(sc, vd.cam_x, vd.cam_y, vd.cam_z, g);
+                        last_g = g;
+                    }
+                    sm = make_float4(last_col.x, last_col.y, last_col.z, u2f(dk));
+                }
+                add_subpixel(sp, sm, sm.x, sm.y, sm.z, sm.w, sm.w > 0.0f ? 1.0f : 0.0f, 1.0f);
+            }
+        }
+        sp.c0 = sp.c0 * nf;
+        sp.c1 = sp.c1 * nf;
+        sp.c2 = sp.c2 * nf;
+        sp.d = sp.d * nf;
+        sp.a = sp.a * nf;
+        sp.w = sp.w * nf;
+        if (s == 0) {
+            o = { sp.c0, sp.c1, sp.c2, sp.d, sp.a, sp.w };
+            if (vd.mode == SGS_TEMPORAL_STATIC) o = temporal_static(sp, vd.hist, pix);
+        } else {
+            // STATIC against the previous sample's output (resolve.cu:250-258)
+            const float hm = o.w;
+            const float w = hm + sp.w;
+            const float hf = hm / w;
+            o.c0 = lerpf(sp.c0, o.c0, hf);
+            o.c1 = lerpf(sp.c1, o.c1, hf);
+            o.c2 = lerpf(sp.c2, o.c2, hf);
+            o.d = lerpf(sp.d, o.d, hf);
+            o.a = lerpf(sp.a, o.a, hf);
+            o.w = w;
+        }
+    }
+    store_output(vd.out, pix, o);
+}
+
+hipError_t launch_batch_resolve(const SceneArgs& sc, const ViewDesc* views, uint32_t n_views,
+                                const uint32_t* block_prefix, uint32_t n_blocks, const uint64_t* fb,
+                                const float4* samples, hipStream_t stream)
+{
+    if (n_blocks == 0 || n_views == 0) return hipSuccess;
+    switch (sc.sh_bases) {
+        case 1: hipLaunchKernelGGL((batch_resolve_kernel<1>), dim3(n_blocks), dim3(kBlockResolve), 0, stream, sc, views, n_views, block_prefix, fb, samples); break;
+        case 4: hipLaunchKernelGGL((batch_resolve_kernel<4>), dim3(n_blocks), dim3(kBlockResolve), 0, stream, sc, views, n_views, block_prefix, fb, samples); break;
+        case 9: hipLaunchKernelGGL((batch_resolve_kernel<9>), dim3(n_blocks), dim3(kBlockResolve), 0, stream, sc, views, n_views, block_prefix, fb, samples); break;
+        default: hipLaunchKernelGGL((batch_resolve_kernel<16>), dim3(n_blocks), dim3(kBlockResolve), 0, stream, sc, views, n_views, block_prefix, fb, samples); break;
     }
     return hipGetLastError();
 }

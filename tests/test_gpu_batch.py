@@ -1,0 +1,165 @@
+"""GPU parity of the batched persistent renderer: every view equals spp successive
+single-view render() calls (RESET/STATIC/REPROJECT first sample, then STATIC) and the
+oracle's SPP loop; queue and overflow schedules and graph replays give the same bits."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import oracle_scene, small_camera, small_scene
+from test_gpu_parity import print_case, to_device
+
+pytestmark = pytest.mark.gpu
+RESOLVE_ATOL = 1e-5
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no ROCm GPU visible")
+    from StochasticGSCudaBackend import _native
+    return _native
+
+
+def settings_of(cam):
+    import torch
+    from StochasticGSCudaBackend import Settings
+    return Settings(torch.from_numpy(np.asarray(cam["w2c"], np.float32)),
+                    torch.from_numpy(np.asarray(cam["cam_position"], np.float32)), cam["width"], cam["height"],
+                    cam["focal_x"], cam["focal_y"], cam["center_x"], cam["center_y"], cam["near_plane"],
+                    cam["far_plane"], cam.get("proper_antialiasing", False), cam.get("supersampling_factor", 1),
+                    False, False, False)
+
+
+def plain_spp(gpu, t, cam, frame0, spp, mode=0, history=None, v2p=None):
+    """spp render() calls of the single-view path, like the reference's SPP loop."""
+    import torch
+    s = settings_of(cam)
+    eye = torch.from_numpy(np.asarray(np.eye(4) if v2p is None else v2p, np.float32)).cuda()
+    hist = tuple(torch.empty(0) for _ in range(4)) if history is None else history
+    kb = (torch.empty((0, 0, 2), dtype=torch.int32, device="cuda"), torch.empty((0, 0, 4), device="cuda"))
+    for k in range(spp):
+        hist = gpu.render(t["means"], t["scales"], t["rotations"], t["opacities"], t["sh"],
+                          s.w2c.cuda(), s.cam_position.cuda(), *s.as_tuple()[2:], torch.empty(0, dtype=torch.uint8,
+                                                                                              device="cuda"),
+                          kb[0], kb[1], *hist, eye, mode if k == 0 else 1, frame_idx=frame0 + k)
+    return hist
+
+
+def assert_close(got, ref, exact=True):
+    for g, r, name in zip(got, ref, ("image", "depth", "alpha", "weight_mass")):
+        g = g.cpu().numpy() if hasattr(g, "cpu") else g
+        r = r.cpu().numpy() if hasattr(r, "cpu") else r
+        err = float(np.abs(g.astype(np.float64) - r.astype(np.float64)).max())
+        assert err <= (0.0 if exact else RESOLVE_ATOL), f"{name}: max abs error {err}"
+
+
+def make_views():
+    cams = [
+        small_camera(64, 48, angle=0.0),
+        small_camera(100, 80, angle=0.7, supersampling_factor=2),
+        small_camera(37, 29, angle=-1.2, proper_antialiasing=True),
+        small_camera(128, 96, angle=2.0, distance=2.2),
+        small_camera(50, 50, angle=3.0),
+    ]
+    spps = [1, 3, 5, 2, 4]
+    frames = [0, 100, 2000, 77777, 5]
+    return cams, spps, frames
+
+
+@pytest.mark.parametrize("tuning", [
+    dict(),                                                   # defaults
+    dict(big_threshold=64, entry_fragments=64),               # most Gaussians through the work queue
+    dict(big_threshold=64, entry_fragments=64, queue_capacity=3),  # queue overflow: scattered in place
+    dict(view_major=1, sequential_threshold=1, blocks_per_cu=1),
+])
+def test_batch_equals_plain_spp_loop(gpu, tuning):
+    import torch
+    from StochasticGSCudaBackend.batch import BatchRenderer, BatchTuning, BatchView
+    arrays = small_scene(3000, seed=41, log_scale_mean=-3.3, log_scale_std=0.8)
+    t = to_device(arrays)
+    cams, spps, frames = make_views()
+    views = [BatchView(settings_of(c), spp=n, frame_idx=f) for c, n, f in zip(cams, spps, frames)]
+    print_case(test="batch_vs_plain", **tuning)
+    r = BatchRenderer(t["means"], t["scales"], t["rotations"], t["opacities"], t["sh"], views, BatchTuning(**tuning))
+    outs = r.render()
+    torch.cuda.synchronize()
+    for v, (c, n, f) in enumerate(zip(cams, spps, frames)):
+        ref = plain_spp(gpu, t, c, f, n)
+        torch.cuda.synchronize()
+        assert_close(outs[v], ref)
+
+
+def test_batch_temporal_modes(gpu):
+    """STATIC and REPROJECT first samples against a given history."""
+    import torch
+    from StochasticGSCudaBackend import History, TemporalMode
+    from StochasticGSCudaBackend.batch import BatchView, render_batch
+    arrays = small_scene(2500, seed=42, log_scale_mean=-3.4)
+    t = to_device(arrays)
+    cam0 = small_camera(80, 60, angle=0.3)
+    hist = plain_spp(gpu, t, cam0, 10, 3)
+    cam1 = small_camera(80, 60, angle=0.32)
+    v2p = (np.asarray(cam0["w2c"], np.float64) @ np.linalg.inv(np.asarray(cam1["w2c"], np.float64))).astype(np.float32)
+    views = [BatchView(settings_of(cam0), spp=2, frame_idx=20, temporal_mode=TemporalMode.STATIC, history=History(*hist)),
+             BatchView(settings_of(cam1), spp=3, frame_idx=30, temporal_mode=TemporalMode.REPROJECT,
+                       history=History(*hist), view_to_previous_view=v2p)]
+    print_case(test="batch_temporal")
+    outs = render_batch(t["means"], t["scales"], t["rotations"], t["opacities"], t["sh"], views)
+    torch.cuda.synchronize()
+    assert_close(outs[0], plain_spp(gpu, t, cam0, 20, 2, mode=1, history=hist))
+    assert_close(outs[1], plain_spp(gpu, t, cam1, 30, 3, mode=2, history=hist, v2p=v2p))
+
+
+def test_batch_matches_oracle(gpu, oracle_mod):
+    import torch
+    from StochasticGSCudaBackend.batch import BatchView, render_batch
+    arrays = small_scene(1500, seed=43, log_scale_mean=-3.4)
+    sc, t = oracle_scene(arrays), to_device(arrays)
+    cams = [small_camera(48, 40, angle=0.1), small_camera(40, 30, angle=1.0, supersampling_factor=2)]
+    views = [BatchView(settings_of(c), spp=3, frame_idx=300 + 10 * i) for i, c in enumerate(cams)]
+    outs = render_batch(t["means"], t["scales"], t["rotations"], t["opacities"], t["sh"], views)
+    torch.cuda.synchronize()
+    for i, c in enumerate(cams):
+        hist = None
+        for k in range(3):
+            ref = oracle_mod.render(sc, c, 300 + 10 * i + k, history=hist, temporal_mode=0 if k == 0 else 1)
+            hist = ref[:4]
+        assert_close(outs[i], hist, exact=False)
+
+
+def test_batch_graph_replay(gpu):
+    """A captured batch replays with new frames identically to eager renders, with eager
+    work and synchronisation between replays."""
+    import torch
+    from StochasticGSCudaBackend.batch import BatchRenderer, BatchView
+    arrays = small_scene(2000, seed=44, log_scale_mean=-3.4)
+    t = to_device(arrays)
+    cams, spps, frames = make_views()
+    views = [BatchView(settings_of(c), spp=n, frame_idx=f) for c, n, f in zip(cams, spps, frames)]
+    graphed = BatchRenderer(t["means"], t["scales"], t["rotations"], t["opacities"], t["sh"], views)
+    eager = BatchRenderer(t["means"], t["scales"], t["rotations"], t["opacities"], t["sh"], views)
+    graph = graphed.capture()
+    for step in range(3):
+        new = [f + 1000 * (step + 1) for f in frames]
+        graphed.set_frames(new)
+        graphed.upload_table()
+        graph.replay()
+        eager.set_frames(new)
+        eager.upload_table()
+        eager.render()
+        torch.cuda.synchronize()
+        _ = (t["means"] * 2.0).sum().item()  # unrelated eager work between replays
+        print_case(test="batch_graph", step=step)
+        for a, b in zip(graphed.outputs, eager.outputs):
+            assert_close(a, b)
+
+
+def test_batch_rejects_unsupported_views(gpu):
+    from StochasticGSCudaBackend.batch import BatchView, render_batch
+    arrays = small_scene(10)
+    t = to_device(arrays)
+    s = settings_of(small_camera(16, 16))._replace(use_gps=True)
+    with pytest.raises(ValueError):
+        render_batch(t["means"], t["scales"], t["rotations"], t["opacities"], t["sh"], [BatchView(s)])
