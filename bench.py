@@ -42,6 +42,10 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["plain", "batched", "all"], default="all",
                     help="run only one path (profiling); the JSON line then reports that path")
+    ap.add_argument("--schedule", choices=["tiles", "scatter"], default="tiles",
+                    help="batched accumulation: LDS tiles (default) or persistent scatter")
+    ap.add_argument("--tile-samples", type=int, choices=[8, 16], default=16,
+                    help="samples per LDS pass of the tile rasteriser")
     ap.add_argument("--copy-probe", action="store_true",
                     help="copy 1 GiB once first, to calibrate HBM byte counters against a known size")
     ap.add_argument("--no-morton", action="store_true",
@@ -165,6 +169,30 @@ def roofline_scatter(scene_n: int, views, ms, work):
     }
 
 
+def roofline_tiles(views, raster_ms: list, n_instances: int):
+    """Algorithmic bytes of one tile-rasteriser launch: per (view, tile, Gaussian) instance
+    4 B of its index and 44 B of Gaussian parameters (means, scales, rotation, opacity: the
+    instance is re-projected in its tile's workgroup), and 8 B per framebuffer word written
+    (every sample of every pixel, written once)."""
+    words = sum(cam["width"] * cam["height"] * cam["spp"] for cam in views)
+    b = 48.0 * n_instances + 8.0 * words
+    t = sum(raster_ms) / len(raster_ms) * 1e-3
+    achieved = b / t / 1e9
+    return {
+        "kernel": "sgs::batch_tile_raster_kernel",
+        "bound": "hbm",
+        "achieved": round(achieved, 2),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "avg_launch_ms": round(t * 1e3, 3),
+        "algorithmic_bytes_per_launch": int(b),
+        "instances_per_launch": int(n_instances),
+        "framebuffer_words_per_launch": int(words),
+    }
+
+
 def roofline_batch(scene_n: int, views, acc_ms: list, work):
     """Algorithmic bytes of one persistent accumulation launch: per view 44 B of parameters
     for every Gaussian (12 B means for the depth cull + 32 B scales/rotation/opacity) and 8 B
@@ -261,8 +289,12 @@ def main() -> int:
     log(f"plain scatter launches: avg {roof_plain['avg_launch_ms']} ms, {roof_plain['achieved']} GB/s algorithmic")
     roof = roof_plain
     if "batched" in result:
-        roof = roofline_batch(spec.n, views, acc_ms, work)
-        log(f"batched accumulate: {roof['avg_launch_ms']} ms, {roof['achieved']} GB/s algorithmic")
+        if args.schedule == "tiles":
+            roof = roofline_tiles(views, acc_ms, result["batched"]["tile_instances"])
+            log(f"tile raster: {roof['avg_launch_ms']} ms, {roof['achieved']} GB/s algorithmic")
+        else:
+            roof = roofline_batch(spec.n, views, acc_ms, work)
+            log(f"batched accumulate: {roof['avg_launch_ms']} ms, {roof['achieved']} GB/s algorithmic")
 
     cpu = None
     if not args.no_cpu:
@@ -285,7 +317,7 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": f"{spec.n} Gaussians (8-component mixture, SH3, Morton order), {len(views)} views "
                                f"on a ring, resolutions 256x256..2048x1152, spp in {{1,4,16}}, ST scatter, ss=1",
-                   "path": ("batched persistent launch (all views and samples)" if "batched" in result
+                   "path": (f"batched render of all views and samples, {args.schedule} schedule" if "batched" in result
                             else "plain per-view render() calls"),
                    "pixel_samples_per_step": S},
         "batched": result.get("batched"),
@@ -336,9 +368,13 @@ def bench_batched(args, scene, views, S, plain_out0, dev):
     library's stream; graph replay; bit comparison with the plain outputs of the same frames."""
     import numpy as np
     import torch
-    from StochasticGSCudaBackend.batch import BatchRenderer, BatchView
+    from StochasticGSCudaBackend.batch import SCHEDULE_SCATTER, SCHEDULE_TILES, BatchRenderer, BatchTuning, BatchView
     bviews = [BatchView(host_settings(cam), spp=cam["spp"], frame_idx=0) for cam in views]
-    br = BatchRenderer(scene["means"], scene["scales"], scene["rotations"], scene["opacities"], scene["sh"], bviews)
+    tuning = BatchTuning(schedule=SCHEDULE_TILES if args.schedule == "tiles" else SCHEDULE_SCATTER,
+                         tile_samples=args.tile_samples)
+    br = BatchRenderer(scene["means"], scene["scales"], scene["rotations"], scene["opacities"], scene["sh"], bviews,
+                       tuning)
+    log(f"batched workspace {br.workspace.numel() / 2**30:.2f} GiB")
     max_diff = None
     if plain_out0 is not None:
         br.set_frames(frames_for_step(views, 0))
@@ -356,7 +392,7 @@ def bench_batched(args, scene, views, S, plain_out0, dev):
         br.upload_table()
         br.render()
     torch.cuda.synchronize()
-    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(5)] for _ in range(args.steps)]
     for e4 in ev:
         for e in e4:
             e.record(stream)
@@ -370,10 +406,14 @@ def bench_batched(args, scene, views, S, plain_out0, dev):
         torch.cuda.synchronize()
         log(f"batched step {k + 1}/{args.steps}: {(time.perf_counter() - ts) * 1e3:.1f} ms")
     batch_s = time.perf_counter() - t_all
-    acc_ms = [e4[1].elapsed_time(e4[2]) for e4 in ev]
-    stage = {"clear_ms": round(float(np.mean([e4[0].elapsed_time(e4[1]) for e4 in ev])), 3),
-             "accumulate_ms": round(float(np.mean(acc_ms)), 3),
-             "resolve_ms": round(float(np.mean([e4[2].elapsed_time(e4[3]) for e4 in ev])), 3)}
+    main_ms = [e5[2].elapsed_time(e5[3]) for e5 in ev]
+    stage = {"clear_ms": round(float(np.mean([e5[0].elapsed_time(e5[1]) for e5 in ev])), 3),
+             "binning_ms": round(float(np.mean([e5[1].elapsed_time(e5[2]) for e5 in ev])), 3),
+             "accumulate_kernel_ms": round(float(np.mean(main_ms)), 3),
+             "accumulate_total_ms": round(float(np.mean([e5[1].elapsed_time(e5[3]) for e5 in ev])), 3),
+             "resolve_ms": round(float(np.mean([e5[3].elapsed_time(e5[4]) for e5 in ev])), 3)}
+    n_inst, overflowed = br.tile_stats()
+    log(f"stages {stage}; tile instances {n_inst} (overflow {overflowed})")
     graph = br.capture()
     torch.cuda.synchronize()
     g_ms = []
@@ -386,7 +426,8 @@ def bench_batched(args, scene, views, S, plain_out0, dev):
         g_ms.append((time.perf_counter() - ts) * 1e3)
     return {"value": round(S * args.steps / batch_s / 1e6, 2), "ms_per_step": round(batch_s * 1e3 / args.steps, 2),
             "graph_ms_per_step": round(float(np.mean(g_ms)), 2), "stages": stage,
-            "max_abs_diff_vs_plain": max_diff}, acc_ms
+            "tile_instances": n_inst, "tile_overflow": overflowed,
+            "max_abs_diff_vs_plain": max_diff}, main_ms
 
 
 if __name__ == "__main__":

@@ -165,8 +165,8 @@ typedef struct sgs_view {
     sgs_history out;                  /* device outputs (H,W,*) */
 } sgs_view;
 
-/* Scheduling of the persistent batched accumulation; zero fields take defaults.
- * None of them changes a result bit. */
+/* Scheduling of the batched accumulation; zero fields take defaults. None of them changes
+ * a result bit. */
 typedef struct sgs_batch_tuning {
     uint32_t sequential_threshold;    /* fragments one lane handles alone (8) */
     uint32_t big_threshold;           /* fragments above which a Gaussian is split into
@@ -176,10 +176,24 @@ typedef struct sgs_batch_tuning {
     uint32_t view_major;              /* 1: items ordered view by view; 0: Gaussian chunk by
                                          chunk across all views (default) */
     uint32_t blocks_per_cu;           /* persistent workgroups per CU (0: occupancy limit) */
+    uint32_t schedule;                /* SGS_BATCH_TILES (default) or SGS_BATCH_SCATTER */
+    uint32_t max_tile_instances;      /* capacity of the tile schedule's (view, tile, Gaussian)
+                                         list (8 per Gaussian and view, within [2^16, 2^31]);
+                                         a batch that needs more runs the scatter schedule */
+    uint32_t tile_samples;            /* samples held in LDS per pass over a tile (8 or 16) */
 } sgs_batch_tuning;
 
-/* Device workspace needed by sgs_render_batch for these views. */
-size_t sgs_render_batch_workspace_size(const sgs_view* views, uint32_t n_views, const sgs_batch_tuning* tuning);
+/* SGS_BATCH_TILES: every (view, Gaussian) is binned into the 16x16 framebuffer tiles its
+ * bounding box overlaps; one workgroup per (view, tile) keeps the tile's words of up to 16
+ * samples in LDS and writes them out once. SGS_BATCH_SCATTER: one persistent launch over
+ * (Gaussian chunk, view) items with 64-bit atomicMin into the framebuffer in device memory
+ * and a work queue for large Gaussians. */
+#define SGS_BATCH_TILES 1u
+#define SGS_BATCH_SCATTER 2u
+
+/* Device workspace needed by sgs_render_batch for these views and n_gaussians Gaussians. */
+size_t sgs_render_batch_workspace_size(const sgs_view* views, uint32_t n_views, uint32_t n_gaussians,
+                                       const sgs_batch_tuning* tuning);
 
 /* Bytes of the encoded view table, and the encoding itself into host memory (for callers
  * that upload it once, e.g. before capturing the render into a graph). */
@@ -188,12 +202,20 @@ int sgs_batch_encode_views(const sgs_view* views, uint32_t n_views, void* host_t
 
 /* Renders all views. view_table_device: the encoded table already in device memory, or
  * NULL to let the library copy it into the workspace (a pageable host-to-device copy,
- * not capturable into a graph). stage_events: NULL, or 4 hipEvent_t (as void*) recorded
- * on the stream before the framebuffer clear, before and after the accumulation launch,
- * and after the resolve, for timing the stages. */
+ * not capturable into a graph). stage_events: NULL, or 5 hipEvent_t (as void*) recorded
+ * on the stream before the counter and framebuffer clears, before the accumulation
+ * (binning, for the tile schedule), before the accumulation's main kernel (the tile
+ * rasteriser or the persistent scatter), after the accumulation and after the resolve, for
+ * timing the stages. */
 int sgs_render_batch(const sgs_gaussians* gaussians, const sgs_view* views, uint32_t n_views,
                      const void* view_table_device, const sgs_batch_tuning* tuning,
                      void* workspace, size_t workspace_bytes, void* const* stage_events, void* stream);
+
+/* After sgs_render_batch on this workspace (waits for the stream): the number of
+ * (view, tile, Gaussian) instances the tile schedule binned, and whether they exceeded
+ * max_tile_instances (then the scatter schedule produced the result). Lets a caller size
+ * max_tile_instances from a previous frame. */
+int sgs_batch_tile_stats(const void* workspace, void* stream, uint64_t* n_instances, uint32_t* overflowed);
 
 #ifdef __cplusplus
 }

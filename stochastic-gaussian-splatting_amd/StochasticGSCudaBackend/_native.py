@@ -65,7 +65,8 @@ class sgs_view(C.Structure):
 class sgs_batch_tuning(C.Structure):
     _fields_ = [("sequential_threshold", C.c_uint32), ("big_threshold", C.c_uint32),
                 ("entry_fragments", C.c_uint32), ("queue_capacity", C.c_uint32), ("view_major", C.c_uint32),
-                ("blocks_per_cu", C.c_uint32)]
+                ("blocks_per_cu", C.c_uint32), ("schedule", C.c_uint32), ("max_tile_instances", C.c_uint32),
+                ("tile_samples", C.c_uint32)]
 
 
 _P = C.POINTER
@@ -86,7 +87,7 @@ _lib.sgs_accumulate.argtypes = [_P(sgs_gaussians), _P(sgs_settings), C.c_void_p,
                                 C.c_uint32, _P(sgs_tuning), C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]
 
 _lib.sgs_render_batch_workspace_size.restype = C.c_size_t
-_lib.sgs_render_batch_workspace_size.argtypes = [C.c_void_p, C.c_uint32, _P(sgs_batch_tuning)]
+_lib.sgs_render_batch_workspace_size.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, _P(sgs_batch_tuning)]
 _lib.sgs_batch_view_table_bytes.restype = C.c_size_t
 _lib.sgs_batch_view_table_bytes.argtypes = [C.c_uint32]
 _lib.sgs_batch_encode_views.restype = C.c_int
@@ -94,6 +95,9 @@ _lib.sgs_batch_encode_views.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p]
 _lib.sgs_render_batch.restype = C.c_int
 _lib.sgs_render_batch.argtypes = [_P(sgs_gaussians), C.c_void_p, C.c_uint32, C.c_void_p, _P(sgs_batch_tuning),
                                   C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p]
+
+_lib.sgs_batch_tile_stats.restype = C.c_int
+_lib.sgs_batch_tile_stats.argtypes = [C.c_void_p, C.c_void_p, _P(C.c_uint64), _P(C.c_uint32)]
 
 k_buffer_size: int = _lib.sgs_k_buffer_size()
 

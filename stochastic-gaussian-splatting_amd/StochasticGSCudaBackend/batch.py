@@ -20,6 +20,10 @@ from . import _native
 from .render import History, Settings, TemporalMode
 
 
+SCHEDULE_TILES = 1    # LDS framebuffer tiles, one workgroup per (view, 16x16 tile)
+SCHEDULE_SCATTER = 2  # persistent scatter with atomicMin into device memory
+
+
 class BatchTuning(NamedTuple):
     """sgs_batch_tuning; 0 keeps the library default. No field changes a result bit."""
     sequential_threshold: int = 0
@@ -28,6 +32,9 @@ class BatchTuning(NamedTuple):
     queue_capacity: int = 0
     view_major: int = 0
     blocks_per_cu: int = 0
+    schedule: int = 0
+    max_tile_instances: int = 0
+    tile_samples: int = 0
 
     def c(self) -> _native.sgs_batch_tuning:
         return _native.sgs_batch_tuning(*[int(v) for v in self])
@@ -99,7 +106,8 @@ class BatchRenderer:
                 cv.history = _native.sgs_history(*[h.data_ptr() for h in hist])
             cv.out = _native.sgs_history(*[o.data_ptr() for o in out])
         self.set_frames([v.frame_idx for v in self.views])
-        nbytes = _native._lib.sgs_render_batch_workspace_size(self.c_views, self.n_views, self._tp)
+        nbytes = _native._lib.sgs_render_batch_workspace_size(self.c_views, self.n_views,
+                                                               self.gaussians.n_gaussians, self._tp)
         self.workspace = _native._workspace(nbytes, self.device)
         tb = _native._lib.sgs_batch_view_table_bytes(self.n_views)
         self._table_host = torch.empty(tb, dtype=torch.uint8, pin_memory=True)
@@ -130,11 +138,14 @@ class BatchRenderer:
             self._copied.record()
 
     def render(self, stage_events: Sequence["torch.cuda.Event"] | None = None) -> list[History]:
-        """stage_events: 4 recorded-once events marked before the clear, before and after the
-        accumulation launch and after the resolve (for timing)."""
+        """stage_events: 5 recorded-once events marked before the clears, before the
+        accumulation, before its main kernel (tile rasteriser or persistent scatter), after
+        the accumulation and after the resolve (for timing)."""
         ev = None
         if stage_events is not None:
-            ev = (C.c_void_p * 4)(*[e.cuda_event for e in stage_events])
+            if len(stage_events) != 5:
+                raise ValueError("stage_events takes 5 events")
+            ev = (C.c_void_p * 5)(*[e.cuda_event for e in stage_events])
         with torch.cuda.device(self.device):
             stream = torch.cuda.current_stream(self.device).cuda_stream
             rc = _native._lib.sgs_render_batch(C.byref(self.gaussians), self.c_views, self.n_views,
@@ -142,6 +153,16 @@ class BatchRenderer:
                                                self.workspace.numel(), ev, stream)
         _native._check(rc, "sgs_render_batch")
         return self.outputs
+
+    def tile_stats(self) -> tuple[int, bool]:
+        """(instances binned by the last render's tile schedule, whether they overflowed
+        max_tile_instances); waits for the current stream."""
+        n, o = C.c_uint64(0), C.c_uint32(0)
+        with torch.cuda.device(self.device):
+            stream = torch.cuda.current_stream(self.device).cuda_stream
+            rc = _native._lib.sgs_batch_tile_stats(self.workspace.data_ptr(), stream, C.byref(n), C.byref(o))
+        _native._check(rc, "sgs_batch_tile_stats")
+        return int(n.value), bool(o.value)
 
     def capture(self) -> "torch.cuda.CUDAGraph":
         """Captures render() into a graph (HIP graph on ROCm); replay() re-renders with the

@@ -51,6 +51,21 @@ hipError_t launch_fill_u64(uint64_t* p, size_t n, uint64_t value, hipStream_t st
     return hipGetLastError();
 }
 
+__global__ void fill_u64_if_kernel(uint64_t* __restrict__ p, size_t n, uint64_t v, const uint32_t* __restrict__ flag)
+{
+    if (!*flag) return;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) p[k] = v;
+}
+
+hipError_t launch_fill_u64_if(uint64_t* p, size_t n, uint64_t value, const uint32_t* run_if_set, hipStream_t stream)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_u64_if_kernel, dim3(grid_for(n, kBlock, 8192u)), dim3(kBlock), 0, stream, p, n, value,
+                       run_if_set);
+    return hipGetLastError();
+}
+
 hipError_t launch_zero_u32(uint32_t* p, size_t n, hipStream_t stream)
 {
     if (n == 0) return hipSuccess;

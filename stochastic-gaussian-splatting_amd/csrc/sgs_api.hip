@@ -321,13 +321,26 @@ BatchTuning resolve_batch_tuning(const sgs_batch_tuning* t)
     r.queue_capacity = (t && t->queue_capacity) ? t->queue_capacity : (1u << 20);
     r.view_major = (t && t->view_major) ? 1u : 0u;
     r.blocks_per_cu = t ? t->blocks_per_cu : 0u;
+    r.schedule = (t && t->schedule == SGS_BATCH_SCATTER) ? 1u : 0u;
+    r.max_tile_instances = t ? t->max_tile_instances : 0u;
+    r.tile_samples = (t && t->tile_samples == 8u) ? 8u : 16u;
     return r;
+}
+
+// default capacity of the tile schedule's instance list: 8 per (Gaussian, view) pair
+uint32_t default_tile_instances(uint32_t n_gaussians, uint32_t n_views)
+{
+    const uint64_t want = 8ull * n_gaussians * n_views;
+    return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(want, 1ull << 16), 1ull << 31);
 }
 
 constexpr uint32_t kResolveBlock = 256;
 
+constexpr uint32_t kBatchTile = 16;
+
+// device view table: ViewDesc[n], then resolve block prefix[n+1] and tile prefix[n+1]
 size_t table_views_bytes(uint32_t n) { return ((size_t)n * sizeof(ViewDesc) + 255u) & ~size_t(255u); }
-size_t table_bytes(uint32_t n) { return table_views_bytes(n) + 4u * ((size_t)n + 1u); }
+size_t table_bytes(uint32_t n) { return table_views_bytes(n) + 8u * ((size_t)n + 1u); }
 
 uint32_t view_ss(const sgs_view& v) { return v.supersampling_factor < 1 ? 1u : (uint32_t)v.supersampling_factor; }
 uint64_t view_fb_pixels(const sgs_view& v)
@@ -335,31 +348,47 @@ uint64_t view_fb_pixels(const sgs_view& v)
     const uint64_t ss = view_ss(v);
     return (uint64_t)v.width * ss * v.height * ss;
 }
+uint32_t view_grid_w(const sgs_view& v) { return (uint32_t)((v.width * view_ss(v) + kBatchTile - 1u) / kBatchTile); }
+uint32_t view_grid_h(const sgs_view& v) { return (uint32_t)((v.height * view_ss(v) + kBatchTile - 1u) / kBatchTile); }
 
+// Workspace of a batch. The counters, the queue's ready flags and the tile counts and
+// cursors are contiguous so one fill clears them all.
 struct BatchLayout {
-    size_t counters, ready, queue, table, fb, samples, total;
-    uint64_t fb_words, n_samples;
-    uint32_t n_blocks;
+    size_t counters, ready, tile_count, tile_cursor, cleared_end, offsets, instances, queue, table, fb, samples, total;
+    uint64_t fb_words, n_samples, n_tiles;
+    uint32_t n_blocks, capacity;
 };
 
-BatchLayout batch_layout(const sgs_view* views, uint32_t n, const BatchTuning& t)
+BatchLayout batch_layout(const sgs_view* views, uint32_t n, uint32_t n_gaussians, const BatchTuning& t)
 {
     BatchLayout L;
     L.fb_words = 0;
     L.n_samples = 0;
+    L.n_tiles = 0;
     uint64_t blocks = 0;
     for (uint32_t v = 0; v < n; v++) {
         const uint64_t P = view_fb_pixels(views[v]);
         L.fb_words += P * views[v].spp;
         if (views[v].temporal_mode == SGS_TEMPORAL_REPROJECT) L.n_samples += P;
         blocks += ((uint64_t)views[v].width * views[v].height + kResolveBlock - 1u) / kResolveBlock;
+        L.n_tiles += (uint64_t)view_grid_w(views[v]) * view_grid_h(views[v]);
     }
     L.n_blocks = (uint32_t)blocks;
+    const bool tiles = t.schedule == 0u;
+    L.capacity = !tiles ? 0u : (t.max_tile_instances ? t.max_tile_instances : default_tile_instances(n_gaussians, n));
+    const size_t tiles_n = tiles ? (size_t)L.n_tiles : 0u;
     Arena a;
     L.counters = a.take(sizeof(BatchCounters));
-    L.ready = a.offset;  // contiguous with the counters: one zero-fill clears both
+    L.ready = a.offset;
     a.offset += 4u * (size_t)t.queue_capacity;
+    L.tile_count = a.offset;
+    a.offset += 4u * tiles_n;
+    L.tile_cursor = a.offset;
+    a.offset += 4u * tiles_n;
+    L.cleared_end = a.offset;
     a.offset = (a.offset + 255u) & ~size_t(255u);
+    L.offsets = a.take(4u * (tiles_n + 1u));
+    L.instances = a.take(4u * (size_t)L.capacity);
     L.queue = a.take(sizeof(BigEntry) * (size_t)t.queue_capacity);
     L.table = a.take(table_bytes(n));
     L.fb = a.take(8u * L.fb_words);
@@ -395,6 +424,9 @@ int check_views(const sgs_view* views, uint32_t n)
         blocks += ((uint64_t)v.width * v.height + kResolveBlock - 1u) / kResolveBlock;
     }
     if (blocks >= (1ull << 31)) return fail(SGS_ERROR_INVALID_ARGUMENT, "too many output pixels in one batch");
+    uint64_t tiles = 0;
+    for (uint32_t i = 0; i < n; i++) tiles += (uint64_t)view_grid_w(views[i]) * view_grid_h(views[i]);
+    if (tiles >= (1ull << 31)) return fail(SGS_ERROR_INVALID_ARGUMENT, "too many framebuffer tiles in one batch");
     return SGS_SUCCESS;
 }
 
@@ -402,8 +434,9 @@ void encode_views(const sgs_view* views, uint32_t n, void* host)
 {
     ViewDesc* d = static_cast<ViewDesc*>(host);
     uint32_t* prefix = reinterpret_cast<uint32_t*>(static_cast<char*>(host) + table_views_bytes(n));
+    uint32_t* tile_prefix = prefix + n + 1u;
     uint64_t fb_off = 0, smp_off = 0;
-    uint32_t block = 0;
+    uint32_t block = 0, tile = 0;
     for (uint32_t i = 0; i < n; i++) {
         const sgs_view& v = views[i];
         ViewDesc x;
@@ -426,6 +459,9 @@ void encode_views(const sgs_view* views, uint32_t n, void* host)
         x.frame0 = v.frame_idx;
         x.mode = v.temporal_mode;
         x.block_begin = block;
+        x.tile_begin = tile;
+        x.grid_w = view_grid_w(v);
+        x.grid_h = view_grid_h(v);
         x.fb_offset = fb_off;
         x.samples_offset = smp_off;
         x.hist = { v.history.image, v.history.depth, v.history.alpha, v.history.weight_mass };
@@ -433,12 +469,15 @@ void encode_views(const sgs_view* views, uint32_t n, void* host)
         std::memcpy(x.v2p, v.view_to_previous_view, sizeof x.v2p);
         d[i] = x;
         prefix[i] = block;
+        tile_prefix[i] = tile;
+        tile += x.grid_w * x.grid_h;
         const uint64_t P = view_fb_pixels(v);
         fb_off += P * v.spp;
         if (v.temporal_mode == SGS_TEMPORAL_REPROJECT) smp_off += P;
         block += (uint32_t)(((uint64_t)v.width * v.height + kResolveBlock - 1u) / kResolveBlock);
     }
     prefix[n] = block;
+    tile_prefix[n] = tile;
 }
 
 }  // namespace
@@ -456,10 +495,11 @@ int sgs_batch_encode_views(const sgs_view* views, uint32_t n_views, void* host_t
     return SGS_SUCCESS;
 }
 
-size_t sgs_render_batch_workspace_size(const sgs_view* views, uint32_t n_views, const sgs_batch_tuning* tuning)
+size_t sgs_render_batch_workspace_size(const sgs_view* views, uint32_t n_views, uint32_t n_gaussians,
+                                       const sgs_batch_tuning* tuning)
 {
     if (!views && n_views) return 0;
-    return batch_layout(views, n_views, resolve_batch_tuning(tuning)).total;
+    return batch_layout(views, n_views, n_gaussians, resolve_batch_tuning(tuning)).total;
 }
 
 int sgs_render_batch(const sgs_gaussians* gaussians, const sgs_view* views, uint32_t n_views,
@@ -471,7 +511,11 @@ int sgs_render_batch(const sgs_gaussians* gaussians, const sgs_view* views, uint
     if (n_views == 0) return SGS_SUCCESS;
     const BatchTuning t = resolve_batch_tuning(tuning);
     if (t.entry_fragments < 64u) return fail(SGS_ERROR_INVALID_ARGUMENT, "entry_fragments must be >= 64");
-    const BatchLayout L = batch_layout(views, n_views, t);
+    if (tuning && tuning->schedule > SGS_BATCH_SCATTER)
+        return fail(SGS_ERROR_INVALID_ARGUMENT, "unknown batch schedule " + std::to_string(tuning->schedule));
+    if (tuning && tuning->tile_samples && tuning->tile_samples != 8u && tuning->tile_samples != 16u)
+        return fail(SGS_ERROR_INVALID_ARGUMENT, "tile_samples must be 8 or 16");
+    const BatchLayout L = batch_layout(views, n_views, gaussians->n_gaussians, t);
     if (!workspace || workspace_bytes < L.total)
         return fail(SGS_ERROR_WORKSPACE_TOO_SMALL,
                     "workspace of " + std::to_string(L.total) + " bytes needed, got " + std::to_string(workspace_bytes));
@@ -493,6 +537,7 @@ int sgs_render_batch(const sgs_gaussians* gaussians, const sgs_view* views, uint
     }
     const ViewDesc* desc = reinterpret_cast<const ViewDesc*>(table);
     const uint32_t* prefix = reinterpret_cast<const uint32_t*>(table + table_views_bytes(n_views));
+    const uint32_t* tile_prefix = prefix + n_views + 1u;
     BatchCounters* counters = reinterpret_cast<BatchCounters*>(ws + L.counters);
     uint32_t* ready = reinterpret_cast<uint32_t*>(ws + L.ready);
     BigEntry* queue = reinterpret_cast<BigEntry*>(ws + L.queue);
@@ -503,16 +548,32 @@ int sgs_render_batch(const sgs_gaussians* gaussians, const sgs_view* views, uint
         return stage_events ? hipEventRecord(static_cast<hipEvent_t>(stage_events[i]), stream) : hipSuccess;
     };
     if ((e = mark(0)) != hipSuccess) return hip_fail(e, "stage event");
-    e = launch_zero_u32(reinterpret_cast<uint32_t*>(counters), (L.ready + 4u * (size_t)t.queue_capacity - L.counters) / 4u,
-                        stream);
-    if (e != hipSuccess) return hip_fail(e, "queue clear");
-    e = launch_fill_u64(fb, L.fb_words, kEmpty, stream);
-    if (e != hipSuccess) return hip_fail(e, "framebuffer clear");
+    e = launch_zero_u32(reinterpret_cast<uint32_t*>(counters), (L.cleared_end - L.counters) / 4u, stream);
+    if (e != hipSuccess) return hip_fail(e, "counter clear");
     const SceneArgs sc = scene_args(gaussians);
-    if ((e = mark(1)) != hipSuccess) return hip_fail(e, "stage event");
-    e = launch_batch_accumulate(sc, desc, n_views, fb, counters, queue, ready, t, stream);
-    if (e != hipSuccess) return hip_fail(e, "batched accumulation");
-    if ((e = mark(2)) != hipSuccess) return hip_fail(e, "stage event");
+    if (t.schedule == 0u) {
+        // tiles write every framebuffer word; on overflow the scatter schedule runs on a
+        // cleared framebuffer instead, both gated on the device flag
+        if ((e = mark(1)) != hipSuccess) return hip_fail(e, "stage event");
+        e = launch_batch_tiles(sc, desc, n_views, tile_prefix, (uint32_t)L.n_tiles,
+                               reinterpret_cast<uint32_t*>(ws + L.tile_count),
+                               reinterpret_cast<uint32_t*>(ws + L.tile_cursor), reinterpret_cast<uint32_t*>(ws + L.offsets),
+                               reinterpret_cast<uint32_t*>(ws + L.instances), L.capacity, t.tile_samples, counters, fb,
+                               stage_events ? static_cast<hipEvent_t>(stage_events[2]) : nullptr, stream);
+        if (e != hipSuccess) return hip_fail(e, "batched tiles");
+        e = launch_fill_u64_if(fb, L.fb_words, kEmpty, &counters->overflow, stream);
+        if (e != hipSuccess) return hip_fail(e, "framebuffer clear");
+        e = launch_batch_accumulate(sc, desc, n_views, fb, counters, queue, ready, t, &counters->overflow, stream);
+        if (e != hipSuccess) return hip_fail(e, "batched accumulation");
+    } else {
+        e = launch_fill_u64(fb, L.fb_words, kEmpty, stream);
+        if (e != hipSuccess) return hip_fail(e, "framebuffer clear");
+        if ((e = mark(1)) != hipSuccess) return hip_fail(e, "stage event");
+        if ((e = mark(2)) != hipSuccess) return hip_fail(e, "stage event");
+        e = launch_batch_accumulate(sc, desc, n_views, fb, counters, queue, ready, t, nullptr, stream);
+        if (e != hipSuccess) return hip_fail(e, "batched accumulation");
+    }
+    if ((e = mark(3)) != hipSuccess) return hip_fail(e, "stage event");
     // sample 0 of REPROJECT views needs the full sample buffer (3x3 neighbourhood moments)
     uint64_t fb_off = 0, smp_off = 0;
     for (uint32_t i = 0; i < n_views; i++) {
@@ -533,7 +594,21 @@ int sgs_render_batch(const sgs_gaussians* gaussians, const sgs_view* views, uint
     }
     e = launch_batch_resolve(sc, desc, n_views, prefix, L.n_blocks, fb, samples, stream);
     if (e != hipSuccess) return hip_fail(e, "batched resolve");
-    if ((e = mark(3)) != hipSuccess) return hip_fail(e, "stage event");
+    if ((e = mark(4)) != hipSuccess) return hip_fail(e, "stage event");
+    return SGS_SUCCESS;
+}
+
+int sgs_batch_tile_stats(const void* workspace, void* stream_ptr, uint64_t* n_instances, uint32_t* overflowed)
+{
+    if (!workspace || !n_instances || !overflowed)
+        return fail(SGS_ERROR_INVALID_ARGUMENT, "workspace and outputs must not be NULL");
+    BatchCounters c;
+    hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
+    hipError_t e = hipMemcpyAsync(&c, workspace, sizeof c, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return hip_fail(e, "counter read");
+    *n_instances = c.total_instances;
+    *overflowed = c.overflow;
     return SGS_SUCCESS;
 }
 

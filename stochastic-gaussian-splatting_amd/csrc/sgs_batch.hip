@@ -15,6 +15,7 @@
 // phase B after the item counter is exhausted), so the protocol cannot deadlock whatever
 // the residency of the grid.
 #include "sgs_internal.h"
+#include "sgs_scan.h"
 
 namespace sgs {
 
@@ -159,8 +160,10 @@ __global__ __launch_bounds__(kBatchBlock) void batch_accumulate_kernel(SceneArgs
                                                                        BigEntry* __restrict__ queue,
                                                                        uint32_t* __restrict__ ready, uint32_t queue_cap,
                                                                        uint32_t seq, uint32_t big_threshold,
-                                                                       uint32_t entry_frags)
+                                                                       uint32_t entry_frags,
+                                                                       const uint32_t* __restrict__ run_if_set)
 {
+    if (run_if_set && !*run_if_set) return;
     __shared__ uint32_t s_item;
     __shared__ uint32_t q_pix[kBatchBlock / 64][kQueue], q_base[kBatchBlock / 64][kQueue], q_thr[kBatchBlock / 64][kQueue];
     __shared__ uint64_t q_frag[kBatchBlock / 64][kQueue];
@@ -303,7 +306,7 @@ __global__ __launch_bounds__(kBatchBlock) void batch_accumulate_kernel(SceneArgs
 
 hipError_t launch_batch_accumulate(const SceneArgs& sc, const ViewDesc* views, uint32_t n_views, uint64_t* fb,
                                    BatchCounters* counters, BigEntry* queue, uint32_t* ready, const BatchTuning& t,
-                                   hipStream_t stream)
+                                   const uint32_t* run_if_set, hipStream_t stream)
 {
     if (sc.n == 0 || n_views == 0) return hipSuccess;
     const uint32_t n_chunks = (sc.n + kBatchBlock - 1u) / kBatchBlock;
@@ -323,7 +326,376 @@ hipError_t launch_batch_accumulate(const SceneArgs& sc, const ViewDesc* views, u
     if (grid > n_items64) grid = n_items64;
     hipLaunchKernelGGL(batch_accumulate_kernel, dim3((unsigned)grid), dim3(kBatchBlock), 0, stream, sc, views, n_views,
                        (uint32_t)n_items64, t.view_major, fb, counters, queue, ready, t.queue_capacity,
-                       t.sequential_threshold, t.big_threshold, t.entry_fragments);
+                       t.sequential_threshold, t.big_threshold, t.entry_fragments, run_if_set);
+    return hipGetLastError();
+}
+
+// ============================================================================================
+// Tile schedule: bin every (view, Gaussian) into the 16x16 tiles its AABB overlaps, then
+// rasterise each (view, tile) in one workgroup that keeps the tile's framebuffer words of up
+// to 16 samples in LDS. Each fragment's alpha is evaluated once; a sample is hashed only if
+// the fragment is closer than that sample's current LDS word, and accepted fragments land with
+// ds_min_u64; the tile is written out once with plain stores. No framebuffer clear is needed
+// because every tile writes all of its pixels. The tested pixels, alpha and thresholds are
+// those of the scatter path, so the framebuffers are bit-identical to it.
+// ============================================================================================
+constexpr uint32_t kTile = 16;
+constexpr uint32_t kWindow = 1024;    // tiles of a workgroup's LDS histogram window
+
+struct TileRange {
+    uint32_t tx0, tx1, ty0, ty1;  // inclusive
+};
+
+// tile range of Gaussian g in view vd (false if culled or its AABB is empty)
+__device__ __forceinline__ bool gaussian_tiles(const SceneArgs& sc, const ViewDesc& vd, uint32_t g, TileRange& t)
+{
+    Projected p;
+    if (!(g < sc.n && project_gaussian(sc.means, sc.scales, sc.rotations, sc.opacities, g, vd.r1, vd.r2, vd.r3, vd.cam,
+                                       p)))
+        return false;
+    const Aabb b = pixel_aabb(p, vd.cam);
+    if (b.x1 <= b.x0 || b.y1 <= b.y0) return false;
+    t.tx0 = b.x0 / kTile;
+    t.tx1 = (b.x1 - 1u) / kTile;
+    t.ty0 = b.y0 / kTile;
+    t.ty1 = (b.y1 - 1u) / kTile;
+    return true;
+}
+
+// Gaussians touching at most kSmallTiles tiles are counted through the workgroup's LDS
+// histogram (one global atomic per touched tile and workgroup); larger ones are spread over
+// their wave's lanes with one global atomic per tile.
+constexpr uint32_t kSmallTiles = 32;
+
+template <bool kFill>
+__device__ __forceinline__ void bin_direct(uint32_t gt, uint32_t g, uint32_t* tile_count, uint32_t* tile_cursor,
+                                           const uint32_t* offsets, uint32_t* instances)
+{
+    if constexpr (kFill) instances[offsets[gt] + atomicAdd(&tile_cursor[gt], 1u)] = g;
+    else atomicAdd(&tile_count[gt], 1u);
+}
+
+template <bool kFill>
+__global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, const ViewDesc* __restrict__ views,
+                                                                uint32_t n_views, uint32_t n_items,
+                                                                uint32_t* __restrict__ tile_count,
+                                                                uint32_t* __restrict__ tile_cursor,
+                                                                const uint32_t* __restrict__ offsets,
+                                                                uint32_t* __restrict__ instances,
+                                                                BatchCounters* __restrict__ ctr)
+{
+    __shared__ uint32_t s_item, s_wx0, s_wx1, s_wy0, s_wy1;
+    __shared__ unsigned long long s_total;
+    __shared__ uint32_t hist[kWindow];
+    __shared__ uint32_t hbase[kWindow];
+    if constexpr (kFill) {
+        if (ctr->overflow) return;
+    }
+    const uint32_t tid = threadIdx.x, lane = lane_id();
+    const uint32_t n_chunks = (sc.n + kBatchBlock - 1u) / kBatchBlock;
+    for (;;) {
+        if (tid == 0) {
+            s_item = atomicAdd(kFill ? &ctr->next_fill_item : &ctr->next_count_item, 1u);
+            s_wx0 = 0xffffffffu;
+            s_wy0 = 0xffffffffu;
+            s_wx1 = 0u;
+            s_wy1 = 0u;
+            s_total = 0ull;
+        }
+        __syncthreads();
+        const uint32_t item = __builtin_amdgcn_readfirstlane(s_item);
+        if (item >= n_items) break;
+        const uint32_t v = item / n_chunks, chunk = item - v * n_chunks;
+        const ViewDesc& vd = views[v];
+        const uint32_t g = chunk * kBatchBlock + tid;
+        TileRange tr;
+        const bool has = gaussian_tiles(sc, vd, g, tr);
+        const uint32_t tw = has ? tr.tx1 - tr.tx0 + 1u : 0u;
+        const uint32_t n_t = has ? tw * (tr.ty1 - tr.ty0 + 1u) : 0u;
+        const bool small = has && n_t <= kSmallTiles;
+        if (small) {
+            atomicMin(&s_wx0, tr.tx0);
+            atomicMax(&s_wx1, tr.tx1);
+            atomicMin(&s_wy0, tr.ty0);
+            atomicMax(&s_wy1, tr.ty1);
+        }
+        if constexpr (!kFill) {
+            if (has) atomicAdd(&s_total, (unsigned long long)n_t);
+        }
+        // large Gaussians: the wave walks each one's tiles
+        uint64_t big = __ballot(has && !small);
+        while (big) {
+            const int l = __builtin_ctzll(big);
+            big &= big - 1ull;
+            const uint32_t bx0 = bcast(tr.tx0, l), by0 = bcast(tr.ty0, l), bw = bcast(tw, l), bn = bcast(n_t, l);
+            const uint32_t bg = bcast(g, l);
+            for (uint32_t k = lane; k < bn; k += 64u) {
+                const uint32_t ty = by0 + k / bw, tx = bx0 + k % bw;
+                bin_direct<kFill>(vd.tile_begin + ty * vd.grid_w + tx, bg, tile_count, tile_cursor, offsets, instances);
+            }
+        }
+        __syncthreads();
+        if constexpr (!kFill) {
+            if (tid == 0 && s_total) atomicAdd(&ctr->total_instances, s_total);
+        }
+        const uint32_t wx0 = s_wx0, wx1 = s_wx1, wy0 = s_wy0, wy1 = s_wy1;
+        if (wx0 <= wx1) {
+            const uint32_t wt = wx1 - wx0 + 1u, ht = wy1 - wy0 + 1u;
+            if (wt * ht <= kWindow) {
+                const uint32_t nw = wt * ht;
+                for (uint32_t i = tid; i < nw; i += kBatchBlock) hist[i] = 0u;
+                __syncthreads();
+                if (small)
+                    for (uint32_t ty = tr.ty0; ty <= tr.ty1; ty++)
+                        for (uint32_t tx = tr.tx0; tx <= tr.tx1; tx++) atomicAdd(&hist[(ty - wy0) * wt + tx - wx0], 1u);
+                __syncthreads();
+                for (uint32_t i = tid; i < nw; i += kBatchBlock) {
+                    const uint32_t c = hist[i];
+                    if (!c) continue;
+                    const uint32_t gt = vd.tile_begin + (wy0 + i / wt) * vd.grid_w + wx0 + i % wt;
+                    if constexpr (kFill) {
+                        hbase[i] = offsets[gt] + atomicAdd(&tile_cursor[gt], c);
+                        hist[i] = 0u;
+                    } else {
+                        atomicAdd(&tile_count[gt], c);
+                    }
+                }
+                if constexpr (kFill) {
+                    __syncthreads();
+                    if (small)
+                        for (uint32_t ty = tr.ty0; ty <= tr.ty1; ty++)
+                            for (uint32_t tx = tr.tx0; tx <= tr.tx1; tx++) {
+                                const uint32_t i = (ty - wy0) * wt + tx - wx0;
+                                instances[hbase[i] + atomicAdd(&hist[i], 1u)] = g;
+                            }
+                }
+            } else if (small) {
+                // spread-out chunk: one global atomic per instance
+                for (uint32_t ty = tr.ty0; ty <= tr.ty1; ty++)
+                    for (uint32_t tx = tr.tx0; tx <= tr.tx1; tx++)
+                        bin_direct<kFill>(vd.tile_begin + ty * vd.grid_w + tx, g, tile_count, tile_cursor, offsets,
+                                          instances);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// inclusive scan of one value per thread over the 256-thread workgroup
+__device__ __forceinline__ uint32_t block_inclusive_scan(uint32_t x, uint32_t* wave_sums)
+{
+    const uint32_t lane = lane_id(), wave = threadIdx.x / 64u;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)x, off, 64);
+        if ((int)lane >= off) x += y;
+    }
+    if (lane == 63u) wave_sums[wave] = x;
+    __syncthreads();
+    for (uint32_t w = 0; w < wave; w++) x += wave_sums[w];
+    return x;
+}
+
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t x, int m)
+{
+    const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)x, m, 64);
+    const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(x >> 32), m, 64);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// The tile's pixels are 16 sub-tiles of 4x4; a wave holds rows 4w..4w+3, i.e. sub-tiles
+// (0..3, w). kSubtiles maxima bound the framebuffer words an instance can still lower.
+constexpr uint32_t kSub = 4, kSubtiles = (kTile / kSub) * (kTile / kSub);
+
+// kLayers: samples held in LDS per pass over the tile's instances
+template <uint32_t kLayers>
+__global__ __launch_bounds__(kBatchBlock) void batch_tile_raster_kernel(SceneArgs sc, const ViewDesc* __restrict__ views,
+                                                                        uint32_t n_views,
+                                                                        const uint32_t* __restrict__ tile_prefix,
+                                                                        const uint32_t* __restrict__ offsets,
+                                                                        const uint32_t* __restrict__ instances,
+                                                                        const BatchCounters* __restrict__ ctr,
+                                                                        uint64_t* __restrict__ fb)
+{
+    __shared__ unsigned long long layer[kLayers][kTile * kTile];
+    __shared__ uint64_t s_frag[kBatchBlock];
+    __shared__ uint64_t s_sub[kSubtiles];
+    __shared__ float s_msx[kBatchBlock], s_msy[kBatchBlock], s_o[kBatchBlock];
+    __shared__ float s_ca[kBatchBlock], s_cb[kBatchBlock], s_cc[kBatchBlock];
+    __shared__ uint32_t s_rect[kBatchBlock], s_end[kBatchBlock];
+    __shared__ uint32_t s_wave[kBatchBlock / 64u];
+    if (ctr->overflow) return;
+    const uint32_t gt = blockIdx.x;
+    uint32_t lo = 0, hi = n_views - 1u;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1u) >> 1;
+        if (tile_prefix[mid] <= gt) lo = mid;
+        else hi = mid - 1u;
+    }
+    const ViewDesc& vd = views[lo];
+    const uint32_t lt = gt - tile_prefix[lo];
+    const uint32_t ox = (lt % vd.grid_w) * kTile, oy = (lt / vd.grid_w) * kTile;
+    const uint32_t W = vd.cam.width_i, H = vd.cam.height_i;
+    const size_t P = (size_t)W * H;
+    const uint32_t begin = offsets[gt], end = offsets[gt + 1u];
+    const uint32_t tid = threadIdx.x, lane = lane_id();
+    const uint32_t ptx = ox + tid % kTile, pty = oy + tid / kTile;
+    const bool inside = ptx < W && pty < H;
+    uint64_t* fbv = fb + vd.fb_offset;
+
+    for (uint32_t s0 = 0; s0 < vd.spp; s0 += kLayers) {
+        const uint32_t G = min(kLayers, vd.spp - s0);
+        // pixels outside the view hold 0 so they never keep an instance alive
+        for (uint32_t s = 0; s < G; s++) layer[s][tid] = inside ? kEmpty : 0ull;
+        const uint32_t fs_group = (vd.frame0 + s0) * 1973u;
+        for (uint32_t batch = begin; batch < end; batch += kBatchBlock) {
+            __syncthreads();
+            // sub-tile maxima of the words so far
+            {
+                uint64_t m = 0;
+                for (uint32_t s = 0; s < G; s++) m = max(m, (uint64_t)layer[s][tid]);
+                m = max(m, shfl_xor_u64(m, 1));
+                m = max(m, shfl_xor_u64(m, 2));
+                m = max(m, shfl_xor_u64(m, 16));
+                m = max(m, shfl_xor_u64(m, 32));
+                if ((lane & 51u) == 0u) s_sub[(tid / 64u) * (kTile / kSub) + (lane & 15u) / kSub] = m;
+            }
+            const uint32_t nb = min(kBatchBlock, end - batch);
+            uint32_t cnt = 0, rect = 0;
+            uint64_t frag = 0;
+            if (tid < nb) {
+                const uint32_t g = instances[batch + tid];
+                Projected p;
+                if (project_gaussian(sc.means, sc.scales, sc.rotations, sc.opacities, g, vd.r1, vd.r2, vd.r3, vd.cam,
+                                     p)) {
+                    const Aabb b = pixel_aabb(p, vd.cam);
+                    const uint32_t cx0 = max(b.x0, ox), cx1 = min(b.x1, ox + kTile);
+                    const uint32_t cy0 = max(b.y0, oy), cy1 = min(b.y1, oy + kTile);
+                    if (cx1 > cx0 && cy1 > cy0) {
+                        const uint32_t rw = cx1 - cx0, rh = cy1 - cy0;
+                        rect = (cx0 - ox) | ((cy0 - oy) << 4) | ((rw - 1u) << 8) | (((65536u + rw - 1u) / rw) << 12);
+                        cnt = rw * rh;
+                    }
+                    frag = fragment_key(p.depth, g);
+                    const ConicExp2 ce = conic_exp2(p.conic_x, p.conic_y, p.conic_z);
+                    s_frag[tid] = frag;
+                    s_msx[tid] = p.mean_x - 0.5f;
+                    s_msy[tid] = p.mean_y - 0.5f;
+                    s_o[tid] = p.opacity;
+                    s_ca[tid] = ce.a;
+                    s_cb[tid] = ce.b;
+                    s_cc[tid] = ce.c;
+                }
+                s_rect[tid] = rect;
+            }
+            __syncthreads();
+            if (cnt) {
+                // an instance that no word of its sub-tiles exceeds cannot win any sample
+                const uint32_t rx0 = rect & 0xfu, ry0 = (rect >> 4) & 0xfu, rw = ((rect >> 8) & 0xfu) + 1u;
+                const uint32_t rh = cnt / rw;
+                uint64_t m = 0;
+                for (uint32_t sy = ry0 / kSub; sy <= (ry0 + rh - 1u) / kSub; sy++)
+                    for (uint32_t sx = rx0 / kSub; sx <= (rx0 + rw - 1u) / kSub; sx++)
+                        m = max(m, s_sub[sy * (kTile / kSub) + sx]);
+                if (!(frag < m)) cnt = 0;
+            }
+            // fragments of the batch's live instances, flattened over the workgroup's lanes
+            const uint32_t incl = block_inclusive_scan(cnt, s_wave);
+            s_end[tid] = incl;
+            __syncthreads();
+            const uint32_t total = s_end[kBatchBlock - 1u];
+            for (uint32_t k = tid; k < total; k += kBatchBlock) {
+                uint32_t a = 0, b = kBatchBlock - 1u;
+                while (a < b) {
+                    const uint32_t m = (a + b) >> 1;
+                    if (s_end[m] > k) b = m;
+                    else a = m + 1u;
+                }
+                const uint32_t j = a;
+                const uint32_t f = k - (j ? s_end[j - 1u] : 0u);
+                const uint32_t rect_j = s_rect[j];
+                const uint32_t rw = ((rect_j >> 8) & 0xfu) + 1u;
+                const uint32_t q = (f * (rect_j >> 12)) >> 16, r = f - q * rw;  // f / rw for f < 256, rw <= 16
+                const uint32_t lx = (rect_j & 0xfu) + r, ly = ((rect_j >> 4) & 0xfu) + q;
+                const uint32_t l = ly * kTile + lx;
+                const uint64_t fj = s_frag[j];
+                // samples whose word this fragment is still below (words only decrease, so a
+                // stale read never drops a winner)
+                uint32_t live = 0;
+                for (uint32_t s = 0; s < G; s += 4u) {
+                    uint64_t w4[4];
+#pragma unroll
+                    for (uint32_t i = 0; i < 4u; i++)
+                        w4[i] = s + i < G ? __hip_atomic_load(&layer[s + i][l], __ATOMIC_RELAXED,
+                                                              __HIP_MEMORY_SCOPE_WORKGROUP)
+                                          : 0ull;
+#pragma unroll
+                    for (uint32_t i = 0; i < 4u; i++)
+                        if (fj < w4[i]) live |= 1u << (s + i);
+                }
+                if (!live) continue;
+                const uint32_t px = ox + lx, py = oy + ly;
+                const ConicExp2 ce = { s_ca[j], s_cb[j], s_cc[j] };
+                const float alpha = fragment_alpha(ce, s_o[j], (float)px - s_msx[j], (float)py - s_msy[j]);
+                if (alpha < kMinAlpha) continue;
+                const uint32_t T = alpha_threshold(alpha);
+                const uint32_t base = (uint32_t)fj ^ (px * 9277u) ^ (py * 2699u);
+                while (live) {
+                    const uint32_t s = (uint32_t)__builtin_ctz(live);
+                    live &= live - 1u;
+                    const uint32_t h = pcg_hash(base ^ (fs_group + s * 1973u));
+                    if ((h >> 9u) < T) atomicMin(&layer[s][l], (unsigned long long)fj);
+                }
+            }
+        }
+        __syncthreads();
+        if (inside)
+            for (uint32_t s = 0; s < G; s++) fbv[(size_t)(s0 + s) * P + (size_t)pty * W + ptx] = layer[s][tid];
+        __syncthreads();
+    }
+}
+
+hipError_t launch_batch_tiles(const SceneArgs& sc, const ViewDesc* views, uint32_t n_views, const uint32_t* tile_prefix,
+                              uint32_t n_tiles, uint32_t* tile_count, uint32_t* tile_cursor, uint32_t* offsets,
+                              uint32_t* instances, uint32_t capacity, uint32_t layers, BatchCounters* counters,
+                              uint64_t* fb, hipEvent_t raster_event, hipStream_t stream)
+{
+    if (n_views == 0 || n_tiles == 0) return hipSuccess;
+    const uint32_t n_chunks = (sc.n + kBatchBlock - 1u) / kBatchBlock;
+    const uint64_t n_items64 = (uint64_t)n_chunks * n_views;
+    if (n_items64 >= 0xffffffffull) return hipErrorInvalidValue;
+    int device = 0, cus = 0, per_cu = 0;
+    hipError_t e = hipGetDevice(&device);
+    if (e != hipSuccess) return e;
+    e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+    if (e != hipSuccess) return e;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(batch_bin_kernel<true>),
+                                                     kBatchBlock, 0);
+    if (e != hipSuccess) return e;
+    uint64_t grid = (uint64_t)cus * (per_cu < 1 ? 1 : per_cu);
+    if (grid > n_items64) grid = n_items64;
+    if (grid < 1) grid = 1;
+    if (sc.n > 0) {
+        hipLaunchKernelGGL((batch_bin_kernel<false>), dim3((unsigned)grid), dim3(kBatchBlock), 0, stream, sc, views,
+                           n_views, (uint32_t)n_items64, tile_count, tile_cursor, offsets, instances, counters);
+    }
+    hipLaunchKernelGGL((exclusive_scan_kernel<CountInput>), dim3(1), dim3(kScanThreads), 0, stream,
+                       CountInput{ tile_count }, offsets, nullptr, n_tiles, n_tiles, &counters->n_instances,
+                       &counters->overflow, capacity, &counters->total_instances);
+    if (sc.n > 0) {
+        hipLaunchKernelGGL((batch_bin_kernel<true>), dim3((unsigned)grid), dim3(kBatchBlock), 0, stream, sc, views,
+                           n_views, (uint32_t)n_items64, tile_count, tile_cursor, offsets, instances, counters);
+    }
+    if (raster_event) {
+        e = hipEventRecord(raster_event, stream);
+        if (e != hipSuccess) return e;
+    }
+    if (layers == 8u)
+        hipLaunchKernelGGL(batch_tile_raster_kernel<8>, dim3(n_tiles), dim3(kBatchBlock), 0, stream, sc, views,
+                           n_views, tile_prefix, offsets, instances, counters, fb);
+    else
+        hipLaunchKernelGGL(batch_tile_raster_kernel<16>, dim3(n_tiles), dim3(kBatchBlock), 0, stream, sc, views,
+                           n_views, tile_prefix, offsets, instances, counters, fb);
     return hipGetLastError();
 }
 

@@ -24,12 +24,12 @@
 // workgroup iteration, one point per lane. Every point uses the per-point counter seed of
 // the scatter path, so the result is bit-identical to it.
 #include "sgs_internal.h"
+#include "sgs_scan.h"
 
 namespace sgs {
 
 constexpr uint32_t kRasterThreads = 256;
 constexpr uint32_t kChunk = 256;
-constexpr uint32_t kScanThreads = 1024;
 constexpr uint32_t kPersistentBlocks = 2048;
 
 // ------------------------------------------------------------------------------------------
@@ -80,71 +80,10 @@ static GpsLayout gps_layout(const Tuning& t)
 size_t hybrid_st_scratch_bytes(uint32_t fb_w, uint32_t fb_h, const Tuning& t) { return st_layout(fb_w, fb_h, t).total; }
 size_t hybrid_gps_scratch_bytes(const Tuning& t) { return gps_layout(t).total; }
 
-// ------------------------------------------------------------------------------------------
-// single-workgroup exclusive scan (n known on the device); out has n + 1 entries
-// ------------------------------------------------------------------------------------------
-struct CountInput {
-    const uint32_t* values;
-    __device__ uint32_t operator()(uint32_t i) const { return values[i]; }
-};
 struct ChunkInput {
     const BinnedGPS* binned;
     __device__ uint32_t operator()(uint32_t i) const { return (binned[i].n_points + kChunk - 1u) / kChunk; }
 };
-
-template <typename Input>
-__global__ __launch_bounds__(kScanThreads) void exclusive_scan_kernel(Input in, uint32_t* __restrict__ out,
-                                                                      const uint32_t* __restrict__ n_device,
-                                                                      uint32_t n_host, uint32_t capacity,
-                                                                      uint32_t* __restrict__ total_out,
-                                                                      uint32_t* __restrict__ overflow_out,
-                                                                      uint32_t overflow_limit)
-{
-    __shared__ uint32_t wave_sums[kScanThreads / 64];
-    uint32_t n = n_device ? *n_device : n_host;
-    if (n > capacity) n = capacity;
-    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-    uint32_t carry = 0;
-    for (uint32_t base = 0; base < n; base += kScanThreads * 4u) {
-        uint32_t v[4];
-        uint32_t local = 0;
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const uint32_t i = base + tid * 4u + (uint32_t)k;
-            v[k] = i < n ? in(i) : 0u;
-            local += v[k];
-        }
-        // inclusive wave scan of the per-thread totals
-        uint32_t incl = local;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const uint32_t y = (uint32_t)__shfl_up((int)incl, off, 64);
-            if ((int)lane >= off) incl += y;
-        }
-        if (lane == 63u) wave_sums[wave] = incl;
-        __syncthreads();
-        uint32_t wave_off = 0, block_total = 0;
-        for (uint32_t w = 0; w < kScanThreads / 64; w++) {
-            const uint32_t s = wave_sums[w];
-            if (w < wave) wave_off += s;
-            block_total += s;
-        }
-        uint32_t run = carry + wave_off + incl - local;
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const uint32_t i = base + tid * 4u + (uint32_t)k;
-            if (i < n) out[i] = run;
-            run += v[k];
-        }
-        carry += block_total;
-        __syncthreads();
-    }
-    if (tid == 0) {
-        out[n] = carry;
-        if (total_out) *total_out = carry;
-        if (overflow_out) *overflow_out = carry > overflow_limit ? 1u : 0u;
-    }
-}
 
 // ------------------------------------------------------------------------------------------
 // tile spans of a binned ST Gaussian
@@ -433,7 +372,7 @@ hipError_t run_hybrid_st(const SceneArgs& sc, const float* w2c, const FbCamera& 
                        t.max_binned, t.tile_size, L.grid_w, tile_count, tile_cursor, tile_offset, instances);
     hipLaunchKernelGGL((exclusive_scan_kernel<CountInput>), dim3(1), dim3(kScanThreads), 0, stream,
                        CountInput{ tile_count }, tile_offset, nullptr, L.n_tiles, L.n_tiles, &counters->n_instances,
-                       &counters->overflow, t.max_tile_instances);
+                       &counters->overflow, t.max_tile_instances, nullptr);
     hipLaunchKernelGGL((tile_instances_kernel<true>), dim3(grid), dim3(256), 0, stream, binned, counters,
                        t.max_binned, t.tile_size, L.grid_w, tile_count, tile_cursor, tile_offset, instances);
     const bool blue = bn != nullptr;
@@ -500,7 +439,7 @@ hipError_t run_hybrid_gps(const SceneArgs& sc, const float* w2c, const FbCamera&
     e = launch_gps_scatter(sc, w2c, cam, fb, frame, max_points, t.sequential_threshold, bin, counters_out, stream);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((exclusive_scan_kernel<ChunkInput>), dim3(1), dim3(kScanThreads), 0, stream,
-                       ChunkInput{ binned }, offsets, &counters->n_binned, 0u, t.max_binned, nullptr, nullptr, 0u);
+                       ChunkInput{ binned }, offsets, &counters->n_binned, 0u, t.max_binned, nullptr, nullptr, 0u, nullptr);
     hipLaunchKernelGGL(gps_chunks_kernel, dim3(kPersistentBlocks), dim3(kChunk), 0, stream, binned, counters,
                        t.max_binned, offsets, cam, fb);
     return hipGetLastError();

@@ -96,6 +96,8 @@ struct alignas(16) ViewDesc {
     uint32_t frame0;            // RNG frame of sample 0; sample s uses frame0 + s
     int32_t mode;               // temporal mode of sample 0; later samples are STATIC
     uint32_t block_begin;       // first resolve block of this view
+    uint32_t tile_begin;        // first 16x16 tile of this view in the batch's tile numbering
+    uint32_t grid_w, grid_h;    // tiles per row / column of this view's framebuffer
     uint64_t fb_offset;         // first framebuffer word of sample 0 (samples are P words apart)
     uint64_t samples_offset;    // REPROJECT views: first float4 of the sample-0 buffer
     HistoryPtrs hist;
@@ -109,8 +111,14 @@ struct BatchCounters {
     uint32_t done_items;        // phase A items finished (release after their queue writes)
     uint32_t produced;          // queue slots reserved
     uint32_t consumed;          // queue slots claimed by phase B waves
-    uint32_t pad[60];
+    uint32_t next_count_item;   // tile binning, counting pass
+    uint32_t next_fill_item;    // tile binning, fill pass
+    uint32_t n_instances;       // (view, tile, Gaussian) instances of the tile schedule
+    uint32_t overflow;          // instances exceed the capacity: the scatter schedule runs instead
+    unsigned long long total_instances;  // 64-bit instance total of the counting pass
+    uint32_t pad[54];
 };
+static_assert(sizeof(BatchCounters) == 256, "BatchCounters layout");
 
 // A fragment range of a large Gaussian in one view, deferred to phase B (64 bytes).
 struct alignas(16) BigEntry {
@@ -123,6 +131,9 @@ struct alignas(16) BigEntry {
 static_assert(sizeof(BigEntry) == 64, "BigEntry layout");
 
 struct BatchTuning {
+    uint32_t schedule;          // 0: LDS tiles, 1: scatter with global atomics
+    uint32_t max_tile_instances;
+    uint32_t tile_samples;      // samples per LDS pass of the tile rasteriser (8 or 16)
     uint32_t sequential_threshold;
     uint32_t big_threshold;     // fragments above which a Gaussian goes to the queue
     uint32_t entry_fragments;   // fragments per queue entry
@@ -159,7 +170,15 @@ hipError_t launch_gps_scatter(const SceneArgs& sc, const float* w2c, const FbCam
 // ---- launchers (sgs_batch.hip) ----
 hipError_t launch_batch_accumulate(const SceneArgs& sc, const ViewDesc* views, uint32_t n_views, uint64_t* fb,
                                    BatchCounters* counters, BigEntry* queue, uint32_t* ready, const BatchTuning& t,
-                                   hipStream_t stream);
+                                   const uint32_t* run_if_set, hipStream_t stream);
+// tile schedule: binning (count, scan, fill) and per-tile LDS rasterisation of every view's
+// samples; the scatter schedule above runs instead (gated on counters->overflow) when the
+// instance list overflows
+hipError_t launch_batch_tiles(const SceneArgs& sc, const ViewDesc* views, uint32_t n_views, const uint32_t* tile_prefix,
+                              uint32_t n_tiles, uint32_t* tile_count, uint32_t* tile_cursor, uint32_t* offsets,
+                              uint32_t* instances, uint32_t capacity, uint32_t layers, BatchCounters* counters,
+                              uint64_t* fb, hipEvent_t raster_event, hipStream_t stream);
+hipError_t launch_fill_u64_if(uint64_t* p, size_t n, uint64_t value, const uint32_t* run_if_set, hipStream_t stream);
 
 // ---- launchers (sgs_hybrid.hip) ----
 size_t hybrid_st_scratch_bytes(uint32_t fb_width, uint32_t fb_height, const Tuning& t);

@@ -100,3 +100,41 @@ def test_python_api_mirrors_reference():
     assert [(m.name, int(m)) for m in TemporalMode] == [("RESET", 0), ("STATIC", 1), ("REPROJECT", 2)]
     s = Settings(*range(15))
     assert s.as_tuple() == tuple(range(15))
+
+
+def _batch_views(native, n=2):
+    views = (native.sgs_view * n)()
+    for i in range(n):
+        v = views[i]
+        v.w2c[:] = [1.0, 0, 0, 0, 0, 1.0, 0, 0, 0, 0, 1.0, 3.0, 0, 0, 0, 1.0]
+        v.width, v.height = 100 + 37 * i, 60 + 5 * i
+        v.focal_x = v.focal_y = 80.0
+        v.center_x, v.center_y = v.width / 2, v.height / 2
+        v.near_plane, v.far_plane = 0.2, 100.0
+        v.supersampling_factor = 1
+        v.spp = 1 + 3 * i
+        v.out = native.sgs_history(0x10000, 0x20000, 0x30000, 0x40000)
+    return views
+
+
+def test_batch_workspace_and_schedule_validation(native):
+    lib = native.library()
+    views = _batch_views(native)
+    tiles = native.sgs_batch_tuning(schedule=1, max_tile_instances=1 << 20)
+    scatter = native.sgs_batch_tuning(schedule=2)
+    w_tiles = lib.sgs_render_batch_workspace_size(views, 2, 1000, C.byref(tiles))
+    w_scatter = lib.sgs_render_batch_workspace_size(views, 2, 1000, C.byref(scatter))
+    # the tile schedule adds the instance list (4 B each) and per-tile counts/cursors/offsets
+    assert w_tiles - w_scatter >= 4 * (1 << 20)
+    assert lib.sgs_render_batch_workspace_size(views, 2, 1000, None) >= w_scatter
+    # the default instance capacity grows with the Gaussian count
+    assert lib.sgs_render_batch_workspace_size(views, 2, 10 ** 6, None) > lib.sgs_render_batch_workspace_size(
+        views, 2, 1000, None)
+    # view table: descriptors, then the resolve-block and tile prefixes
+    assert lib.sgs_batch_view_table_bytes(2) % 4 == 0
+    bad = native.sgs_batch_tuning(schedule=7)
+    g = _gaussians(native)
+    rc = lib.sgs_render_batch(C.byref(g), views, 2, None, C.byref(bad), 0x100000, 1 << 40, None, None)
+    assert rc == -1 and b"schedule" in lib.sgs_last_error()
+    host = (C.c_uint8 * lib.sgs_batch_view_table_bytes(2))()
+    assert lib.sgs_batch_encode_views(views, 2, host) == 0

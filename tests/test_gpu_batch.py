@@ -1,6 +1,7 @@
-"""GPU parity of the batched persistent renderer: every view equals spp successive
-single-view render() calls (RESET/STATIC/REPROJECT first sample, then STATIC) and the
-oracle's SPP loop; queue and overflow schedules and graph replays give the same bits."""
+"""GPU parity of the batched renderer: every view equals spp successive single-view
+render() calls (RESET/STATIC/REPROJECT first sample, then STATIC) and the oracle's SPP
+loop; the tile and scatter schedules, their overflow paths and graph replays give the
+same bits."""
 from __future__ import annotations
 
 import numpy as np
@@ -69,10 +70,13 @@ def make_views():
 
 
 @pytest.mark.parametrize("tuning", [
-    dict(),                                                   # defaults
-    dict(big_threshold=64, entry_fragments=64),               # most Gaussians through the work queue
-    dict(big_threshold=64, entry_fragments=64, queue_capacity=3),  # queue overflow: scattered in place
-    dict(view_major=1, sequential_threshold=1, blocks_per_cu=1),
+    dict(),                                                   # defaults (tile schedule)
+    dict(max_tile_instances=16),                              # tile list overflow: scatter fallback
+    dict(tile_samples=8),                                     # 8 samples per LDS pass
+    dict(schedule=2),                                         # scatter schedule
+    dict(schedule=2, big_threshold=64, entry_fragments=64),   # most Gaussians through the work queue
+    dict(schedule=2, big_threshold=64, entry_fragments=64, queue_capacity=3),  # queue overflow: in place
+    dict(schedule=2, view_major=1, sequential_threshold=1, blocks_per_cu=1),
 ])
 def test_batch_equals_plain_spp_loop(gpu, tuning):
     import torch
@@ -89,6 +93,35 @@ def test_batch_equals_plain_spp_loop(gpu, tuning):
         ref = plain_spp(gpu, t, c, f, n)
         torch.cuda.synchronize()
         assert_close(outs[v], ref)
+
+
+@pytest.mark.parametrize("case", ["many_samples", "many_samples_8", "wide_views"])
+def test_batch_tile_schedule_edges(gpu, case):
+    """More than 16 samples (several LDS passes per tile), and views wider than a
+    workgroup's 1024-tile binning window with large Gaussians (direct binning), each
+    against the scatter schedule and the plain SPP loop."""
+    import torch
+    from StochasticGSCudaBackend.batch import BatchRenderer, BatchTuning, BatchView
+    if case.startswith("many_samples"):
+        arrays = small_scene(2000, seed=45, log_scale_mean=-3.3, log_scale_std=0.8)
+        cams, spps, frames = [small_camera(72, 40, angle=0.4), small_camera(33, 47, angle=1.9)], [20, 17], [9, 4000]
+    else:
+        arrays = small_scene(1500, seed=46, log_scale_mean=-2.6, log_scale_std=0.6)
+        cams = [small_camera(640, 520, angle=0.2), small_camera(300, 200, angle=2.5, supersampling_factor=3)]
+        spps, frames = [2, 1], [50, 60]
+    t = to_device(arrays)
+    views = [BatchView(settings_of(c), spp=n, frame_idx=f) for c, n, f in zip(cams, spps, frames)]
+    print_case(test="batch_tile_edges", case=case)
+    tiles = BatchRenderer(t["means"], t["scales"], t["rotations"], t["opacities"], t["sh"], views,
+                          BatchTuning(tile_samples=8 if case.endswith("_8") else 0))
+    scatter = BatchRenderer(t["means"], t["scales"], t["rotations"], t["opacities"], t["sh"], views,
+                            BatchTuning(schedule=2))
+    tiles.render()
+    scatter.render()
+    torch.cuda.synchronize()
+    for v, (c, n, f) in enumerate(zip(cams, spps, frames)):
+        assert_close(tiles.outputs[v], scatter.outputs[v])
+        assert_close(tiles.outputs[v], plain_spp(gpu, t, c, f, n))
 
 
 def test_batch_temporal_modes(gpu):
