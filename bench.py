@@ -46,6 +46,8 @@ def parse() -> argparse.Namespace:
                     help="batched accumulation: LDS tiles (default) or persistent scatter")
     ap.add_argument("--tile-samples", type=int, choices=[8, 16], default=16,
                     help="samples per LDS pass of the tile rasteriser")
+    ap.add_argument("--view-major", action="store_true",
+                    help="batched work items view by view instead of Gaussian chunk by chunk")
     ap.add_argument("--copy-probe", action="store_true",
                     help="copy 1 GiB once first, to calibrate HBM byte counters against a known size")
     ap.add_argument("--no-morton", action="store_true",
@@ -371,7 +373,7 @@ def bench_batched(args, scene, views, S, plain_out0, dev):
     from StochasticGSCudaBackend.batch import SCHEDULE_SCATTER, SCHEDULE_TILES, BatchRenderer, BatchTuning, BatchView
     bviews = [BatchView(host_settings(cam), spp=cam["spp"], frame_idx=0) for cam in views]
     tuning = BatchTuning(schedule=SCHEDULE_TILES if args.schedule == "tiles" else SCHEDULE_SCATTER,
-                         tile_samples=args.tile_samples)
+                         tile_samples=args.tile_samples, view_major=int(args.view_major))
     br = BatchRenderer(scene["means"], scene["scales"], scene["rotations"], scene["opacities"], scene["sh"], bviews,
                        tuning)
     log(f"batched workspace {br.workspace.numel() / 2**30:.2f} GiB")

@@ -354,7 +354,8 @@ uint32_t view_grid_h(const sgs_view& v) { return (uint32_t)((v.height * view_ss(
 // Workspace of a batch. The counters, the queue's ready flags and the tile counts and
 // cursors are contiguous so one fill clears them all.
 struct BatchLayout {
-    size_t counters, ready, tile_count, tile_cursor, cleared_end, offsets, instances, queue, table, fb, samples, total;
+    size_t counters, ready, tile_count, tile_cursor, cleared_end, offsets, instances, centroids, chunk_order, queue, table,
+        fb, samples, total;
     uint64_t fb_words, n_samples, n_tiles;
     uint32_t n_blocks, capacity;
 };
@@ -389,6 +390,9 @@ BatchLayout batch_layout(const sgs_view* views, uint32_t n, uint32_t n_gaussians
     a.offset = (a.offset + 255u) & ~size_t(255u);
     L.offsets = a.take(4u * (tiles_n + 1u));
     L.instances = a.take(4u * (size_t)L.capacity);
+    const size_t n_chunks = tiles ? ((size_t)n_gaussians + 255u) / 256u : 0u;
+    L.centroids = a.take(16u * n_chunks);
+    L.chunk_order = a.take(4u * n_chunks * n);
     L.queue = a.take(sizeof(BigEntry) * (size_t)t.queue_capacity);
     L.table = a.take(table_bytes(n));
     L.fb = a.take(8u * L.fb_words);
@@ -555,10 +559,16 @@ int sgs_render_batch(const sgs_gaussians* gaussians, const sgs_view* views, uint
         // tiles write every framebuffer word; on overflow the scatter schedule runs on a
         // cleared framebuffer instead, both gated on the device flag
         if ((e = mark(1)) != hipSuccess) return hip_fail(e, "stage event");
-        e = launch_batch_tiles(sc, desc, n_views, tile_prefix, (uint32_t)L.n_tiles,
-                               reinterpret_cast<uint32_t*>(ws + L.tile_count),
-                               reinterpret_cast<uint32_t*>(ws + L.tile_cursor), reinterpret_cast<uint32_t*>(ws + L.offsets),
-                               reinterpret_cast<uint32_t*>(ws + L.instances), L.capacity, t.tile_samples, counters, fb,
+        TileBuffers tb;
+        tb.tile_count = reinterpret_cast<uint32_t*>(ws + L.tile_count);
+        tb.tile_cursor = reinterpret_cast<uint32_t*>(ws + L.tile_cursor);
+        tb.offsets = reinterpret_cast<uint32_t*>(ws + L.offsets);
+        tb.instances = reinterpret_cast<uint32_t*>(ws + L.instances);
+        tb.centroids = reinterpret_cast<float4*>(ws + L.centroids);
+        tb.chunk_order = reinterpret_cast<uint32_t*>(ws + L.chunk_order);
+        tb.n_tiles = (uint32_t)L.n_tiles;
+        tb.capacity = L.capacity;
+        e = launch_batch_tiles(sc, desc, n_views, tile_prefix, tb, t.tile_samples, counters, fb,
                                stage_events ? static_cast<hipEvent_t>(stage_events[2]) : nullptr, stream);
         if (e != hipSuccess) return hip_fail(e, "batched tiles");
         e = launch_fill_u64_if(fb, L.fb_words, kEmpty, &counters->overflow, stream);

@@ -375,16 +375,30 @@ __device__ __forceinline__ void bin_direct(uint32_t gt, uint32_t g, uint32_t* ti
     else atomicAdd(&tile_count[gt], 1u);
 }
 
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t x)
+{
+    for (int off = 32; off > 0; off >>= 1) x = min(x, (uint32_t)__shfl_xor((int)x, off, 64));
+    return x;
+}
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t x)
+{
+    for (int off = 32; off > 0; off >>= 1) x = max(x, (uint32_t)__shfl_xor((int)x, off, 64));
+    return x;
+}
+
+// Items are (view, chunk rank): the fill pass takes each view's chunks nearest first (chunk_order), so
+// every tile's list comes out roughly front to back; the counting pass takes them in index order.
 template <bool kFill>
 __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, const ViewDesc* __restrict__ views,
                                                                 uint32_t n_views, uint32_t n_items,
+                                                                const uint32_t* __restrict__ chunk_order,
                                                                 uint32_t* __restrict__ tile_count,
                                                                 uint32_t* __restrict__ tile_cursor,
                                                                 const uint32_t* __restrict__ offsets,
                                                                 uint32_t* __restrict__ instances,
                                                                 BatchCounters* __restrict__ ctr)
 {
-    __shared__ uint32_t s_item, s_wx0, s_wx1, s_wy0, s_wy1;
+    __shared__ uint32_t s_wx0, s_wx1, s_wy0, s_wy1;
     __shared__ unsigned long long s_total;
     __shared__ uint32_t hist[kWindow];
     __shared__ uint32_t hbase[kWindow];
@@ -393,19 +407,19 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
     }
     const uint32_t tid = threadIdx.x, lane = lane_id();
     const uint32_t n_chunks = (sc.n + kBatchBlock - 1u) / kBatchBlock;
-    for (;;) {
+    if (tid == 0) s_total = 0ull;
+    // items are dealt round-robin: a shared claim counter would serialise ~n_items atomics
+    // on one address, and the per-item work is nearly uniform
+    for (uint32_t item = blockIdx.x; item < n_items; item += gridDim.x) {
         if (tid == 0) {
-            s_item = atomicAdd(kFill ? &ctr->next_fill_item : &ctr->next_count_item, 1u);
             s_wx0 = 0xffffffffu;
             s_wy0 = 0xffffffffu;
             s_wx1 = 0u;
             s_wy1 = 0u;
-            s_total = 0ull;
         }
         __syncthreads();
-        const uint32_t item = __builtin_amdgcn_readfirstlane(s_item);
-        if (item >= n_items) break;
-        const uint32_t v = item / n_chunks, chunk = item - v * n_chunks;
+        const uint32_t v = item / n_chunks;
+        const uint32_t chunk = kFill ? chunk_order[item] : item - v * n_chunks;
         const ViewDesc& vd = views[v];
         const uint32_t g = chunk * kBatchBlock + tid;
         TileRange tr;
@@ -413,14 +427,23 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
         const uint32_t tw = has ? tr.tx1 - tr.tx0 + 1u : 0u;
         const uint32_t n_t = has ? tw * (tr.ty1 - tr.ty0 + 1u) : 0u;
         const bool small = has && n_t <= kSmallTiles;
-        if (small) {
-            atomicMin(&s_wx0, tr.tx0);
-            atomicMax(&s_wx1, tr.tx1);
-            atomicMin(&s_wy0, tr.ty0);
-            atomicMax(&s_wy1, tr.ty1);
+        {
+            // window of the small Gaussians: wave reductions, one LDS atomic per wave
+            const uint32_t mx0 = wave_min_u32(small ? tr.tx0 : 0xffffffffu);
+            const uint32_t my0 = wave_min_u32(small ? tr.ty0 : 0xffffffffu);
+            const uint32_t mx1 = wave_max_u32(small ? tr.tx1 : 0u);
+            const uint32_t my1 = wave_max_u32(small ? tr.ty1 : 0u);
+            if (lane == 0u && mx0 <= mx1) {
+                atomicMin(&s_wx0, mx0);
+                atomicMax(&s_wx1, mx1);
+                atomicMin(&s_wy0, my0);
+                atomicMax(&s_wy1, my1);
+            }
         }
         if constexpr (!kFill) {
-            if (has) atomicAdd(&s_total, (unsigned long long)n_t);
+            uint32_t sum = n_t;  // a Gaussian touches < 2^24 tiles, so a wave's sum fits 32 bits
+            for (int off = 32; off > 0; off >>= 1) sum += (uint32_t)__shfl_xor((int)sum, off, 64);
+            if (lane == 0u && sum) atomicAdd(&s_total, (unsigned long long)sum);
         }
         // large Gaussians: the wave walks each one's tiles
         uint64_t big = __ballot(has && !small);
@@ -435,9 +458,6 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
             }
         }
         __syncthreads();
-        if constexpr (!kFill) {
-            if (tid == 0 && s_total) atomicAdd(&ctr->total_instances, s_total);
-        }
         const uint32_t wx0 = s_wx0, wx1 = s_wx1, wy0 = s_wy0, wy1 = s_wy1;
         if (wx0 <= wx1) {
             const uint32_t wt = wx1 - wx0 + 1u, ht = wy1 - wy0 + 1u;
@@ -479,6 +499,127 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
         }
         __syncthreads();
     }
+    if constexpr (!kFill) {
+        if (tid == 0 && s_total) atomicAdd(&ctr->total_instances, s_total);
+    }
+}
+
+// ---- per-view front-to-back order of the Gaussian chunks ----
+__global__ __launch_bounds__(kBatchBlock) void chunk_centroid_kernel(SceneArgs sc, float4* __restrict__ centroids)
+{
+    __shared__ float s_sum[kBatchBlock / 64u][4];
+    const uint32_t g = blockIdx.x * kBatchBlock + threadIdx.x;
+    float x = 0.0f, y = 0.0f, z = 0.0f, c = 0.0f;
+    if (g < sc.n) {
+        x = sc.means[3u * g + 0u];
+        y = sc.means[3u * g + 1u];
+        z = sc.means[3u * g + 2u];
+        c = 1.0f;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        x += __shfl_xor(x, off, 64);
+        y += __shfl_xor(y, off, 64);
+        z += __shfl_xor(z, off, 64);
+        c += __shfl_xor(c, off, 64);
+    }
+    const uint32_t wave = threadIdx.x / 64u;
+    if (lane_id() == 0u) {
+        s_sum[wave][0] = x;
+        s_sum[wave][1] = y;
+        s_sum[wave][2] = z;
+        s_sum[wave][3] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0u) {
+        float sx = 0.0f, sy = 0.0f, sz = 0.0f, sc_ = 0.0f;
+        for (uint32_t w = 0; w < kBatchBlock / 64u; w++) {
+            sx += s_sum[w][0];
+            sy += s_sum[w][1];
+            sz += s_sum[w][2];
+            sc_ += s_sum[w][3];
+        }
+        centroids[blockIdx.x] = make_float4(sx / sc_, sy / sc_, sz / sc_, 0.0f);
+    }
+}
+
+constexpr uint32_t kOrderThreads = 1024, kOrderBuckets = 4096;
+
+__device__ __forceinline__ uint32_t order_bucket(float z, float zmin, float scale)
+{
+    if (!(z == z)) return kOrderBuckets - 1u;
+    const float t = (z - zmin) * scale;
+    if (!(t > 0.0f)) return 0u;
+    return t >= (float)(kOrderBuckets - 1u) ? kOrderBuckets - 1u : (uint32_t)t;
+}
+
+// One workgroup per view: counting sort of the chunks by the view depth of their centroid.
+__global__ __launch_bounds__(kOrderThreads) void chunk_order_kernel(const ViewDesc* __restrict__ views,
+                                                                     const float4* __restrict__ centroids,
+                                                                     uint32_t n_chunks, uint32_t* __restrict__ order)
+{
+    __shared__ uint32_t hist[kOrderBuckets];
+    __shared__ float s_lo[kOrderThreads / 64u], s_hi[kOrderThreads / 64u];
+    __shared__ uint32_t s_wave[kOrderThreads / 64u];
+    const ViewDesc& vd = views[blockIdx.x];
+    const float4 r3 = vd.r3;
+    const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid / 64u;
+    float lo = 3.0e38f, hi = -3.0e38f;
+    for (uint32_t c = tid; c < n_chunks; c += kOrderThreads) {
+        const float4 m = centroids[c];
+        const float z = r3.x * m.x + r3.y * m.y + r3.z * m.z + r3.w;
+        if (z == z) {
+            lo = fminf(lo, z);
+            hi = fmaxf(hi, z);
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        lo = fminf(lo, __shfl_xor(lo, off, 64));
+        hi = fmaxf(hi, __shfl_xor(hi, off, 64));
+    }
+    if (lane == 0u) {
+        s_lo[wave] = lo;
+        s_hi[wave] = hi;
+    }
+    for (uint32_t i = tid; i < kOrderBuckets; i += kOrderThreads) hist[i] = 0u;
+    __syncthreads();
+    float zmin = 3.0e38f, zmax = -3.0e38f;
+    for (uint32_t w = 0; w < kOrderThreads / 64u; w++) {
+        zmin = fminf(zmin, s_lo[w]);
+        zmax = fmaxf(zmax, s_hi[w]);
+    }
+    const float scale = zmax > zmin ? (float)(kOrderBuckets - 1u) / (zmax - zmin) : 0.0f;
+    for (uint32_t c = tid; c < n_chunks; c += kOrderThreads) {
+        const float4 m = centroids[c];
+        atomicAdd(&hist[order_bucket(r3.x * m.x + r3.y * m.y + r3.z * m.z + r3.w, zmin, scale)], 1u);
+    }
+    __syncthreads();
+    // exclusive scan of the histogram, 4 buckets per thread
+    uint32_t v[4], local = 0;
+    for (int k = 0; k < 4; k++) {
+        v[k] = hist[tid * 4u + (uint32_t)k];
+        local += v[k];
+    }
+    uint32_t incl = local;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)incl, off, 64);
+        if ((int)lane >= off) incl += y;
+    }
+    if (lane == 63u) s_wave[wave] = incl;
+    __syncthreads();
+    uint32_t run = incl - local;
+    for (uint32_t w = 0; w < wave; w++) run += s_wave[w];
+    for (int k = 0; k < 4; k++) {
+        hist[tid * 4u + (uint32_t)k] = run;
+        run += v[k];
+    }
+    __syncthreads();
+    uint32_t* out = order + (size_t)blockIdx.x * n_chunks;
+    for (uint32_t c = tid; c < n_chunks; c += kOrderThreads) {
+        const float4 m = centroids[c];
+        // the bucket is recomputed with the same IEEE operations, so it matches the count
+        const uint32_t pos = atomicAdd(&hist[order_bucket(r3.x * m.x + r3.y * m.y + r3.z * m.z + r3.w, zmin, scale)], 1u);
+        if (pos < n_chunks) out[pos] = c;
+    }
 }
 
 // inclusive scan of one value per thread over the 256-thread workgroup
@@ -506,6 +647,28 @@ __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t x, int m)
 // The tile's pixels are 16 sub-tiles of 4x4; a wave holds rows 4w..4w+3, i.e. sub-tiles
 // (0..3, w). kSubtiles maxima bound the framebuffer words an instance can still lower.
 constexpr uint32_t kSub = 4, kSubtiles = (kTile / kSub) * (kTile / kSub);
+// A tile's instances are rasterised front to back in groups of kSortGroup, sorted in LDS by
+// a key of the view depth's float bits with the low kSortIndexBits replaced by the index in
+// the group. Order never changes a result (atomicMin); it lets the depth bounds reject
+// whole batches once the tile is covered.
+constexpr uint32_t kSortIndexBits = 10, kSortGroup = 1u << kSortIndexBits;
+
+// ascending bitonic sort of n (a power of two) keys in LDS by the workgroup
+__device__ __forceinline__ void bitonic_sort(uint32_t* s, uint32_t n)
+{
+    for (uint32_t k = 2; k <= n; k <<= 1)
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = threadIdx.x; i < n / 2u; i += kBatchBlock) {
+                const uint32_t lo = 2u * i - (i & (j - 1u)), hi = lo + j;
+                const uint32_t a = s[lo], b = s[hi];
+                if ((a > b) == ((lo & k) == 0u)) {
+                    s[lo] = b;
+                    s[hi] = a;
+                }
+            }
+            __syncthreads();
+        }
+}
 
 // kLayers: samples held in LDS per pass over the tile's instances
 template <uint32_t kLayers>
@@ -518,12 +681,15 @@ __global__ __launch_bounds__(kBatchBlock) void batch_tile_raster_kernel(SceneArg
                                                                         uint64_t* __restrict__ fb)
 {
     __shared__ unsigned long long layer[kLayers][kTile * kTile];
+    __shared__ uint32_t s_key[kSortGroup];
     __shared__ uint64_t s_frag[kBatchBlock];
     __shared__ uint64_t s_sub[kSubtiles];
     __shared__ float s_msx[kBatchBlock], s_msy[kBatchBlock], s_o[kBatchBlock];
     __shared__ float s_ca[kBatchBlock], s_cb[kBatchBlock], s_cc[kBatchBlock];
     __shared__ uint32_t s_rect[kBatchBlock], s_end[kBatchBlock];
     __shared__ uint32_t s_wave[kBatchBlock / 64u];
+    __shared__ uint32_t s_gmin[kBatchBlock / 64u];
+    __shared__ uint64_t s_gmax[kBatchBlock / 64u];
     if (ctr->overflow) return;
     const uint32_t gt = blockIdx.x;
     uint32_t lo = 0, hi = n_views - 1u;
@@ -548,103 +714,142 @@ __global__ __launch_bounds__(kBatchBlock) void batch_tile_raster_kernel(SceneArg
         // pixels outside the view hold 0 so they never keep an instance alive
         for (uint32_t s = 0; s < G; s++) layer[s][tid] = inside ? kEmpty : 0ull;
         const uint32_t fs_group = (vd.frame0 + s0) * 1973u;
-        for (uint32_t batch = begin; batch < end; batch += kBatchBlock) {
+        for (uint32_t sb = begin; sb < end; sb += kSortGroup) {
+            const uint32_t ns = min(kSortGroup, end - sb);
+            uint32_t n2 = 64u;
+            while (n2 < ns) n2 <<= 1;
             __syncthreads();
-            // sub-tile maxima of the words so far
+            uint32_t kmin = 0xffffffffu;
+            for (uint32_t i = tid; i < n2; i += kBatchBlock) {
+                uint32_t key = 0xffffffffu;
+                if (i < ns) {
+                    const uint32_t g = instances[sb + i];
+                    key = g < sc.n ? (f2u(view_depth(sc.means, g, vd.r3)) & ~(kSortGroup - 1u)) | i : 0xffffffffu;
+                }
+                s_key[i] = key;
+                kmin = min(kmin, key);
+            }
             {
                 uint64_t m = 0;
                 for (uint32_t s = 0; s < G; s++) m = max(m, (uint64_t)layer[s][tid]);
-                m = max(m, shfl_xor_u64(m, 1));
-                m = max(m, shfl_xor_u64(m, 2));
-                m = max(m, shfl_xor_u64(m, 16));
-                m = max(m, shfl_xor_u64(m, 32));
-                if ((lane & 51u) == 0u) s_sub[(tid / 64u) * (kTile / kSub) + (lane & 15u) / kSub] = m;
+                for (int off = 32; off > 0; off >>= 1) m = max(m, shfl_xor_u64(m, off));
+                kmin = wave_min_u32(kmin);
+                if (lane == 0u) {
+                    s_gmin[tid / 64u] = kmin;
+                    s_gmax[tid / 64u] = m;
+                }
             }
-            const uint32_t nb = min(kBatchBlock, end - batch);
-            uint32_t cnt = 0, rect = 0;
-            uint64_t frag = 0;
-            if (tid < nb) {
-                const uint32_t g = instances[batch + tid];
-                Projected p;
-                if (project_gaussian(sc.means, sc.scales, sc.rotations, sc.opacities, g, vd.r1, vd.r2, vd.r3, vd.cam,
-                                     p)) {
-                    const Aabb b = pixel_aabb(p, vd.cam);
-                    const uint32_t cx0 = max(b.x0, ox), cx1 = min(b.x1, ox + kTile);
-                    const uint32_t cy0 = max(b.y0, oy), cy1 = min(b.y1, oy + kTile);
-                    if (cx1 > cx0 && cy1 > cy0) {
-                        const uint32_t rw = cx1 - cx0, rh = cy1 - cy0;
-                        rect = (cx0 - ox) | ((cy0 - oy) << 4) | ((rw - 1u) << 8) | (((65536u + rw - 1u) / rw) << 12);
-                        cnt = rw * rh;
+            __syncthreads();
+            {
+                // a group whose nearest key lies behind every word of the tile has nothing to add
+                uint32_t gmin = 0xffffffffu;
+                uint64_t tmax = 0;
+                for (uint32_t w = 0; w < kBatchBlock / 64u; w++) {
+                    gmin = min(gmin, s_gmin[w]);
+                    tmax = max(tmax, s_gmax[w]);
+                }
+                if ((gmin & ~(kSortGroup - 1u)) > (uint32_t)(tmax >> 32)) continue;
+            }
+            bitonic_sort(s_key, n2);
+            for (uint32_t b0 = 0; b0 < ns; b0 += kBatchBlock) {
+                __syncthreads();
+                // sub-tile maxima of the words so far
+                {
+                    uint64_t m = 0;
+                    for (uint32_t s = 0; s < G; s++) m = max(m, (uint64_t)layer[s][tid]);
+                    m = max(m, shfl_xor_u64(m, 1));
+                    m = max(m, shfl_xor_u64(m, 2));
+                    m = max(m, shfl_xor_u64(m, 16));
+                    m = max(m, shfl_xor_u64(m, 32));
+                    if ((lane & 51u) == 0u) s_sub[(tid / 64u) * (kTile / kSub) + (lane & 15u) / kSub] = m;
+                }
+                __syncthreads();
+                // keys ascend: once the nearest remaining key lies behind every word of the
+                // tile, no later instance of the group can win a sample
+                uint64_t tile_max = 0;
+                for (uint32_t q = 0; q < kSubtiles; q++) tile_max = max(tile_max, s_sub[q]);
+                if ((s_key[b0] & ~(kSortGroup - 1u)) > (uint32_t)(tile_max >> 32)) break;
+                const uint32_t nb = min(kBatchBlock, ns - b0);
+                uint32_t cnt = 0, rect = 0;
+                if (tid < nb) {
+                    const uint32_t idx = s_key[b0 + tid] & (kSortGroup - 1u);
+                    const uint32_t g = idx < ns ? instances[sb + idx] : 0xffffffffu;
+                    Projected p;
+                    if (g < sc.n && project_gaussian(sc.means, sc.scales, sc.rotations, sc.opacities, g, vd.r1, vd.r2,
+                                                     vd.r3, vd.cam, p)) {
+                        const Aabb b = pixel_aabb(p, vd.cam);
+                        const uint32_t cx0 = max(b.x0, ox), cx1 = min(b.x1, ox + kTile);
+                        const uint32_t cy0 = max(b.y0, oy), cy1 = min(b.y1, oy + kTile);
+                        const uint64_t frag = fragment_key(p.depth, g);
+                        if (cx1 > cx0 && cy1 > cy0) {
+                            const uint32_t rw = cx1 - cx0, rh = cy1 - cy0;
+                            rect = (cx0 - ox) | ((cy0 - oy) << 4) | ((rw - 1u) << 8) |
+                                   (((65536u + rw - 1u) / rw) << 12);
+                            // an instance that no word of its sub-tiles exceeds cannot win
+                            uint64_t m = 0;
+                            for (uint32_t sy = (cy0 - oy) / kSub; sy <= (cy1 - 1u - oy) / kSub; sy++)
+                                for (uint32_t sx = (cx0 - ox) / kSub; sx <= (cx1 - 1u - ox) / kSub; sx++)
+                                    m = max(m, s_sub[sy * (kTile / kSub) + sx]);
+                            if (frag < m) cnt = rw * rh;
+                        }
+                        const ConicExp2 ce = conic_exp2(p.conic_x, p.conic_y, p.conic_z);
+                        s_frag[tid] = frag;
+                        s_msx[tid] = p.mean_x - 0.5f;
+                        s_msy[tid] = p.mean_y - 0.5f;
+                        s_o[tid] = p.opacity;
+                        s_ca[tid] = ce.a;
+                        s_cb[tid] = ce.b;
+                        s_cc[tid] = ce.c;
                     }
-                    frag = fragment_key(p.depth, g);
-                    const ConicExp2 ce = conic_exp2(p.conic_x, p.conic_y, p.conic_z);
-                    s_frag[tid] = frag;
-                    s_msx[tid] = p.mean_x - 0.5f;
-                    s_msy[tid] = p.mean_y - 0.5f;
-                    s_o[tid] = p.opacity;
-                    s_ca[tid] = ce.a;
-                    s_cb[tid] = ce.b;
-                    s_cc[tid] = ce.c;
                 }
                 s_rect[tid] = rect;
-            }
-            __syncthreads();
-            if (cnt) {
-                // an instance that no word of its sub-tiles exceeds cannot win any sample
-                const uint32_t rx0 = rect & 0xfu, ry0 = (rect >> 4) & 0xfu, rw = ((rect >> 8) & 0xfu) + 1u;
-                const uint32_t rh = cnt / rw;
-                uint64_t m = 0;
-                for (uint32_t sy = ry0 / kSub; sy <= (ry0 + rh - 1u) / kSub; sy++)
-                    for (uint32_t sx = rx0 / kSub; sx <= (rx0 + rw - 1u) / kSub; sx++)
-                        m = max(m, s_sub[sy * (kTile / kSub) + sx]);
-                if (!(frag < m)) cnt = 0;
-            }
-            // fragments of the batch's live instances, flattened over the workgroup's lanes
-            const uint32_t incl = block_inclusive_scan(cnt, s_wave);
-            s_end[tid] = incl;
-            __syncthreads();
-            const uint32_t total = s_end[kBatchBlock - 1u];
-            for (uint32_t k = tid; k < total; k += kBatchBlock) {
-                uint32_t a = 0, b = kBatchBlock - 1u;
-                while (a < b) {
-                    const uint32_t m = (a + b) >> 1;
-                    if (s_end[m] > k) b = m;
-                    else a = m + 1u;
-                }
-                const uint32_t j = a;
-                const uint32_t f = k - (j ? s_end[j - 1u] : 0u);
-                const uint32_t rect_j = s_rect[j];
-                const uint32_t rw = ((rect_j >> 8) & 0xfu) + 1u;
-                const uint32_t q = (f * (rect_j >> 12)) >> 16, r = f - q * rw;  // f / rw for f < 256, rw <= 16
-                const uint32_t lx = (rect_j & 0xfu) + r, ly = ((rect_j >> 4) & 0xfu) + q;
-                const uint32_t l = ly * kTile + lx;
-                const uint64_t fj = s_frag[j];
-                // samples whose word this fragment is still below (words only decrease, so a
-                // stale read never drops a winner)
-                uint32_t live = 0;
-                for (uint32_t s = 0; s < G; s += 4u) {
-                    uint64_t w4[4];
+                // fragments of the batch's live instances, flattened over the workgroup's lanes
+                const uint32_t incl = block_inclusive_scan(cnt, s_wave);
+                s_end[tid] = incl;
+                __syncthreads();
+                const uint32_t total = s_end[kBatchBlock - 1u];
+                for (uint32_t k = tid; k < total; k += kBatchBlock) {
+                    uint32_t a = 0, bb = kBatchBlock - 1u;
+                    while (a < bb) {
+                        const uint32_t m = (a + bb) >> 1;
+                        if (s_end[m] > k) bb = m;
+                        else a = m + 1u;
+                    }
+                    const uint32_t j = a;
+                    const uint32_t f = k - (j ? s_end[j - 1u] : 0u);
+                    const uint32_t rect_j = s_rect[j];
+                    const uint32_t rw = ((rect_j >> 8) & 0xfu) + 1u;
+                    const uint32_t q = (f * (rect_j >> 12)) >> 16, r = f - q * rw;  // f / rw for f < 256, rw <= 16
+                    const uint32_t lx = (rect_j & 0xfu) + r, ly = ((rect_j >> 4) & 0xfu) + q;
+                    const uint32_t l = ly * kTile + lx;
+                    const uint64_t fj = s_frag[j];
+                    // samples whose word this fragment is still below (words only decrease, so
+                    // a stale read never drops a winner)
+                    uint32_t live = 0;
+                    for (uint32_t s = 0; s < G; s += 4u) {
+                        uint64_t w4[4];
 #pragma unroll
-                    for (uint32_t i = 0; i < 4u; i++)
-                        w4[i] = s + i < G ? __hip_atomic_load(&layer[s + i][l], __ATOMIC_RELAXED,
-                                                              __HIP_MEMORY_SCOPE_WORKGROUP)
-                                          : 0ull;
+                        for (uint32_t i = 0; i < 4u; i++)
+                            w4[i] = s + i < G ? __hip_atomic_load(&layer[s + i][l], __ATOMIC_RELAXED,
+                                                                  __HIP_MEMORY_SCOPE_WORKGROUP)
+                                              : 0ull;
 #pragma unroll
-                    for (uint32_t i = 0; i < 4u; i++)
-                        if (fj < w4[i]) live |= 1u << (s + i);
-                }
-                if (!live) continue;
-                const uint32_t px = ox + lx, py = oy + ly;
-                const ConicExp2 ce = { s_ca[j], s_cb[j], s_cc[j] };
-                const float alpha = fragment_alpha(ce, s_o[j], (float)px - s_msx[j], (float)py - s_msy[j]);
-                if (alpha < kMinAlpha) continue;
-                const uint32_t T = alpha_threshold(alpha);
-                const uint32_t base = (uint32_t)fj ^ (px * 9277u) ^ (py * 2699u);
-                while (live) {
-                    const uint32_t s = (uint32_t)__builtin_ctz(live);
-                    live &= live - 1u;
-                    const uint32_t h = pcg_hash(base ^ (fs_group + s * 1973u));
-                    if ((h >> 9u) < T) atomicMin(&layer[s][l], (unsigned long long)fj);
+                        for (uint32_t i = 0; i < 4u; i++)
+                            if (fj < w4[i]) live |= 1u << (s + i);
+                    }
+                    if (!live) continue;
+                    const uint32_t px = ox + lx, py = oy + ly;
+                    const ConicExp2 ce = { s_ca[j], s_cb[j], s_cc[j] };
+                    const float alpha = fragment_alpha(ce, s_o[j], (float)px - s_msx[j], (float)py - s_msy[j]);
+                    if (alpha < kMinAlpha) continue;
+                    const uint32_t T = alpha_threshold(alpha);
+                    const uint32_t base = (uint32_t)fj ^ (px * 9277u) ^ (py * 2699u);
+                    while (live) {
+                        const uint32_t s = (uint32_t)__builtin_ctz(live);
+                        live &= live - 1u;
+                        const uint32_t h = pcg_hash(base ^ (fs_group + s * 1973u));
+                        if ((h >> 9u) < T) atomicMin(&layer[s][l], (unsigned long long)fj);
+                    }
                 }
             }
         }
@@ -656,11 +861,10 @@ __global__ __launch_bounds__(kBatchBlock) void batch_tile_raster_kernel(SceneArg
 }
 
 hipError_t launch_batch_tiles(const SceneArgs& sc, const ViewDesc* views, uint32_t n_views, const uint32_t* tile_prefix,
-                              uint32_t n_tiles, uint32_t* tile_count, uint32_t* tile_cursor, uint32_t* offsets,
-                              uint32_t* instances, uint32_t capacity, uint32_t layers, BatchCounters* counters,
-                              uint64_t* fb, hipEvent_t raster_event, hipStream_t stream)
+                              const TileBuffers& tb, uint32_t layers, BatchCounters* counters, uint64_t* fb,
+                              hipEvent_t raster_event, hipStream_t stream)
 {
-    if (n_views == 0 || n_tiles == 0) return hipSuccess;
+    if (n_views == 0 || tb.n_tiles == 0) return hipSuccess;
     const uint32_t n_chunks = (sc.n + kBatchBlock - 1u) / kBatchBlock;
     const uint64_t n_items64 = (uint64_t)n_chunks * n_views;
     if (n_items64 >= 0xffffffffull) return hipErrorInvalidValue;
@@ -676,26 +880,31 @@ hipError_t launch_batch_tiles(const SceneArgs& sc, const ViewDesc* views, uint32
     if (grid > n_items64) grid = n_items64;
     if (grid < 1) grid = 1;
     if (sc.n > 0) {
+        hipLaunchKernelGGL(chunk_centroid_kernel, dim3(n_chunks), dim3(kBatchBlock), 0, stream, sc, tb.centroids);
+        hipLaunchKernelGGL(chunk_order_kernel, dim3(n_views), dim3(kOrderThreads), 0, stream, views, tb.centroids,
+                           n_chunks, tb.chunk_order);
         hipLaunchKernelGGL((batch_bin_kernel<false>), dim3((unsigned)grid), dim3(kBatchBlock), 0, stream, sc, views,
-                           n_views, (uint32_t)n_items64, tile_count, tile_cursor, offsets, instances, counters);
+                           n_views, (uint32_t)n_items64, tb.chunk_order, tb.tile_count, tb.tile_cursor, tb.offsets,
+                           tb.instances, counters);
     }
     hipLaunchKernelGGL((exclusive_scan_kernel<CountInput>), dim3(1), dim3(kScanThreads), 0, stream,
-                       CountInput{ tile_count }, offsets, nullptr, n_tiles, n_tiles, &counters->n_instances,
-                       &counters->overflow, capacity, &counters->total_instances);
+                       CountInput{ tb.tile_count }, tb.offsets, nullptr, tb.n_tiles, tb.n_tiles, &counters->n_instances,
+                       &counters->overflow, tb.capacity, &counters->total_instances);
     if (sc.n > 0) {
         hipLaunchKernelGGL((batch_bin_kernel<true>), dim3((unsigned)grid), dim3(kBatchBlock), 0, stream, sc, views,
-                           n_views, (uint32_t)n_items64, tile_count, tile_cursor, offsets, instances, counters);
+                           n_views, (uint32_t)n_items64, tb.chunk_order, tb.tile_count, tb.tile_cursor, tb.offsets,
+                           tb.instances, counters);
     }
     if (raster_event) {
         e = hipEventRecord(raster_event, stream);
         if (e != hipSuccess) return e;
     }
     if (layers == 8u)
-        hipLaunchKernelGGL(batch_tile_raster_kernel<8>, dim3(n_tiles), dim3(kBatchBlock), 0, stream, sc, views,
-                           n_views, tile_prefix, offsets, instances, counters, fb);
+        hipLaunchKernelGGL(batch_tile_raster_kernel<8>, dim3(tb.n_tiles), dim3(kBatchBlock), 0, stream, sc, views,
+                           n_views, tile_prefix, tb.offsets, tb.instances, counters, fb);
     else
-        hipLaunchKernelGGL(batch_tile_raster_kernel<16>, dim3(n_tiles), dim3(kBatchBlock), 0, stream, sc, views,
-                           n_views, tile_prefix, offsets, instances, counters, fb);
+        hipLaunchKernelGGL(batch_tile_raster_kernel<16>, dim3(tb.n_tiles), dim3(kBatchBlock), 0, stream, sc, views,
+                           n_views, tile_prefix, tb.offsets, tb.instances, counters, fb);
     return hipGetLastError();
 }
 

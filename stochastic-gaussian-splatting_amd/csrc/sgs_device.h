@@ -301,6 +301,13 @@ struct Projected {
     float bx0, bx1, by0, by1;
 };
 
+// view-space depth of Gaussian g (the projection's depth, bit for bit)
+__device__ __forceinline__ float view_depth(const float* __restrict__ means, uint32_t g, const float4 r3)
+{
+    const float mx = means[3u * g + 0u], my = means[3u * g + 1u], mz = means[3u * g + 2u];
+    return r3.x * mx + r3.y * my + r3.z * mz + r3.w;
+}
+
 // EWA projection with culling (gs_utils.h:38-193); returns false if culled.
 // w2c rows r1, r2, r3 (row-major, 4 floats each).
 __device__ __forceinline__ bool project_gaussian(const float* __restrict__ means, const float* __restrict__ scales,
@@ -310,7 +317,7 @@ __device__ __forceinline__ bool project_gaussian(const float* __restrict__ means
                                                  const FbCamera& cam, Projected& o)
 {
     const float mx = means[3u * g + 0u], my = means[3u * g + 1u], mz = means[3u * g + 2u];
-    const float depth = r3.x * mx + r3.y * my + r3.z * mz + r3.w;
+    const float depth = view_depth(means, g, r3);
     if (!(depth >= cam.near_plane && depth <= cam.far_plane)) return false;
     float opacity = 1.0f / (1.0f + exp_f(-opacities[g]));
     if (opacity < kMinAlpha) return false;

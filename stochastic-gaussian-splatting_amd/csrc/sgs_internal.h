@@ -111,12 +111,10 @@ struct BatchCounters {
     uint32_t done_items;        // phase A items finished (release after their queue writes)
     uint32_t produced;          // queue slots reserved
     uint32_t consumed;          // queue slots claimed by phase B waves
-    uint32_t next_count_item;   // tile binning, counting pass
-    uint32_t next_fill_item;    // tile binning, fill pass
     uint32_t n_instances;       // (view, tile, Gaussian) instances of the tile schedule
     uint32_t overflow;          // instances exceed the capacity: the scatter schedule runs instead
     unsigned long long total_instances;  // 64-bit instance total of the counting pass
-    uint32_t pad[54];
+    uint32_t pad[56];
 };
 static_assert(sizeof(BatchCounters) == 256, "BatchCounters layout");
 
@@ -174,10 +172,18 @@ hipError_t launch_batch_accumulate(const SceneArgs& sc, const ViewDesc* views, u
 // tile schedule: binning (count, scan, fill) and per-tile LDS rasterisation of every view's
 // samples; the scatter schedule above runs instead (gated on counters->overflow) when the
 // instance list overflows
+struct TileBuffers {
+    uint32_t* tile_count;       // [n_tiles] instances per tile (zeroed by the caller)
+    uint32_t* tile_cursor;      // [n_tiles] fill cursors (zeroed by the caller)
+    uint32_t* offsets;          // [n_tiles + 1] exclusive scan of tile_count
+    uint32_t* instances;        // [capacity] Gaussian index of each (view, tile, Gaussian) instance
+    float4* centroids;          // [n_chunks] mean position of each chunk of 256 Gaussians
+    uint32_t* chunk_order;      // [n_views * n_chunks] chunks of each view, nearest first
+    uint32_t n_tiles, capacity;
+};
 hipError_t launch_batch_tiles(const SceneArgs& sc, const ViewDesc* views, uint32_t n_views, const uint32_t* tile_prefix,
-                              uint32_t n_tiles, uint32_t* tile_count, uint32_t* tile_cursor, uint32_t* offsets,
-                              uint32_t* instances, uint32_t capacity, uint32_t layers, BatchCounters* counters,
-                              uint64_t* fb, hipEvent_t raster_event, hipStream_t stream);
+                              const TileBuffers& tb, uint32_t layers, BatchCounters* counters, uint64_t* fb,
+                              hipEvent_t raster_event, hipStream_t stream);
 hipError_t launch_fill_u64_if(uint64_t* p, size_t n, uint64_t value, const uint32_t* run_if_set, hipStream_t stream);
 
 // ---- launchers (sgs_hybrid.hip) ----
