@@ -44,7 +44,7 @@ def parse() -> argparse.Namespace:
                     help="run only one path (profiling); the JSON line then reports that path")
     ap.add_argument("--schedule", choices=["tiles", "scatter"], default="tiles",
                     help="batched accumulation: LDS tiles (default) or persistent scatter")
-    ap.add_argument("--tile-samples", type=int, choices=[8, 16], default=16,
+    ap.add_argument("--tile-samples", type=int, choices=[8, 16], default=8,
                     help="samples per LDS pass of the tile rasteriser")
     ap.add_argument("--view-major", action="store_true",
                     help="batched work items view by view instead of Gaussian chunk by chunk")

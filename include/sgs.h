@@ -180,7 +180,8 @@ typedef struct sgs_batch_tuning {
     uint32_t max_tile_instances;      /* capacity of the tile schedule's (view, tile, Gaussian)
                                          list (8 per Gaussian and view, within [2^16, 2^31]);
                                          a batch that needs more runs the scatter schedule */
-    uint32_t tile_samples;            /* samples held in LDS per pass over a tile (8 or 16) */
+    uint32_t tile_samples;            /* samples held in LDS per pass over a tile (8 or 16;
+                                         default 8) */
 } sgs_batch_tuning;
 
 /* SGS_BATCH_TILES: every (view, Gaussian) is binned into the 16x16 framebuffer tiles its

@@ -323,7 +323,7 @@ BatchTuning resolve_batch_tuning(const sgs_batch_tuning* t)
     r.blocks_per_cu = t ? t->blocks_per_cu : 0u;
     r.schedule = (t && t->schedule == SGS_BATCH_SCATTER) ? 1u : 0u;
     r.max_tile_instances = t ? t->max_tile_instances : 0u;
-    r.tile_samples = (t && t->tile_samples == 8u) ? 8u : 16u;
+    r.tile_samples = (t && t->tile_samples == 16u) ? 16u : 8u;
     return r;
 }
 
