@@ -178,8 +178,9 @@ typedef struct sgs_batch_tuning {
     uint32_t blocks_per_cu;           /* persistent workgroups per CU (0: occupancy limit) */
     uint32_t schedule;                /* SGS_BATCH_TILES (default) or SGS_BATCH_SCATTER */
     uint32_t max_tile_instances;      /* capacity of the tile schedule's (view, tile, Gaussian)
-                                         list (8 per Gaussian and view, within [2^16, 2^31]);
-                                         a batch that needs more runs the scatter schedule */
+                                         list, 16 bytes of workspace each (default 4 per
+                                         Gaussian and view, within [2^16, 2^31]); a batch that
+                                         needs more runs the scatter schedule */
     uint32_t tile_samples;            /* samples held in LDS per pass over a tile (8 or 16;
                                          default 8) */
 } sgs_batch_tuning;

@@ -327,10 +327,11 @@ BatchTuning resolve_batch_tuning(const sgs_batch_tuning* t)
     return r;
 }
 
-// default capacity of the tile schedule's instance list: 8 per (Gaussian, view) pair
+// default capacity of the tile schedule's instance list (16 bytes of workspace each): 4 per
+// (Gaussian, view) pair
 uint32_t default_tile_instances(uint32_t n_gaussians, uint32_t n_views)
 {
-    const uint64_t want = 8ull * n_gaussians * n_views;
+    const uint64_t want = 4ull * n_gaussians * n_views;
     return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(want, 1ull << 16), 1ull << 31);
 }
 
@@ -354,8 +355,8 @@ uint32_t view_grid_h(const sgs_view& v) { return (uint32_t)((v.height * view_ss(
 // Workspace of a batch. The counters, the queue's ready flags and the tile counts and
 // cursors are contiguous so one fill clears them all.
 struct BatchLayout {
-    size_t counters, ready, tile_count, tile_cursor, cleared_end, offsets, instances, centroids, chunk_order, queue, table,
-        fb, samples, total;
+    size_t counters, ready, tile_count, tile_cursor, cleared_end, offsets, instances, keys, sorted_instances, sorted_keys,
+        centroids, chunk_order, queue, table, fb, samples, total;
     uint64_t fb_words, n_samples, n_tiles;
     uint32_t n_blocks, capacity;
 };
@@ -390,6 +391,9 @@ BatchLayout batch_layout(const sgs_view* views, uint32_t n, uint32_t n_gaussians
     a.offset = (a.offset + 255u) & ~size_t(255u);
     L.offsets = a.take(4u * (tiles_n + 1u));
     L.instances = a.take(4u * (size_t)L.capacity);
+    L.keys = a.take(4u * (size_t)L.capacity);
+    L.sorted_instances = a.take(4u * (size_t)L.capacity);
+    L.sorted_keys = a.take(4u * (size_t)L.capacity);
     const size_t n_chunks = tiles ? ((size_t)n_gaussians + 255u) / 256u : 0u;
     L.centroids = a.take(16u * n_chunks);
     L.chunk_order = a.take(4u * n_chunks * n);
@@ -564,6 +568,9 @@ int sgs_render_batch(const sgs_gaussians* gaussians, const sgs_view* views, uint
         tb.tile_cursor = reinterpret_cast<uint32_t*>(ws + L.tile_cursor);
         tb.offsets = reinterpret_cast<uint32_t*>(ws + L.offsets);
         tb.instances = reinterpret_cast<uint32_t*>(ws + L.instances);
+        tb.keys = reinterpret_cast<uint32_t*>(ws + L.keys);
+        tb.sorted_instances = reinterpret_cast<uint32_t*>(ws + L.sorted_instances);
+        tb.sorted_keys = reinterpret_cast<uint32_t*>(ws + L.sorted_keys);
         tb.centroids = reinterpret_cast<float4*>(ws + L.centroids);
         tb.chunk_order = reinterpret_cast<uint32_t*>(ws + L.chunk_order);
         tb.n_tiles = (uint32_t)L.n_tiles;

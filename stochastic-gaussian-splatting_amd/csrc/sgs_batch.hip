@@ -346,13 +346,16 @@ struct TileRange {
     uint32_t tx0, tx1, ty0, ty1;  // inclusive
 };
 
-// tile range of Gaussian g in view vd (false if culled or its AABB is empty)
-__device__ __forceinline__ bool gaussian_tiles(const SceneArgs& sc, const ViewDesc& vd, uint32_t g, TileRange& t)
+// tile range and depth key (float bits of the view depth) of Gaussian g in view vd (false
+// if culled or its AABB is empty)
+__device__ __forceinline__ bool gaussian_tiles(const SceneArgs& sc, const ViewDesc& vd, uint32_t g, TileRange& t,
+                                               uint32_t& key)
 {
     Projected p;
     if (!(g < sc.n && project_gaussian(sc.means, sc.scales, sc.rotations, sc.opacities, g, vd.r1, vd.r2, vd.r3, vd.cam,
                                        p)))
         return false;
+    key = f2u(p.depth);
     const Aabb b = pixel_aabb(p, vd.cam);
     if (b.x1 <= b.x0 || b.y1 <= b.y0) return false;
     t.tx0 = b.x0 / kTile;
@@ -368,11 +371,17 @@ __device__ __forceinline__ bool gaussian_tiles(const SceneArgs& sc, const ViewDe
 constexpr uint32_t kSmallTiles = 32;
 
 template <bool kFill>
-__device__ __forceinline__ void bin_direct(uint32_t gt, uint32_t g, uint32_t* tile_count, uint32_t* tile_cursor,
-                                           const uint32_t* offsets, uint32_t* instances)
+__device__ __forceinline__ void bin_direct(uint32_t gt, uint32_t g, uint32_t key, uint32_t* tile_count,
+                                           uint32_t* tile_cursor, const uint32_t* offsets, uint32_t* instances,
+                                           uint32_t* keys)
 {
-    if constexpr (kFill) instances[offsets[gt] + atomicAdd(&tile_cursor[gt], 1u)] = g;
-    else atomicAdd(&tile_count[gt], 1u);
+    if constexpr (kFill) {
+        const uint32_t slot = offsets[gt] + atomicAdd(&tile_cursor[gt], 1u);
+        instances[slot] = g;
+        keys[slot] = key;
+    } else {
+        atomicAdd(&tile_count[gt], 1u);
+    }
 }
 
 __device__ __forceinline__ uint32_t wave_min_u32(uint32_t x)
@@ -396,6 +405,7 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
                                                                 uint32_t* __restrict__ tile_cursor,
                                                                 const uint32_t* __restrict__ offsets,
                                                                 uint32_t* __restrict__ instances,
+                                                                uint32_t* __restrict__ keys,
                                                                 BatchCounters* __restrict__ ctr)
 {
     __shared__ uint32_t s_wx0, s_wx1, s_wy0, s_wy1;
@@ -423,7 +433,8 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
         const ViewDesc& vd = views[v];
         const uint32_t g = chunk * kBatchBlock + tid;
         TileRange tr;
-        const bool has = gaussian_tiles(sc, vd, g, tr);
+        uint32_t key = 0;
+        const bool has = gaussian_tiles(sc, vd, g, tr, key);
         const uint32_t tw = has ? tr.tx1 - tr.tx0 + 1u : 0u;
         const uint32_t n_t = has ? tw * (tr.ty1 - tr.ty0 + 1u) : 0u;
         const bool small = has && n_t <= kSmallTiles;
@@ -451,10 +462,11 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
             const int l = __builtin_ctzll(big);
             big &= big - 1ull;
             const uint32_t bx0 = bcast(tr.tx0, l), by0 = bcast(tr.ty0, l), bw = bcast(tw, l), bn = bcast(n_t, l);
-            const uint32_t bg = bcast(g, l);
+            const uint32_t bg = bcast(g, l), bkey = bcast(key, l);
             for (uint32_t k = lane; k < bn; k += 64u) {
                 const uint32_t ty = by0 + k / bw, tx = bx0 + k % bw;
-                bin_direct<kFill>(vd.tile_begin + ty * vd.grid_w + tx, bg, tile_count, tile_cursor, offsets, instances);
+                bin_direct<kFill>(vd.tile_begin + ty * vd.grid_w + tx, bg, bkey, tile_count, tile_cursor, offsets,
+                                  instances, keys);
             }
         }
         __syncthreads();
@@ -486,15 +498,17 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
                         for (uint32_t ty = tr.ty0; ty <= tr.ty1; ty++)
                             for (uint32_t tx = tr.tx0; tx <= tr.tx1; tx++) {
                                 const uint32_t i = (ty - wy0) * wt + tx - wx0;
-                                instances[hbase[i] + atomicAdd(&hist[i], 1u)] = g;
+                                const uint32_t slot = hbase[i] + atomicAdd(&hist[i], 1u);
+                                instances[slot] = g;
+                                keys[slot] = key;
                             }
                 }
             } else if (small) {
                 // spread-out chunk: one global atomic per instance
                 for (uint32_t ty = tr.ty0; ty <= tr.ty1; ty++)
                     for (uint32_t tx = tr.tx0; tx <= tr.tx1; tx++)
-                        bin_direct<kFill>(vd.tile_begin + ty * vd.grid_w + tx, g, tile_count, tile_cursor, offsets,
-                                          instances);
+                        bin_direct<kFill>(vd.tile_begin + ty * vd.grid_w + tx, g, key, tile_count, tile_cursor,
+                                          offsets, instances, keys);
             }
         }
         __syncthreads();
@@ -622,6 +636,81 @@ __global__ __launch_bounds__(kOrderThreads) void chunk_order_kernel(const ViewDe
     }
 }
 
+// ---- per-tile counting sort by depth ----
+// One workgroup per tile: the tile's keys are binned into up to kSortBins buckets over the
+// tile's own key range (the top bits of key - min), and the instances are written bucket by
+// bucket, nearest first. keys_out holds each instance's bucket floor (min + bucket << shift),
+// a non-decreasing lower bound of the true keys from that position on.
+constexpr uint32_t kSortBins = 1024;
+
+__global__ __launch_bounds__(kBatchBlock) void tile_sort_kernel(const uint32_t* __restrict__ offsets,
+                                                                const uint32_t* __restrict__ keys_in,
+                                                                const uint32_t* __restrict__ inst_in,
+                                                                uint32_t* __restrict__ keys_out,
+                                                                uint32_t* __restrict__ inst_out,
+                                                                const BatchCounters* __restrict__ ctr)
+{
+    __shared__ uint32_t hist[kSortBins];
+    __shared__ uint32_t s_lo[kBatchBlock / 64u], s_hi[kBatchBlock / 64u], s_wave[kBatchBlock / 64u];
+    if (ctr->overflow) return;
+    const uint32_t begin = offsets[blockIdx.x], end = offsets[blockIdx.x + 1u];
+    if (end <= begin) return;
+    const uint32_t n = end - begin, tid = threadIdx.x, lane = lane_id(), wave = tid / 64u;
+    uint32_t lo = 0xffffffffu, hi = 0u;
+    for (uint32_t i = tid; i < n; i += kBatchBlock) {
+        const uint32_t k = keys_in[begin + i];
+        lo = min(lo, k);
+        hi = max(hi, k);
+    }
+    lo = wave_min_u32(lo);
+    hi = wave_max_u32(hi);
+    if (lane == 0u) {
+        s_lo[wave] = lo;
+        s_hi[wave] = hi;
+    }
+    for (uint32_t i = tid; i < kSortBins; i += kBatchBlock) hist[i] = 0u;
+    __syncthreads();
+    uint32_t kmin = 0xffffffffu, kmax = 0u;
+    for (uint32_t w = 0; w < kBatchBlock / 64u; w++) {
+        kmin = min(kmin, s_lo[w]);
+        kmax = max(kmax, s_hi[w]);
+    }
+    const uint32_t range = kmax - kmin;
+    const uint32_t bits = range ? 32u - (uint32_t)__clz(range) : 0u;
+    const uint32_t shift = bits > 10u ? bits - 10u : 0u;  // (range >> shift) < 1024
+    for (uint32_t i = tid; i < n; i += kBatchBlock) atomicAdd(&hist[(keys_in[begin + i] - kmin) >> shift], 1u);
+    __syncthreads();
+    // exclusive scan of the buckets, 4 per thread
+    uint32_t v[4], local = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < 4u; q++) {
+        v[q] = hist[tid * 4u + q];
+        local += v[q];
+    }
+    uint32_t incl = local;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)incl, off, 64);
+        if ((int)lane >= off) incl += y;
+    }
+    if (lane == 63u) s_wave[wave] = incl;
+    __syncthreads();
+    uint32_t run = incl - local;
+    for (uint32_t w = 0; w < wave; w++) run += s_wave[w];
+#pragma unroll
+    for (uint32_t q = 0; q < 4u; q++) {
+        hist[tid * 4u + q] = run;
+        run += v[q];
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < n; i += kBatchBlock) {
+        const uint32_t bucket = (keys_in[begin + i] - kmin) >> shift;
+        const uint32_t slot = begin + atomicAdd(&hist[bucket], 1u);
+        inst_out[slot] = inst_in[begin + i];
+        keys_out[slot] = kmin + (bucket << shift);
+    }
+}
+
 // inclusive scan of one value per thread over the 256-thread workgroup
 __device__ __forceinline__ uint32_t block_inclusive_scan(uint32_t x, uint32_t* wave_sums)
 {
@@ -647,49 +736,65 @@ __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t x, int m)
 // The tile's pixels are 16 sub-tiles of 4x4; a wave holds rows 4w..4w+3, i.e. sub-tiles
 // (0..3, w). kSubtiles maxima bound the framebuffer words an instance can still lower.
 constexpr uint32_t kSub = 4, kSubtiles = (kTile / kSub) * (kTile / kSub);
-// A tile's instances are rasterised front to back in groups of kSortGroup, sorted in LDS by
-// a key of the view depth's float bits with the low kSortIndexBits replaced by the index in
-// the group. Order never changes a result (atomicMin); it lets the depth bounds reject
-// whole batches once the tile is covered.
-constexpr uint32_t kSortIndexBits = 10, kSortGroup = 1u << kSortIndexBits;
-
-// ascending bitonic sort of n (a power of two) keys in LDS by the workgroup
-__device__ __forceinline__ void bitonic_sort(uint32_t* s, uint32_t n)
+// One queued fragment (instance j << 8 | tile pixel l): alpha once, then every sample word of
+// the pixel the fragment still lies in front of gets its hash test and, on acceptance, the
+// LDS atomicMin. All kLayers words are read (rows at and beyond G are masked off).
+template <uint32_t kLayers>
+__device__ __forceinline__ void tile_fragment(uint32_t entry, unsigned long long (*layer)[kTile * kTile],
+                                              const uint64_t* s_frag, const float* s_msx, const float* s_msy,
+                                              const float* s_o, const float* s_ca, const float* s_cb,
+                                              const float* s_cc, uint32_t ox, uint32_t oy, uint32_t G,
+                                              uint32_t fs_group)
 {
-    for (uint32_t k = 2; k <= n; k <<= 1)
-        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-            for (uint32_t i = threadIdx.x; i < n / 2u; i += kBatchBlock) {
-                const uint32_t lo = 2u * i - (i & (j - 1u)), hi = lo + j;
-                const uint32_t a = s[lo], b = s[hi];
-                if ((a > b) == ((lo & k) == 0u)) {
-                    s[lo] = b;
-                    s[hi] = a;
-                }
-            }
-            __syncthreads();
-        }
+    const uint32_t j = entry >> 8, l = entry & 0xffu;
+    const uint32_t px = ox + (l & 0xfu), py = oy + (l >> 4);
+    const ConicExp2 ce = { s_ca[j], s_cb[j], s_cc[j] };
+    const float alpha = fragment_alpha(ce, s_o[j], (float)px - s_msx[j], (float)py - s_msy[j]);
+    if (alpha < kMinAlpha) return;
+    const uint64_t fj = s_frag[j];
+    uint32_t live = 0;
+#pragma unroll
+    for (uint32_t s0 = 0; s0 < kLayers; s0 += 4u) {
+        uint64_t w[4];
+#pragma unroll
+        for (uint32_t i = 0; i < 4u; i++)
+            w[i] = __hip_atomic_load(&layer[s0 + i][l], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+#pragma unroll
+        for (uint32_t i = 0; i < 4u; i++) live |= fj < w[i] ? 1u << (s0 + i) : 0u;
+    }
+    // words only decrease, so a stale read never drops a winner
+    live &= (1u << G) - 1u;
+    if (!live) return;
+    const uint32_t T = alpha_threshold(alpha);
+    const uint32_t base = (uint32_t)fj ^ (px * 9277u) ^ (py * 2699u);
+    while (live) {
+        const uint32_t s = (uint32_t)__builtin_ctz(live);
+        live &= live - 1u;
+        const uint32_t h = pcg_hash(base ^ (fs_group + s * 1973u));
+        if ((h >> 9u) < T) atomicMin(&layer[s][l], (unsigned long long)fj);
+    }
 }
 
 // kLayers: samples held in LDS per pass over the tile's instances
 template <uint32_t kLayers>
-__global__ __launch_bounds__(kBatchBlock) void batch_tile_raster_kernel(SceneArgs sc, const ViewDesc* __restrict__ views,
+__global__ __launch_bounds__(kBatchBlock, kLayers <= 8u ? 5 : 3) void batch_tile_raster_kernel(SceneArgs sc, const ViewDesc* __restrict__ views,
                                                                         uint32_t n_views,
                                                                         const uint32_t* __restrict__ tile_prefix,
                                                                         const uint32_t* __restrict__ offsets,
                                                                         const uint32_t* __restrict__ instances,
+                                                                        const uint32_t* __restrict__ keys,
                                                                         const BatchCounters* __restrict__ ctr,
                                                                         uint64_t* __restrict__ fb)
 {
     __shared__ unsigned long long layer[kLayers][kTile * kTile];
-    __shared__ uint32_t s_key[kSortGroup];
     __shared__ uint64_t s_frag[kBatchBlock];
     __shared__ uint64_t s_sub[kSubtiles];
+    __shared__ uint32_t s_pmax[kTile * kTile];  // depth bits of each pixel's largest word
+    __shared__ uint16_t s_queue[kBatchBlock / 64u][128];  // per-wave (instance << 8 | pixel) queue
     __shared__ float s_msx[kBatchBlock], s_msy[kBatchBlock], s_o[kBatchBlock];
     __shared__ float s_ca[kBatchBlock], s_cb[kBatchBlock], s_cc[kBatchBlock];
     __shared__ uint32_t s_rect[kBatchBlock], s_end[kBatchBlock];
     __shared__ uint32_t s_wave[kBatchBlock / 64u];
-    __shared__ uint32_t s_gmin[kBatchBlock / 64u];
-    __shared__ uint64_t s_gmax[kBatchBlock / 64u];
     if (ctr->overflow) return;
     const uint32_t gt = blockIdx.x;
     uint32_t lo = 0, hi = n_views - 1u;
@@ -714,144 +819,131 @@ __global__ __launch_bounds__(kBatchBlock) void batch_tile_raster_kernel(SceneArg
         // pixels outside the view hold 0 so they never keep an instance alive
         for (uint32_t s = 0; s < G; s++) layer[s][tid] = inside ? kEmpty : 0ull;
         const uint32_t fs_group = (vd.frame0 + s0) * 1973u;
-        for (uint32_t sb = begin; sb < end; sb += kSortGroup) {
-            const uint32_t ns = min(kSortGroup, end - sb);
-            uint32_t n2 = 64u;
-            while (n2 < ns) n2 <<= 1;
+        for (uint32_t b0 = begin; b0 < end; b0 += kBatchBlock) {
             __syncthreads();
-            uint32_t kmin = 0xffffffffu;
-            for (uint32_t i = tid; i < n2; i += kBatchBlock) {
-                uint32_t key = 0xffffffffu;
-                if (i < ns) {
-                    const uint32_t g = instances[sb + i];
-                    key = g < sc.n ? (f2u(view_depth(sc.means, g, vd.r3)) & ~(kSortGroup - 1u)) | i : 0xffffffffu;
-                }
-                s_key[i] = key;
-                kmin = min(kmin, key);
-            }
+            // pixel and sub-tile maxima of the words so far
             {
                 uint64_t m = 0;
                 for (uint32_t s = 0; s < G; s++) m = max(m, (uint64_t)layer[s][tid]);
-                for (int off = 32; off > 0; off >>= 1) m = max(m, shfl_xor_u64(m, off));
-                kmin = wave_min_u32(kmin);
-                if (lane == 0u) {
-                    s_gmin[tid / 64u] = kmin;
-                    s_gmax[tid / 64u] = m;
-                }
+                s_pmax[tid] = (uint32_t)(m >> 32);
+                m = max(m, shfl_xor_u64(m, 1));
+                m = max(m, shfl_xor_u64(m, 2));
+                m = max(m, shfl_xor_u64(m, 16));
+                m = max(m, shfl_xor_u64(m, 32));
+                if ((lane & 51u) == 0u) s_sub[(tid / 64u) * (kTile / kSub) + (lane & 15u) / kSub] = m;
             }
             __syncthreads();
-            {
-                // a group whose nearest key lies behind every word of the tile has nothing to add
-                uint32_t gmin = 0xffffffffu;
-                uint64_t tmax = 0;
-                for (uint32_t w = 0; w < kBatchBlock / 64u; w++) {
-                    gmin = min(gmin, s_gmin[w]);
-                    tmax = max(tmax, s_gmax[w]);
-                }
-                if ((gmin & ~(kSortGroup - 1u)) > (uint32_t)(tmax >> 32)) continue;
-            }
-            bitonic_sort(s_key, n2);
-            for (uint32_t b0 = 0; b0 < ns; b0 += kBatchBlock) {
-                __syncthreads();
-                // sub-tile maxima of the words so far
-                {
-                    uint64_t m = 0;
-                    for (uint32_t s = 0; s < G; s++) m = max(m, (uint64_t)layer[s][tid]);
-                    m = max(m, shfl_xor_u64(m, 1));
-                    m = max(m, shfl_xor_u64(m, 2));
-                    m = max(m, shfl_xor_u64(m, 16));
-                    m = max(m, shfl_xor_u64(m, 32));
-                    if ((lane & 51u) == 0u) s_sub[(tid / 64u) * (kTile / kSub) + (lane & 15u) / kSub] = m;
-                }
-                __syncthreads();
-                // keys ascend: once the nearest remaining key lies behind every word of the
-                // tile, no later instance of the group can win a sample
-                uint64_t tile_max = 0;
-                for (uint32_t q = 0; q < kSubtiles; q++) tile_max = max(tile_max, s_sub[q]);
-                if ((s_key[b0] & ~(kSortGroup - 1u)) > (uint32_t)(tile_max >> 32)) break;
-                const uint32_t nb = min(kBatchBlock, ns - b0);
-                uint32_t cnt = 0, rect = 0;
-                if (tid < nb) {
-                    const uint32_t idx = s_key[b0 + tid] & (kSortGroup - 1u);
-                    const uint32_t g = idx < ns ? instances[sb + idx] : 0xffffffffu;
-                    Projected p;
-                    if (g < sc.n && project_gaussian(sc.means, sc.scales, sc.rotations, sc.opacities, g, vd.r1, vd.r2,
-                                                     vd.r3, vd.cam, p)) {
-                        const Aabb b = pixel_aabb(p, vd.cam);
-                        const uint32_t cx0 = max(b.x0, ox), cx1 = min(b.x1, ox + kTile);
-                        const uint32_t cy0 = max(b.y0, oy), cy1 = min(b.y1, oy + kTile);
-                        const uint64_t frag = fragment_key(p.depth, g);
-                        if (cx1 > cx0 && cy1 > cy0) {
-                            const uint32_t rw = cx1 - cx0, rh = cy1 - cy0;
-                            rect = (cx0 - ox) | ((cy0 - oy) << 4) | ((rw - 1u) << 8) |
-                                   (((65536u + rw - 1u) / rw) << 12);
-                            // an instance that no word of its sub-tiles exceeds cannot win
-                            uint64_t m = 0;
-                            for (uint32_t sy = (cy0 - oy) / kSub; sy <= (cy1 - 1u - oy) / kSub; sy++)
-                                for (uint32_t sx = (cx0 - ox) / kSub; sx <= (cx1 - 1u - ox) / kSub; sx++)
-                                    m = max(m, s_sub[sy * (kTile / kSub) + sx]);
-                            if (frag < m) cnt = rw * rh;
-                        }
-                        const ConicExp2 ce = conic_exp2(p.conic_x, p.conic_y, p.conic_z);
-                        s_frag[tid] = frag;
-                        s_msx[tid] = p.mean_x - 0.5f;
-                        s_msy[tid] = p.mean_y - 0.5f;
-                        s_o[tid] = p.opacity;
-                        s_ca[tid] = ce.a;
-                        s_cb[tid] = ce.b;
-                        s_cc[tid] = ce.c;
+            // bucket floors ascend: once the nearest remaining one lies behind every word of
+            // the tile, no later instance can win a sample
+            uint64_t tile_max = 0;
+            for (uint32_t q = 0; q < kSubtiles; q++) tile_max = max(tile_max, s_sub[q]);
+            if (keys[b0] > (uint32_t)(tile_max >> 32)) break;
+            const uint32_t nb = min(kBatchBlock, end - b0);
+            uint32_t cnt = 0, rect = 0;
+            if (tid < nb) {
+                const uint32_t g = instances[b0 + tid];
+                Projected p;
+                if (g < sc.n && project_gaussian(sc.means, sc.scales, sc.rotations, sc.opacities, g, vd.r1, vd.r2,
+                                                 vd.r3, vd.cam, p)) {
+                    const Aabb b = pixel_aabb(p, vd.cam);
+                    const uint32_t cx0 = max(b.x0, ox), cx1 = min(b.x1, ox + kTile);
+                    const uint32_t cy0 = max(b.y0, oy), cy1 = min(b.y1, oy + kTile);
+                    const uint64_t frag = fragment_key(p.depth, g);
+                    if (cx1 > cx0 && cy1 > cy0) {
+                        const uint32_t rw = cx1 - cx0, rh = cy1 - cy0;
+                        rect = (cx0 - ox) | ((cy0 - oy) << 4) | ((rw - 1u) << 8) |
+                               (((65536u + rw - 1u) / rw) << 12);
+                        // an instance that no word of its sub-tiles exceeds cannot win
+                        uint64_t m = 0;
+                        for (uint32_t sy = (cy0 - oy) / kSub; sy <= (cy1 - 1u - oy) / kSub; sy++)
+                            for (uint32_t sx = (cx0 - ox) / kSub; sx <= (cx1 - 1u - ox) / kSub; sx++)
+                                m = max(m, s_sub[sy * (kTile / kSub) + sx]);
+                        if (frag < m) cnt = rw * rh;
                     }
-                }
-                s_rect[tid] = rect;
-                // fragments of the batch's live instances, flattened over the workgroup's lanes
-                const uint32_t incl = block_inclusive_scan(cnt, s_wave);
-                s_end[tid] = incl;
-                __syncthreads();
-                const uint32_t total = s_end[kBatchBlock - 1u];
-                for (uint32_t k = tid; k < total; k += kBatchBlock) {
-                    uint32_t a = 0, bb = kBatchBlock - 1u;
-                    while (a < bb) {
-                        const uint32_t m = (a + bb) >> 1;
-                        if (s_end[m] > k) bb = m;
-                        else a = m + 1u;
-                    }
-                    const uint32_t j = a;
-                    const uint32_t f = k - (j ? s_end[j - 1u] : 0u);
-                    const uint32_t rect_j = s_rect[j];
-                    const uint32_t rw = ((rect_j >> 8) & 0xfu) + 1u;
-                    const uint32_t q = (f * (rect_j >> 12)) >> 16, r = f - q * rw;  // f / rw for f < 256, rw <= 16
-                    const uint32_t lx = (rect_j & 0xfu) + r, ly = ((rect_j >> 4) & 0xfu) + q;
-                    const uint32_t l = ly * kTile + lx;
-                    const uint64_t fj = s_frag[j];
-                    // samples whose word this fragment is still below (words only decrease, so
-                    // a stale read never drops a winner)
-                    uint32_t live = 0;
-                    for (uint32_t s = 0; s < G; s += 4u) {
-                        uint64_t w4[4];
-#pragma unroll
-                        for (uint32_t i = 0; i < 4u; i++)
-                            w4[i] = s + i < G ? __hip_atomic_load(&layer[s + i][l], __ATOMIC_RELAXED,
-                                                                  __HIP_MEMORY_SCOPE_WORKGROUP)
-                                              : 0ull;
-#pragma unroll
-                        for (uint32_t i = 0; i < 4u; i++)
-                            if (fj < w4[i]) live |= 1u << (s + i);
-                    }
-                    if (!live) continue;
-                    const uint32_t px = ox + lx, py = oy + ly;
-                    const ConicExp2 ce = { s_ca[j], s_cb[j], s_cc[j] };
-                    const float alpha = fragment_alpha(ce, s_o[j], (float)px - s_msx[j], (float)py - s_msy[j]);
-                    if (alpha < kMinAlpha) continue;
-                    const uint32_t T = alpha_threshold(alpha);
-                    const uint32_t base = (uint32_t)fj ^ (px * 9277u) ^ (py * 2699u);
-                    while (live) {
-                        const uint32_t s = (uint32_t)__builtin_ctz(live);
-                        live &= live - 1u;
-                        const uint32_t h = pcg_hash(base ^ (fs_group + s * 1973u));
-                        if ((h >> 9u) < T) atomicMin(&layer[s][l], (unsigned long long)fj);
-                    }
+                    const ConicExp2 ce = conic_exp2(p.conic_x, p.conic_y, p.conic_z);
+                    s_frag[tid] = frag;
+                    s_msx[tid] = p.mean_x - 0.5f;
+                    s_msy[tid] = p.mean_y - 0.5f;
+                    s_o[tid] = p.opacity;
+                    s_ca[tid] = ce.a;
+                    s_cb[tid] = ce.b;
+                    s_cc[tid] = ce.c;
                 }
             }
+            s_rect[tid] = rect;
+            // fragments of the batch's live instances, flattened over the workgroup's lanes
+            const uint32_t incl = block_inclusive_scan(cnt, s_wave);
+            s_end[tid] = incl;
+            __syncthreads();
+            const uint32_t total = s_end[kBatchBlock - 1u];
+            // Each lane walks a contiguous range of the batch's fragments (an instance's
+            // rectangle advances without division). Fragments that some sample word of
+            // their pixel still lies behind go to the wave's LDS queue; alpha and the
+            // samples are evaluated on full queues of 64, so no lane idles on a rejected
+            // fragment.
+            const uint32_t per = (total + kBatchBlock - 1u) / kBatchBlock;
+            uint32_t k = tid * per;
+            const uint32_t kend = min(k + per, total);
+            uint32_t j = 0, jend = 0, rect_j = 0, rw = 1, q = 0, r = 0, fdepth = 0;
+            if (k < kend) {
+                uint32_t bb = kBatchBlock - 1u;
+                while (j < bb) {
+                    const uint32_t m = (j + bb) >> 1;
+                    if (s_end[m] > k) bb = m;
+                    else j = m + 1u;
+                }
+                jend = s_end[j];
+                const uint32_t f = k - (j ? s_end[j - 1u] : 0u);
+                rect_j = s_rect[j];
+                rw = ((rect_j >> 8) & 0xfu) + 1u;
+                q = (f * (rect_j >> 12)) >> 16;  // f / rw for f < 256, rw <= 16
+                r = f - q * rw;
+                fdepth = (uint32_t)(s_frag[j] >> 32);
+            }
+            uint16_t* queue = s_queue[tid / 64u];
+            uint32_t count = 0;  // wave-uniform
+            for (uint32_t it = 0; it < per; it++, k++) {
+                bool cand = false;
+                uint32_t entry = 0;
+                if (k < kend) {
+                    if (k == jend) {
+                        while (s_end[j] <= k) j++;
+                        jend = s_end[j];
+                        rect_j = s_rect[j];
+                        rw = ((rect_j >> 8) & 0xfu) + 1u;
+                        q = 0;
+                        r = 0;
+                        fdepth = (uint32_t)(s_frag[j] >> 32);
+                    }
+                    const uint32_t l = (((rect_j >> 4) & 0xfu) + q) * kTile + (rect_j & 0xfu) + r;
+                    if (++r == rw) {
+                        r = 0;
+                        q++;
+                    }
+                    // some sample word of the pixel was behind the fragment at the batch start
+                    cand = fdepth <= s_pmax[l];
+                    entry = (j << 8) | l;
+                }
+                const uint64_t m = __ballot(cand);
+                if (!m) continue;
+                if (cand) queue[count + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = (uint16_t)entry;
+                count += (uint32_t)__popcll(m);
+                if (count >= 64u) {
+                    __builtin_amdgcn_wave_barrier();
+                    tile_fragment<kLayers>(queue[lane], layer, s_frag, s_msx, s_msy, s_o, s_ca, s_cb, s_cc, ox, oy,
+                                           G, fs_group);
+                    count -= 64u;
+                    __builtin_amdgcn_wave_barrier();
+                    const uint32_t rest = lane < count ? queue[64u + lane] : 0u;
+                    __builtin_amdgcn_wave_barrier();
+                    if (lane < count) queue[lane] = rest;
+                    __builtin_amdgcn_wave_barrier();
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+            if (lane < count)
+                tile_fragment<kLayers>(queue[lane], layer, s_frag, s_msx, s_msy, s_o, s_ca, s_cb, s_cc, ox, oy, G,
+                                       fs_group);
         }
         __syncthreads();
         if (inside)
@@ -885,7 +977,7 @@ hipError_t launch_batch_tiles(const SceneArgs& sc, const ViewDesc* views, uint32
                            n_chunks, tb.chunk_order);
         hipLaunchKernelGGL((batch_bin_kernel<false>), dim3((unsigned)grid), dim3(kBatchBlock), 0, stream, sc, views,
                            n_views, (uint32_t)n_items64, tb.chunk_order, tb.tile_count, tb.tile_cursor, tb.offsets,
-                           tb.instances, counters);
+                           tb.instances, tb.keys, counters);
     }
     hipLaunchKernelGGL((exclusive_scan_kernel<CountInput>), dim3(1), dim3(kScanThreads), 0, stream,
                        CountInput{ tb.tile_count }, tb.offsets, nullptr, tb.n_tiles, tb.n_tiles, &counters->n_instances,
@@ -893,18 +985,20 @@ hipError_t launch_batch_tiles(const SceneArgs& sc, const ViewDesc* views, uint32
     if (sc.n > 0) {
         hipLaunchKernelGGL((batch_bin_kernel<true>), dim3((unsigned)grid), dim3(kBatchBlock), 0, stream, sc, views,
                            n_views, (uint32_t)n_items64, tb.chunk_order, tb.tile_count, tb.tile_cursor, tb.offsets,
-                           tb.instances, counters);
+                           tb.instances, tb.keys, counters);
     }
+    hipLaunchKernelGGL(tile_sort_kernel, dim3(tb.n_tiles), dim3(kBatchBlock), 0, stream, tb.offsets, tb.keys,
+                       tb.instances, tb.sorted_keys, tb.sorted_instances, counters);
     if (raster_event) {
         e = hipEventRecord(raster_event, stream);
         if (e != hipSuccess) return e;
     }
     if (layers == 8u)
         hipLaunchKernelGGL(batch_tile_raster_kernel<8>, dim3(tb.n_tiles), dim3(kBatchBlock), 0, stream, sc, views,
-                           n_views, tile_prefix, tb.offsets, tb.instances, counters, fb);
+                           n_views, tile_prefix, tb.offsets, tb.sorted_instances, tb.sorted_keys, counters, fb);
     else
         hipLaunchKernelGGL(batch_tile_raster_kernel<16>, dim3(tb.n_tiles), dim3(kBatchBlock), 0, stream, sc, views,
-                           n_views, tile_prefix, tb.offsets, tb.instances, counters, fb);
+                           n_views, tile_prefix, tb.offsets, tb.sorted_instances, tb.sorted_keys, counters, fb);
     return hipGetLastError();
 }
 

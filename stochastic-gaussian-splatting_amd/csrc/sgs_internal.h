@@ -177,6 +177,9 @@ struct TileBuffers {
     uint32_t* tile_cursor;      // [n_tiles] fill cursors (zeroed by the caller)
     uint32_t* offsets;          // [n_tiles + 1] exclusive scan of tile_count
     uint32_t* instances;        // [capacity] Gaussian index of each (view, tile, Gaussian) instance
+    uint32_t* keys;             // [capacity] its depth key (float bits of the view depth)
+    uint32_t* sorted_instances; // [capacity] each tile's instances, nearest depth bucket first
+    uint32_t* sorted_keys;      // [capacity] their depth bucket floors (non-decreasing per tile)
     float4* centroids;          // [n_chunks] mean position of each chunk of 256 Gaussians
     uint32_t* chunk_order;      // [n_views * n_chunks] chunks of each view, nearest first
     uint32_t n_tiles, capacity;
