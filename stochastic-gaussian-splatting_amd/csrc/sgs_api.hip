@@ -44,6 +44,11 @@ FbCamera make_fb_camera(uint32_t width, uint32_t height, float focal_x, float fo
     c.near_plane = near_plane;
     c.far_plane = far_plane;
     c.proper_antialiasing = proper_antialiasing ? 1 : 0;
+    // the same IEEE single-precision operations the projection would do per Gaussian
+    c.clip_left = (-0.15f * c.width - c.center_x) / c.focal_x;
+    c.clip_right = (1.15f * c.width - c.center_x) / c.focal_x;
+    c.clip_top = (-0.15f * c.height - c.center_y) / c.focal_y;
+    c.clip_bottom = (1.15f * c.height - c.center_y) / c.focal_y;
     return c;
 }
 
@@ -356,7 +361,7 @@ uint32_t view_grid_h(const sgs_view& v) { return (uint32_t)((v.height * view_ss(
 // cursors are contiguous so one fill clears them all.
 struct BatchLayout {
     size_t counters, ready, tile_count, tile_cursor, cleared_end, offsets, instances, keys, sorted_instances, sorted_keys,
-        centroids, chunk_order, queue, table, fb, samples, total;
+        centroids, cov3d, view_keys, queue, table, fb, samples, total;
     uint64_t fb_words, n_samples, n_tiles;
     uint32_t n_blocks, capacity;
 };
@@ -396,7 +401,8 @@ BatchLayout batch_layout(const sgs_view* views, uint32_t n, uint32_t n_gaussians
     L.sorted_keys = a.take(4u * (size_t)L.capacity);
     const size_t n_chunks = tiles ? ((size_t)n_gaussians + 255u) / 256u : 0u;
     L.centroids = a.take(16u * n_chunks);
-    L.chunk_order = a.take(4u * n_chunks * n);
+    L.view_keys = a.take(8u * (size_t)n);
+    L.cov3d = a.take(tiles ? 32u * (size_t)n_gaussians : 0u);
     L.queue = a.take(sizeof(BigEntry) * (size_t)t.queue_capacity);
     L.table = a.take(table_bytes(n));
     L.fb = a.take(8u * L.fb_words);
@@ -572,7 +578,8 @@ int sgs_render_batch(const sgs_gaussians* gaussians, const sgs_view* views, uint
         tb.sorted_instances = reinterpret_cast<uint32_t*>(ws + L.sorted_instances);
         tb.sorted_keys = reinterpret_cast<uint32_t*>(ws + L.sorted_keys);
         tb.centroids = reinterpret_cast<float4*>(ws + L.centroids);
-        tb.chunk_order = reinterpret_cast<uint32_t*>(ws + L.chunk_order);
+        tb.view_keys = reinterpret_cast<uint2*>(ws + L.view_keys);
+        tb.cov3d = reinterpret_cast<float4*>(ws + L.cov3d);
         tb.n_tiles = (uint32_t)L.n_tiles;
         tb.capacity = L.capacity;
         e = launch_batch_tiles(sc, desc, n_views, tile_prefix, tb, t.tile_samples, counters, fb,

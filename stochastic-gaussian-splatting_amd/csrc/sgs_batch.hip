@@ -348,13 +348,11 @@ struct TileRange {
 
 // tile range and depth key (float bits of the view depth) of Gaussian g in view vd (false
 // if culled or its AABB is empty)
-__device__ __forceinline__ bool gaussian_tiles(const SceneArgs& sc, const ViewDesc& vd, uint32_t g, TileRange& t,
-                                               uint32_t& key)
+__device__ __forceinline__ bool gaussian_tiles(const SceneArgs& sc, const float4* __restrict__ cov3d,
+                                               const ViewDesc& vd, uint32_t g, TileRange& t, uint32_t& key)
 {
     Projected p;
-    if (!(g < sc.n && project_gaussian(sc.means, sc.scales, sc.rotations, sc.opacities, g, vd.r1, vd.r2, vd.r3, vd.cam,
-                                       p)))
-        return false;
+    if (!(g < sc.n && project_cached(sc.means, cov3d, g, vd.r1, vd.r2, vd.r3, vd.cam, p))) return false;
     key = f2u(p.depth);
     const Aabb b = pixel_aabb(p, vd.cam);
     if (b.x1 <= b.x0 || b.y1 <= b.y0) return false;
@@ -395,12 +393,21 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t x)
     return x;
 }
 
-// Items are (view, chunk rank): the fill pass takes each view's chunks nearest first (chunk_order), so
-// every tile's list comes out roughly front to back; the counting pass takes them in index order.
+// Items are (view, chunk of 256 Gaussians), view by view.
+// view-independent part of every Gaussian's projection, once per render
+__global__ __launch_bounds__(kBatchBlock) void cov3d_kernel(SceneArgs sc, float4* __restrict__ cache)
+{
+    const uint32_t g = blockIdx.x * kBatchBlock + threadIdx.x;
+    if (g >= sc.n) return;
+    Cov3d c = { 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f };
+    const bool valid = gaussian_cov3d(sc.scales, sc.rotations, sc.opacities, g, c);
+    store_cov3d(cache, g, valid, c);
+}
+
 template <bool kFill>
-__global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, const ViewDesc* __restrict__ views,
+__global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, const float4* __restrict__ cov3d,
+                                                                const ViewDesc* __restrict__ views,
                                                                 uint32_t n_views, uint32_t n_items,
-                                                                const uint32_t* __restrict__ chunk_order,
                                                                 uint32_t* __restrict__ tile_count,
                                                                 uint32_t* __restrict__ tile_cursor,
                                                                 const uint32_t* __restrict__ offsets,
@@ -429,12 +436,12 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
         }
         __syncthreads();
         const uint32_t v = item / n_chunks;
-        const uint32_t chunk = kFill ? chunk_order[item] : item - v * n_chunks;
+        const uint32_t chunk = item - v * n_chunks;
         const ViewDesc& vd = views[v];
         const uint32_t g = chunk * kBatchBlock + tid;
         TileRange tr;
         uint32_t key = 0;
-        const bool has = gaussian_tiles(sc, vd, g, tr, key);
+        const bool has = gaussian_tiles(sc, cov3d, vd, g, tr, key);
         const uint32_t tw = has ? tr.tx1 - tr.tx0 + 1u : 0u;
         const uint32_t n_t = has ? tw * (tr.ty1 - tr.ty0 + 1u) : 0u;
         const bool small = has && n_t <= kSmallTiles;
@@ -518,7 +525,7 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
     }
 }
 
-// ---- per-view front-to-back order of the Gaussian chunks ----
+// ---- depth range of each view (chunk centroids) ----
 __global__ __launch_bounds__(kBatchBlock) void chunk_centroid_kernel(SceneArgs sc, float4* __restrict__ centroids)
 {
     __shared__ float s_sum[kBatchBlock / 64u][4];
@@ -556,29 +563,21 @@ __global__ __launch_bounds__(kBatchBlock) void chunk_centroid_kernel(SceneArgs s
     }
 }
 
-constexpr uint32_t kOrderThreads = 1024, kOrderBuckets = 4096;
+constexpr uint32_t kRangeThreads = 1024;
 
-__device__ __forceinline__ uint32_t order_bucket(float z, float zmin, float scale)
+// One workgroup per view: the depth-key bucketing of the per-tile sort. Keys are float bits
+// of positive depths, so (key - kmin) >> shift over the view's depth range of the chunk
+// centroids gives ordered buckets of equal relative depth width.
+__global__ __launch_bounds__(kRangeThreads) void view_key_range_kernel(const ViewDesc* __restrict__ views,
+                                                                        const float4* __restrict__ centroids,
+                                                                        uint32_t n_chunks, uint2* __restrict__ view_keys)
 {
-    if (!(z == z)) return kOrderBuckets - 1u;
-    const float t = (z - zmin) * scale;
-    if (!(t > 0.0f)) return 0u;
-    return t >= (float)(kOrderBuckets - 1u) ? kOrderBuckets - 1u : (uint32_t)t;
-}
-
-// One workgroup per view: counting sort of the chunks by the view depth of their centroid.
-__global__ __launch_bounds__(kOrderThreads) void chunk_order_kernel(const ViewDesc* __restrict__ views,
-                                                                     const float4* __restrict__ centroids,
-                                                                     uint32_t n_chunks, uint32_t* __restrict__ order)
-{
-    __shared__ uint32_t hist[kOrderBuckets];
-    __shared__ float s_lo[kOrderThreads / 64u], s_hi[kOrderThreads / 64u];
-    __shared__ uint32_t s_wave[kOrderThreads / 64u];
+    __shared__ float s_lo[kRangeThreads / 64u], s_hi[kRangeThreads / 64u];
     const ViewDesc& vd = views[blockIdx.x];
     const float4 r3 = vd.r3;
     const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid / 64u;
     float lo = 3.0e38f, hi = -3.0e38f;
-    for (uint32_t c = tid; c < n_chunks; c += kOrderThreads) {
+    for (uint32_t c = tid; c < n_chunks; c += kRangeThreads) {
         const float4 m = centroids[c];
         const float z = r3.x * m.x + r3.y * m.y + r3.z * m.z + r3.w;
         if (z == z) {
@@ -594,56 +593,44 @@ __global__ __launch_bounds__(kOrderThreads) void chunk_order_kernel(const ViewDe
         s_lo[wave] = lo;
         s_hi[wave] = hi;
     }
-    for (uint32_t i = tid; i < kOrderBuckets; i += kOrderThreads) hist[i] = 0u;
     __syncthreads();
-    float zmin = 3.0e38f, zmax = -3.0e38f;
-    for (uint32_t w = 0; w < kOrderThreads / 64u; w++) {
-        zmin = fminf(zmin, s_lo[w]);
-        zmax = fmaxf(zmax, s_hi[w]);
-    }
-    const float scale = zmax > zmin ? (float)(kOrderBuckets - 1u) / (zmax - zmin) : 0.0f;
-    for (uint32_t c = tid; c < n_chunks; c += kOrderThreads) {
-        const float4 m = centroids[c];
-        atomicAdd(&hist[order_bucket(r3.x * m.x + r3.y * m.y + r3.z * m.z + r3.w, zmin, scale)], 1u);
-    }
-    __syncthreads();
-    // exclusive scan of the histogram, 4 buckets per thread
-    uint32_t v[4], local = 0;
-    for (int k = 0; k < 4; k++) {
-        v[k] = hist[tid * 4u + (uint32_t)k];
-        local += v[k];
-    }
-    uint32_t incl = local;
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t y = (uint32_t)__shfl_up((int)incl, off, 64);
-        if ((int)lane >= off) incl += y;
-    }
-    if (lane == 63u) s_wave[wave] = incl;
-    __syncthreads();
-    uint32_t run = incl - local;
-    for (uint32_t w = 0; w < wave; w++) run += s_wave[w];
-    for (int k = 0; k < 4; k++) {
-        hist[tid * 4u + (uint32_t)k] = run;
-        run += v[k];
-    }
-    __syncthreads();
-    uint32_t* out = order + (size_t)blockIdx.x * n_chunks;
-    for (uint32_t c = tid; c < n_chunks; c += kOrderThreads) {
-        const float4 m = centroids[c];
-        // the bucket is recomputed with the same IEEE operations, so it matches the count
-        const uint32_t pos = atomicAdd(&hist[order_bucket(r3.x * m.x + r3.y * m.y + r3.z * m.z + r3.w, zmin, scale)], 1u);
-        if (pos < n_chunks) out[pos] = c;
+    if (tid == 0u) {
+        float zmin = 3.0e38f, zmax = -3.0e38f;
+        for (uint32_t w = 0; w < kRangeThreads / 64u; w++) {
+            zmin = fminf(zmin, s_lo[w]);
+            zmax = fmaxf(zmax, s_hi[w]);
+        }
+        const float lo_d = fmaxf(zmin, vd.cam.near_plane), hi_d = fminf(zmax, vd.cam.far_plane);
+        const uint32_t kmin = f2u(lo_d), kmax = hi_d > lo_d ? f2u(hi_d) : kmin;
+        const uint32_t range = kmax - kmin;
+        const uint32_t bits = range ? 32u - (uint32_t)__clz(range) : 0u;
+        view_keys[blockIdx.x] = make_uint2(kmin, bits > 9u ? bits - 9u : 0u);  // (range >> shift) < 512
     }
 }
 
 // ---- per-tile counting sort by depth ----
-// One workgroup per tile: the tile's keys are binned into up to kSortBins buckets over the
-// tile's own key range (the top bits of key - min), and the instances are written bucket by
-// bucket, nearest first. keys_out holds each instance's bucket floor (min + bucket << shift),
-// a non-decreasing lower bound of the true keys from that position on.
+// One workgroup per tile: the tile's instances are written bucket by bucket, nearest first.
+// Bucket 0 takes keys below the view's kmin; bucket b >= 1 takes (key - kmin) >> shift = b - 1
+// (clamped to the last bucket). keys_out holds each instance's bucket floor (0, or
+// kmin + (b - 1) << shift), a non-decreasing lower bound of the true keys from that position on.
 constexpr uint32_t kSortBins = 1024;
 
-__global__ __launch_bounds__(kBatchBlock) void tile_sort_kernel(const uint32_t* __restrict__ offsets,
+__device__ __forceinline__ uint32_t sort_bucket(uint32_t key, uint2 vk)
+{
+    if (key < vk.x) return 0u;
+    return 1u + min((key - vk.x) >> vk.y, kSortBins - 2u);
+}
+
+__device__ __forceinline__ uint32_t bucket_floor(uint32_t b, uint2 vk)
+{
+    if (b == 0u) return 0u;
+    const uint64_t f = (uint64_t)vk.x + ((uint64_t)(b - 1u) << vk.y);
+    return f > 0xffffffffull ? 0xffffffffu : (uint32_t)f;
+}
+
+__global__ __launch_bounds__(kBatchBlock) void tile_sort_kernel(uint32_t n_views, const uint32_t* __restrict__ tile_prefix,
+                                                                const uint2* __restrict__ view_keys,
+                                                                const uint32_t* __restrict__ offsets,
                                                                 const uint32_t* __restrict__ keys_in,
                                                                 const uint32_t* __restrict__ inst_in,
                                                                 uint32_t* __restrict__ keys_out,
@@ -651,34 +638,31 @@ __global__ __launch_bounds__(kBatchBlock) void tile_sort_kernel(const uint32_t* 
                                                                 const BatchCounters* __restrict__ ctr)
 {
     __shared__ uint32_t hist[kSortBins];
-    __shared__ uint32_t s_lo[kBatchBlock / 64u], s_hi[kBatchBlock / 64u], s_wave[kBatchBlock / 64u];
+    __shared__ uint32_t s_wave[kBatchBlock / 64u];
     if (ctr->overflow) return;
-    const uint32_t begin = offsets[blockIdx.x], end = offsets[blockIdx.x + 1u];
+    const uint32_t gt = blockIdx.x;
+    const uint32_t begin = offsets[gt], end = offsets[gt + 1u];
     if (end <= begin) return;
+    uint32_t lo = 0, hi = n_views - 1u;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1u) >> 1;
+        if (tile_prefix[mid] <= gt) lo = mid;
+        else hi = mid - 1u;
+    }
+    const uint2 vk = view_keys[lo];
     const uint32_t n = end - begin, tid = threadIdx.x, lane = lane_id(), wave = tid / 64u;
-    uint32_t lo = 0xffffffffu, hi = 0u;
-    for (uint32_t i = tid; i < n; i += kBatchBlock) {
-        const uint32_t k = keys_in[begin + i];
-        lo = min(lo, k);
-        hi = max(hi, k);
-    }
-    lo = wave_min_u32(lo);
-    hi = wave_max_u32(hi);
-    if (lane == 0u) {
-        s_lo[wave] = lo;
-        s_hi[wave] = hi;
-    }
     for (uint32_t i = tid; i < kSortBins; i += kBatchBlock) hist[i] = 0u;
     __syncthreads();
-    uint32_t kmin = 0xffffffffu, kmax = 0u;
-    for (uint32_t w = 0; w < kBatchBlock / 64u; w++) {
-        kmin = min(kmin, s_lo[w]);
-        kmax = max(kmax, s_hi[w]);
+    // four keys in flight per lane
+    uint32_t i = tid;
+    for (; i + 3u * kBatchBlock < n; i += 4u * kBatchBlock) {
+        uint32_t k[4];
+#pragma unroll
+        for (uint32_t u = 0; u < 4u; u++) k[u] = keys_in[begin + i + u * kBatchBlock];
+#pragma unroll
+        for (uint32_t u = 0; u < 4u; u++) atomicAdd(&hist[sort_bucket(k[u], vk)], 1u);
     }
-    const uint32_t range = kmax - kmin;
-    const uint32_t bits = range ? 32u - (uint32_t)__clz(range) : 0u;
-    const uint32_t shift = bits > 10u ? bits - 10u : 0u;  // (range >> shift) < 1024
-    for (uint32_t i = tid; i < n; i += kBatchBlock) atomicAdd(&hist[(keys_in[begin + i] - kmin) >> shift], 1u);
+    for (; i < n; i += kBatchBlock) atomicAdd(&hist[sort_bucket(keys_in[begin + i], vk)], 1u);
     __syncthreads();
     // exclusive scan of the buckets, 4 per thread
     uint32_t v[4], local = 0;
@@ -703,11 +687,27 @@ __global__ __launch_bounds__(kBatchBlock) void tile_sort_kernel(const uint32_t* 
         run += v[q];
     }
     __syncthreads();
-    for (uint32_t i = tid; i < n; i += kBatchBlock) {
-        const uint32_t bucket = (keys_in[begin + i] - kmin) >> shift;
-        const uint32_t slot = begin + atomicAdd(&hist[bucket], 1u);
+    i = tid;
+    for (; i + 3u * kBatchBlock < n; i += 4u * kBatchBlock) {
+        uint32_t k[4], g[4];
+#pragma unroll
+        for (uint32_t u = 0; u < 4u; u++) {
+            k[u] = keys_in[begin + i + u * kBatchBlock];
+            g[u] = inst_in[begin + i + u * kBatchBlock];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < 4u; u++) {
+            const uint32_t b = sort_bucket(k[u], vk);
+            const uint32_t slot = begin + atomicAdd(&hist[b], 1u);
+            inst_out[slot] = g[u];
+            keys_out[slot] = bucket_floor(b, vk);
+        }
+    }
+    for (; i < n; i += kBatchBlock) {
+        const uint32_t b = sort_bucket(keys_in[begin + i], vk);
+        const uint32_t slot = begin + atomicAdd(&hist[b], 1u);
         inst_out[slot] = inst_in[begin + i];
-        keys_out[slot] = kmin + (bucket << shift);
+        keys_out[slot] = bucket_floor(b, vk);
     }
 }
 
@@ -777,7 +777,9 @@ __device__ __forceinline__ void tile_fragment(uint32_t entry, unsigned long long
 
 // kLayers: samples held in LDS per pass over the tile's instances
 template <uint32_t kLayers>
-__global__ __launch_bounds__(kBatchBlock, kLayers <= 8u ? 5 : 3) void batch_tile_raster_kernel(SceneArgs sc, const ViewDesc* __restrict__ views,
+__global__ __launch_bounds__(kBatchBlock, kLayers <= 8u ? 5 : 3) void batch_tile_raster_kernel(SceneArgs sc,
+                                                                        const float4* __restrict__ cov3d,
+                                                                        const ViewDesc* __restrict__ views,
                                                                         uint32_t n_views,
                                                                         const uint32_t* __restrict__ tile_prefix,
                                                                         const uint32_t* __restrict__ offsets,
@@ -843,8 +845,7 @@ __global__ __launch_bounds__(kBatchBlock, kLayers <= 8u ? 5 : 3) void batch_tile
             if (tid < nb) {
                 const uint32_t g = instances[b0 + tid];
                 Projected p;
-                if (g < sc.n && project_gaussian(sc.means, sc.scales, sc.rotations, sc.opacities, g, vd.r1, vd.r2,
-                                                 vd.r3, vd.cam, p)) {
+                if (g < sc.n && project_cached(sc.means, cov3d, g, vd.r1, vd.r2, vd.r3, vd.cam, p)) {
                     const Aabb b = pixel_aabb(p, vd.cam);
                     const uint32_t cx0 = max(b.x0, ox), cx1 = min(b.x1, ox + kTile);
                     const uint32_t cy0 = max(b.y0, oy), cy1 = min(b.y1, oy + kTile);
@@ -973,32 +974,33 @@ hipError_t launch_batch_tiles(const SceneArgs& sc, const ViewDesc* views, uint32
     if (grid < 1) grid = 1;
     if (sc.n > 0) {
         hipLaunchKernelGGL(chunk_centroid_kernel, dim3(n_chunks), dim3(kBatchBlock), 0, stream, sc, tb.centroids);
-        hipLaunchKernelGGL(chunk_order_kernel, dim3(n_views), dim3(kOrderThreads), 0, stream, views, tb.centroids,
-                           n_chunks, tb.chunk_order);
-        hipLaunchKernelGGL((batch_bin_kernel<false>), dim3((unsigned)grid), dim3(kBatchBlock), 0, stream, sc, views,
-                           n_views, (uint32_t)n_items64, tb.chunk_order, tb.tile_count, tb.tile_cursor, tb.offsets,
+        hipLaunchKernelGGL(view_key_range_kernel, dim3(n_views), dim3(kRangeThreads), 0, stream, views, tb.centroids,
+                           n_chunks, tb.view_keys);
+        hipLaunchKernelGGL(cov3d_kernel, dim3(n_chunks), dim3(kBatchBlock), 0, stream, sc, tb.cov3d);
+        hipLaunchKernelGGL((batch_bin_kernel<false>), dim3((unsigned)grid), dim3(kBatchBlock), 0, stream, sc, tb.cov3d,
+                           views, n_views, (uint32_t)n_items64, tb.tile_count, tb.tile_cursor, tb.offsets,
                            tb.instances, tb.keys, counters);
     }
     hipLaunchKernelGGL((exclusive_scan_kernel<CountInput>), dim3(1), dim3(kScanThreads), 0, stream,
                        CountInput{ tb.tile_count }, tb.offsets, nullptr, tb.n_tiles, tb.n_tiles, &counters->n_instances,
                        &counters->overflow, tb.capacity, &counters->total_instances);
     if (sc.n > 0) {
-        hipLaunchKernelGGL((batch_bin_kernel<true>), dim3((unsigned)grid), dim3(kBatchBlock), 0, stream, sc, views,
-                           n_views, (uint32_t)n_items64, tb.chunk_order, tb.tile_count, tb.tile_cursor, tb.offsets,
+        hipLaunchKernelGGL((batch_bin_kernel<true>), dim3((unsigned)grid), dim3(kBatchBlock), 0, stream, sc, tb.cov3d,
+                           views, n_views, (uint32_t)n_items64, tb.tile_count, tb.tile_cursor, tb.offsets,
                            tb.instances, tb.keys, counters);
     }
-    hipLaunchKernelGGL(tile_sort_kernel, dim3(tb.n_tiles), dim3(kBatchBlock), 0, stream, tb.offsets, tb.keys,
-                       tb.instances, tb.sorted_keys, tb.sorted_instances, counters);
+    hipLaunchKernelGGL(tile_sort_kernel, dim3(tb.n_tiles), dim3(kBatchBlock), 0, stream, n_views, tile_prefix,
+                       tb.view_keys, tb.offsets, tb.keys, tb.instances, tb.sorted_keys, tb.sorted_instances, counters);
     if (raster_event) {
         e = hipEventRecord(raster_event, stream);
         if (e != hipSuccess) return e;
     }
     if (layers == 8u)
-        hipLaunchKernelGGL(batch_tile_raster_kernel<8>, dim3(tb.n_tiles), dim3(kBatchBlock), 0, stream, sc, views,
-                           n_views, tile_prefix, tb.offsets, tb.sorted_instances, tb.sorted_keys, counters, fb);
+        hipLaunchKernelGGL(batch_tile_raster_kernel<8>, dim3(tb.n_tiles), dim3(kBatchBlock), 0, stream, sc, tb.cov3d,
+                           views, n_views, tile_prefix, tb.offsets, tb.sorted_instances, tb.sorted_keys, counters, fb);
     else
-        hipLaunchKernelGGL(batch_tile_raster_kernel<16>, dim3(tb.n_tiles), dim3(kBatchBlock), 0, stream, sc, views,
-                           n_views, tile_prefix, tb.offsets, tb.sorted_instances, tb.sorted_keys, counters, fb);
+        hipLaunchKernelGGL(batch_tile_raster_kernel<16>, dim3(tb.n_tiles), dim3(kBatchBlock), 0, stream, sc, tb.cov3d,
+                           views, n_views, tile_prefix, tb.offsets, tb.sorted_instances, tb.sorted_keys, counters, fb);
     return hipGetLastError();
 }
 

@@ -291,6 +291,9 @@ struct FbCamera {
     float ss;                   // supersampling factor as float
     uint32_t width_i, height_i; // framebuffer size
     int32_t proper_antialiasing;
+    // view-space clamp of the Jacobian's evaluation point, 15% outside the framebuffer
+    // (gs_utils.h:84-90): (-0.15 * width - center_x) / focal_x etc., computed once per view
+    float clip_left, clip_right, clip_top, clip_bottom;
 };
 
 struct Projected {
@@ -308,19 +311,18 @@ __device__ __forceinline__ float view_depth(const float* __restrict__ means, uin
     return r3.x * mx + r3.y * my + r3.z * mz + r3.w;
 }
 
-// EWA projection with culling (gs_utils.h:38-193); returns false if culled.
-// w2c rows r1, r2, r3 (row-major, 4 floats each).
-__device__ __forceinline__ bool project_gaussian(const float* __restrict__ means, const float* __restrict__ scales,
-                                                 const float* __restrict__ rotations,
-                                                 const float* __restrict__ opacities, uint32_t g,
-                                                 const float4 r1, const float4 r2, const float4 r3,
-                                                 const FbCamera& cam, Projected& o)
+// View-independent part of the projection (gs_utils.h:38-80): sigmoid opacity and the 3D
+// covariance R S S^T R^T of the normalised quaternion. False if the Gaussian is culled in
+// every view (opacity below 1/255 or a degenerate quaternion).
+struct Cov3d {
+    float c11, c12, c13, c22, c23, c33, opacity;
+};
+
+__device__ __forceinline__ bool gaussian_cov3d(const float* __restrict__ scales, const float* __restrict__ rotations,
+                                               const float* __restrict__ opacities, uint32_t g, Cov3d& c)
 {
-    const float mx = means[3u * g + 0u], my = means[3u * g + 1u], mz = means[3u * g + 2u];
-    const float depth = view_depth(means, g, r3);
-    if (!(depth >= cam.near_plane && depth <= cam.far_plane)) return false;
-    float opacity = 1.0f / (1.0f + exp_f(-opacities[g]));
-    if (opacity < kMinAlpha) return false;
+    c.opacity = 1.0f / (1.0f + exp_f(-opacities[g]));
+    if (c.opacity < kMinAlpha) return false;
 
     const float vx = exp_f(2.0f * scales[3u * g + 0u]);
     const float vy = exp_f(2.0f * scales[3u * g + 1u]);
@@ -339,22 +341,28 @@ __device__ __forceinline__ bool project_gaussian(const float* __restrict__ means
     const float S11 = R11 * vx, S12 = R12 * vy, S13 = R13 * vz;
     const float S21 = R21 * vx, S22 = R22 * vy, S23 = R23 * vz;
     const float S31 = R31 * vx, S32 = R32 * vy, S33 = R33 * vz;
-    const float c11 = S11 * R11 + S12 * R12 + S13 * R13;
-    const float c12 = S11 * R21 + S12 * R22 + S13 * R23;
-    const float c13 = S11 * R31 + S12 * R32 + S13 * R33;
-    const float c22 = S21 * R21 + S22 * R22 + S23 * R23;
-    const float c23 = S21 * R31 + S22 * R32 + S23 * R33;
-    const float c33 = S31 * R31 + S32 * R32 + S33 * R33;
+    c.c11 = S11 * R11 + S12 * R12 + S13 * R13;
+    c.c12 = S11 * R21 + S12 * R22 + S13 * R23;
+    c.c13 = S11 * R31 + S12 * R32 + S13 * R33;
+    c.c22 = S21 * R21 + S22 * R22 + S23 * R23;
+    c.c23 = S21 * R31 + S22 * R32 + S23 * R33;
+    c.c33 = S31 * R31 + S32 * R32 + S33 * R33;
+    return true;
+}
 
+// View-dependent part of the EWA projection (gs_utils.h:80-193) for a Gaussian at view
+// depth `depth` (already inside [near, far]); returns false if culled.
+__device__ __forceinline__ bool project_cov3d(float mx, float my, float mz, float depth, const Cov3d& cv,
+                                              const float4 r1, const float4 r2, const float4 r3, const FbCamera& cam,
+                                              Projected& o)
+{
+    float opacity = cv.opacity;
+    const float c11 = cv.c11, c12 = cv.c12, c13 = cv.c13, c22 = cv.c22, c23 = cv.c23, c33 = cv.c33;
     const float x = (r1.x * mx + r1.y * my + r1.z * mz + r1.w) / depth;
     const float y = (r2.x * mx + r2.y * my + r2.z * mz + r2.w) / depth;
 
-    const float clip_left = (-0.15f * cam.width - cam.center_x) / cam.focal_x;
-    const float clip_right = (1.15f * cam.width - cam.center_x) / cam.focal_x;
-    const float clip_top = (-0.15f * cam.height - cam.center_y) / cam.focal_y;
-    const float clip_bottom = (1.15f * cam.height - cam.center_y) / cam.focal_y;
-    const float xc = fmaxf(clip_left, fminf(x, clip_right));
-    const float yc = fmaxf(clip_top, fminf(y, clip_bottom));
+    const float xc = fmaxf(cam.clip_left, fminf(x, cam.clip_right));
+    const float yc = fmaxf(cam.clip_top, fminf(y, cam.clip_bottom));
     const float j11 = cam.focal_x / depth;
     const float j13 = -j11 * xc;
     const float j22 = cam.focal_y / depth;
@@ -402,6 +410,44 @@ __device__ __forceinline__ bool project_gaussian(const float* __restrict__ means
     o.cov_y = cxy;
     o.cov_z = cyy;
     return true;
+}
+
+// EWA projection with culling (gs_utils.h:38-193); returns false if culled.
+// w2c rows r1, r2, r3 (row-major, 4 floats each).
+__device__ __forceinline__ bool project_gaussian(const float* __restrict__ means, const float* __restrict__ scales,
+                                                 const float* __restrict__ rotations,
+                                                 const float* __restrict__ opacities, uint32_t g,
+                                                 const float4 r1, const float4 r2, const float4 r3,
+                                                 const FbCamera& cam, Projected& o)
+{
+    const float mx = means[3u * g + 0u], my = means[3u * g + 1u], mz = means[3u * g + 2u];
+    const float depth = view_depth(means, g, r3);
+    if (!(depth >= cam.near_plane && depth <= cam.far_plane)) return false;
+    Cov3d cv;
+    if (!gaussian_cov3d(scales, rotations, opacities, g, cv)) return false;
+    return project_cov3d(mx, my, mz, depth, cv, r1, r2, r3, cam, o);
+}
+
+// Cached view-independent part: two float4 per Gaussian, (c11, c12, c13, c22) and
+// (c23, c33, opacity, 0); opacity < 0 marks a Gaussian culled in every view.
+__device__ __forceinline__ void store_cov3d(float4* __restrict__ cache, uint32_t g, bool valid, const Cov3d& c)
+{
+    cache[2u * g] = make_float4(c.c11, c.c12, c.c13, c.c22);
+    cache[2u * g + 1u] = make_float4(c.c23, c.c33, valid ? c.opacity : -1.0f, 0.0f);
+}
+
+// project_gaussian with the view-independent part read from the cache (bit-identical)
+__device__ __forceinline__ bool project_cached(const float* __restrict__ means, const float4* __restrict__ cache,
+                                               uint32_t g, const float4 r1, const float4 r2, const float4 r3,
+                                               const FbCamera& cam, Projected& o)
+{
+    const float mx = means[3u * g + 0u], my = means[3u * g + 1u], mz = means[3u * g + 2u];
+    const float depth = view_depth(means, g, r3);
+    if (!(depth >= cam.near_plane && depth <= cam.far_plane)) return false;
+    const float4 a = cache[2u * g], b = cache[2u * g + 1u];
+    if (b.z < 0.0f) return false;
+    const Cov3d cv = { a.x, a.y, a.z, a.w, b.x, b.y, b.z };
+    return project_cov3d(mx, my, mz, depth, cv, r1, r2, r3, cam, o);
 }
 
 // ------------------------------------------------------------------------------------------

@@ -181,7 +181,8 @@ struct TileBuffers {
     uint32_t* sorted_instances; // [capacity] each tile's instances, nearest depth bucket first
     uint32_t* sorted_keys;      // [capacity] their depth bucket floors (non-decreasing per tile)
     float4* centroids;          // [n_chunks] mean position of each chunk of 256 Gaussians
-    uint32_t* chunk_order;      // [n_views * n_chunks] chunks of each view, nearest first
+    float4* cov3d;              // [2 * n] view-independent projection terms of each Gaussian
+    uint2* view_keys;           // [n_views] depth-key bucketing of each view (kmin, shift)
     uint32_t n_tiles, capacity;
 };
 hipError_t launch_batch_tiles(const SceneArgs& sc, const ViewDesc* views, uint32_t n_views, const uint32_t* tile_prefix,
