@@ -171,13 +171,19 @@ def roofline_scatter(scene_n: int, views, ms, work):
     }
 
 
-def roofline_tiles(views, raster_ms: list, n_instances: int):
-    """Algorithmic bytes of one tile-rasteriser launch: per (view, tile, Gaussian) instance
-    4 B of its index and 44 B of Gaussian parameters (means, scales, rotation, opacity: the
-    instance is re-projected in its tile's workgroup), and 8 B per framebuffer word written
-    (every sample of every pixel, written once)."""
+# HBM bytes per launch of the tile rasteriser on the default workload (6M Gaussians, 64 views,
+# 8 samples per LDS pass), from rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0 (calibrated on a
+# 1 GiB device copy); the commands and raw counters are in profiles/tile_raster_c4_6M_64views.md.
+PROFILED_RASTER_TRAFFIC = {"bytes": 7.17e9, "profile": "profiles/tile_raster_c4_6M_64views.md"}
+
+
+def roofline_tiles(views, raster_ms: list, n_instances: int, visible_pairs: int, default_workload: bool):
+    """Algorithmic bytes of one tile-rasteriser launch: 44 B of Gaussian parameters per
+    visible (view, Gaussian) pair (means 12 B + the cached covariance and opacity 32 B, read
+    at least once), 8 B per (view, tile, Gaussian) list entry (index + depth bucket) and 8 B
+    per framebuffer word written (every sample of every pixel, once)."""
     words = sum(cam["width"] * cam["height"] * cam["spp"] for cam in views)
-    b = 48.0 * n_instances + 8.0 * words
+    b = 44.0 * visible_pairs + 8.0 * n_instances + 8.0 * words
     t = sum(raster_ms) / len(raster_ms) * 1e-3
     achieved = b / t / 1e9
     return {
@@ -187,10 +193,13 @@ def roofline_tiles(views, raster_ms: list, n_instances: int):
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "traffic": PROFILED_RASTER_TRAFFIC["bytes"] if default_workload else None,
+        "traffic_source": PROFILED_RASTER_TRAFFIC["profile"] if default_workload else None,
+        "issue_bound": "VALU (1.0 wave-instruction per CU-cycle, SQ counters in the same profile)",
         "avg_launch_ms": round(t * 1e3, 3),
         "algorithmic_bytes_per_launch": int(b),
         "instances_per_launch": int(n_instances),
+        "visible_pairs_per_launch": int(visible_pairs),
         "framebuffer_words_per_launch": int(words),
     }
 
@@ -292,7 +301,9 @@ def main() -> int:
     roof = roof_plain
     if "batched" in result:
         if args.schedule == "tiles":
-            roof = roofline_tiles(views, acc_ms, result["batched"]["tile_instances"])
+            default_workload = (spec.n == 6_000_000 and len(views) == 64 and args.tile_samples == 8)
+            roof = roofline_tiles(views, acc_ms, result["batched"]["tile_instances"], result["batched"]["visible_pairs"],
+                                  default_workload)
             log(f"tile raster: {roof['avg_launch_ms']} ms, {roof['achieved']} GB/s algorithmic")
         else:
             roof = roofline_batch(spec.n, views, acc_ms, work)
@@ -414,7 +425,7 @@ def bench_batched(args, scene, views, S, plain_out0, dev):
              "accumulate_kernel_ms": round(float(np.mean(main_ms)), 3),
              "accumulate_total_ms": round(float(np.mean([e5[1].elapsed_time(e5[3]) for e5 in ev])), 3),
              "resolve_ms": round(float(np.mean([e5[3].elapsed_time(e5[4]) for e5 in ev])), 3)}
-    n_inst, overflowed = br.tile_stats()
+    n_inst, n_visible, overflowed = br.tile_stats()
     log(f"stages {stage}; tile instances {n_inst} (overflow {overflowed})")
     graph = br.capture()
     torch.cuda.synchronize()
@@ -428,7 +439,7 @@ def bench_batched(args, scene, views, S, plain_out0, dev):
         g_ms.append((time.perf_counter() - ts) * 1e3)
     return {"value": round(S * args.steps / batch_s / 1e6, 2), "ms_per_step": round(batch_s * 1e3 / args.steps, 2),
             "graph_ms_per_step": round(float(np.mean(g_ms)), 2), "stages": stage,
-            "tile_instances": n_inst, "tile_overflow": overflowed,
+            "tile_instances": n_inst, "visible_pairs": n_visible, "tile_overflow": overflowed,
             "max_abs_diff_vs_plain": max_diff}, main_ms
 
 

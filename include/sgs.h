@@ -214,10 +214,12 @@ int sgs_render_batch(const sgs_gaussians* gaussians, const sgs_view* views, uint
                      void* workspace, size_t workspace_bytes, void* const* stage_events, void* stream);
 
 /* After sgs_render_batch on this workspace (waits for the stream): the number of
- * (view, tile, Gaussian) instances the tile schedule binned, and whether they exceeded
- * max_tile_instances (then the scatter schedule produced the result). Lets a caller size
- * max_tile_instances from a previous frame. */
-int sgs_batch_tile_stats(const void* workspace, void* stream, uint64_t* n_instances, uint32_t* overflowed);
+ * (view, tile, Gaussian) instances the tile schedule binned, the (view, Gaussian) pairs that
+ * touched at least one tile, and whether the instances exceeded max_tile_instances (then
+ * the scatter schedule produced the result). Lets a caller size max_tile_instances from a
+ * previous frame. */
+int sgs_batch_tile_stats(const void* workspace, void* stream, uint64_t* n_instances, uint64_t* visible_pairs,
+                         uint32_t* overflowed);
 
 #ifdef __cplusplus
 }

@@ -97,7 +97,7 @@ _lib.sgs_render_batch.argtypes = [_P(sgs_gaussians), C.c_void_p, C.c_uint32, C.c
                                   C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p]
 
 _lib.sgs_batch_tile_stats.restype = C.c_int
-_lib.sgs_batch_tile_stats.argtypes = [C.c_void_p, C.c_void_p, _P(C.c_uint64), _P(C.c_uint32)]
+_lib.sgs_batch_tile_stats.argtypes = [C.c_void_p, C.c_void_p, _P(C.c_uint64), _P(C.c_uint64), _P(C.c_uint32)]
 
 k_buffer_size: int = _lib.sgs_k_buffer_size()
 

@@ -154,15 +154,17 @@ class BatchRenderer:
         _native._check(rc, "sgs_render_batch")
         return self.outputs
 
-    def tile_stats(self) -> tuple[int, bool]:
-        """(instances binned by the last render's tile schedule, whether they overflowed
-        max_tile_instances); waits for the current stream."""
-        n, o = C.c_uint64(0), C.c_uint32(0)
+    def tile_stats(self) -> tuple[int, int, bool]:
+        """(instances binned by the last render's tile schedule, (view, Gaussian) pairs that
+        touched a tile, whether the instances overflowed max_tile_instances); waits for the
+        current stream."""
+        n, vis, o = C.c_uint64(0), C.c_uint64(0), C.c_uint32(0)
         with torch.cuda.device(self.device):
             stream = torch.cuda.current_stream(self.device).cuda_stream
-            rc = _native._lib.sgs_batch_tile_stats(self.workspace.data_ptr(), stream, C.byref(n), C.byref(o))
+            rc = _native._lib.sgs_batch_tile_stats(self.workspace.data_ptr(), stream, C.byref(n), C.byref(vis),
+                                                   C.byref(o))
         _native._check(rc, "sgs_batch_tile_stats")
-        return int(n.value), bool(o.value)
+        return int(n.value), int(vis.value), bool(o.value)
 
     def capture(self) -> "torch.cuda.CUDAGraph":
         """Captures render() into a graph (HIP graph on ROCm); replay() re-renders with the

@@ -622,9 +622,10 @@ int sgs_render_batch(const sgs_gaussians* gaussians, const sgs_view* views, uint
     return SGS_SUCCESS;
 }
 
-int sgs_batch_tile_stats(const void* workspace, void* stream_ptr, uint64_t* n_instances, uint32_t* overflowed)
+int sgs_batch_tile_stats(const void* workspace, void* stream_ptr, uint64_t* n_instances, uint64_t* visible_pairs,
+                         uint32_t* overflowed)
 {
-    if (!workspace || !n_instances || !overflowed)
+    if (!workspace || !n_instances || !visible_pairs || !overflowed)
         return fail(SGS_ERROR_INVALID_ARGUMENT, "workspace and outputs must not be NULL");
     BatchCounters c;
     hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
@@ -632,6 +633,7 @@ int sgs_batch_tile_stats(const void* workspace, void* stream_ptr, uint64_t* n_in
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
     if (e != hipSuccess) return hip_fail(e, "counter read");
     *n_instances = c.total_instances;
+    *visible_pairs = c.visible_pairs;
     *overflowed = c.overflow;
     return SGS_SUCCESS;
 }

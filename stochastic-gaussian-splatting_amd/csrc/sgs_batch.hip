@@ -416,7 +416,7 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
                                                                 BatchCounters* __restrict__ ctr)
 {
     __shared__ uint32_t s_wx0, s_wx1, s_wy0, s_wy1;
-    __shared__ unsigned long long s_total;
+    __shared__ unsigned long long s_total, s_visible;
     __shared__ uint32_t hist[kWindow];
     __shared__ uint32_t hbase[kWindow];
     if constexpr (kFill) {
@@ -424,7 +424,10 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
     }
     const uint32_t tid = threadIdx.x, lane = lane_id();
     const uint32_t n_chunks = (sc.n + kBatchBlock - 1u) / kBatchBlock;
-    if (tid == 0) s_total = 0ull;
+    if (tid == 0) {
+        s_total = 0ull;
+        s_visible = 0ull;
+    }
     // items are dealt round-robin: a shared claim counter would serialise ~n_items atomics
     // on one address, and the per-item work is nearly uniform
     for (uint32_t item = blockIdx.x; item < n_items; item += gridDim.x) {
@@ -462,6 +465,8 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
             uint32_t sum = n_t;  // a Gaussian touches < 2^24 tiles, so a wave's sum fits 32 bits
             for (int off = 32; off > 0; off >>= 1) sum += (uint32_t)__shfl_xor((int)sum, off, 64);
             if (lane == 0u && sum) atomicAdd(&s_total, (unsigned long long)sum);
+            const uint32_t vis = (uint32_t)__popcll(__ballot(has));
+            if (lane == 0u && vis) atomicAdd(&s_visible, (unsigned long long)vis);
         }
         // large Gaussians: the wave walks each one's tiles
         uint64_t big = __ballot(has && !small);
@@ -522,6 +527,7 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
     }
     if constexpr (!kFill) {
         if (tid == 0 && s_total) atomicAdd(&ctr->total_instances, s_total);
+        if (tid == 0 && s_visible) atomicAdd(&ctr->visible_pairs, s_visible);
     }
 }
 

@@ -114,7 +114,8 @@ struct BatchCounters {
     uint32_t n_instances;       // (view, tile, Gaussian) instances of the tile schedule
     uint32_t overflow;          // instances exceed the capacity: the scatter schedule runs instead
     unsigned long long total_instances;  // 64-bit instance total of the counting pass
-    uint32_t pad[56];
+    unsigned long long visible_pairs;    // (view, Gaussian) pairs with at least one tile
+    uint32_t pad[54];
 };
 static_assert(sizeof(BatchCounters) == 256, "BatchCounters layout");
 
