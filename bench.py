@@ -40,6 +40,12 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--gaussians", type=int, default=6_000_000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU oracle sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=6.0,
+                    help="length of the serving run at 80%% of saturated load (0: skip serving)")
+    ap.add_argument("--serve-streams", type=int, default=32)
+    ap.add_argument("--serve-inflight", type=int, default=1, help="batches on the GPU at once")
+    ap.add_argument("--serve-window-ms", type=float, default=35.0,
+                    help="batching window: a free slot waits until the oldest request is this old")
     ap.add_argument("--only", choices=["plain", "batched", "all"], default="all",
                     help="run only one path (profiling); the JSON line then reports that path")
     ap.add_argument("--schedule", choices=["tiles", "scatter"], default="tiles",
@@ -252,6 +258,134 @@ def cpu_baseline(scene_np, views, budget_s: float):
                       f"accumulate+sample+resolve, {dt:.1f} s"}
 
 
+# ---------------------------------------------------------------------------------------------
+# serving: 32 streams on smooth trajectories, Poisson arrivals, continuous batching
+# ---------------------------------------------------------------------------------------------
+SERVE_SIZES = [(1280, 720), (1920, 1080), (2560, 1440)]
+SERVE_SPP = [1, 2, 8]
+
+
+def serving_streams(n_streams: int, seed: int = 3):
+    import numpy as np
+    from StochasticGSCudaBackend.serve import StreamSpec
+    from StochasticGSCudaBackend.synthetic import camera, orbit_eye
+    rng = np.random.default_rng(seed)
+    specs, paths = [], []
+    for _ in range(n_streams):
+        w, h = SERVE_SIZES[int(rng.integers(len(SERVE_SIZES)))]
+        c = camera(orbit_eye(0.0, 3.0), w, h)
+        specs.append(StreamSpec(w, h, c["focal_x"], c["focal_y"], c["center_x"], c["center_y"], c["near_plane"],
+                                c["far_plane"]))
+        # orbit phase, angular speed (rad/s), radius, height amplitude
+        paths.append((rng.uniform(0, 2 * math.pi), rng.uniform(0.2, 0.6) * rng.choice([-1, 1]),
+                      rng.uniform(2.6, 3.4), rng.uniform(-0.3, 0.3)))
+    return specs, paths
+
+
+def serving_requests(specs, paths, times, seed: int):
+    """One request per arrival time: a uniformly chosen stream, its pose at that time, spp
+    from {1, 2, 8}."""
+    import numpy as np
+    from StochasticGSCudaBackend.serve import Request
+    from StochasticGSCudaBackend.synthetic import camera, orbit_eye
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for t in times:
+        s = int(rng.integers(len(specs)))
+        phase, omega, radius, height = paths[s]
+        eye = orbit_eye(phase + omega * t, radius, height * math.sin(0.5 * t))
+        c = camera(eye, specs[s].width, specs[s].height)
+        reqs.append(Request(stream=s, w2c=c["w2c"], cam_position=c["cam_position"],
+                            spp=int(SERVE_SPP[int(rng.integers(len(SERVE_SPP)))]), arrival=float(t)))
+    return reqs
+
+
+def bench_serving(args, scene, dev):
+    """Saturated throughput (every request queued at t=0), then Poisson arrivals at 80 % of it:
+    p50/p99 latency (arrival to completion seen by the host). The plain per-request path is
+    timed on the same requests (spp single-view render() calls each) and its latency at the
+    same arrivals follows from a FIFO single-server queue over those service times."""
+    import numpy as np
+    import torch
+    from StochasticGSCudaBackend import _native
+    from StochasticGSCudaBackend.serve import Server, latency_stats
+    specs, paths = serving_streams(args.serve_streams)
+    server = Server(scene["means"], scene["scales"], scene["rotations"], scene["opacities"], scene["sh"], specs,
+                    max_views=args.serve_streams, inflight=args.serve_inflight,
+                    max_wait=args.serve_window_ms * 1e-3)
+    # warm-up and saturation: 4 frames per stream queued at once
+    warm = serving_requests(specs, paths, [0.0] * args.serve_streams, seed=10)
+    server.run(warm)
+    sat = serving_requests(specs, paths, [0.0] * (4 * args.serve_streams), seed=11)
+    t0 = time.perf_counter()
+    server.run(sat)
+    sat_s = time.perf_counter() - t0
+    sat_rate = len(sat) / sat_s
+    log(f"serving saturated: {sat_rate:.1f} req/s ({server.batches} batches so far)")
+    # 80 % load, Poisson arrivals
+    rate = 0.8 * sat_rate
+    rng = np.random.default_rng(12)
+    n = max(1, int(rate * args.serve_seconds))
+    times = np.cumsum(rng.exponential(1.0 / rate, n))
+    load = serving_requests(specs, paths, times, seed=13)
+    b0 = server.batches
+    t0 = time.perf_counter()
+    done = server.run(load)
+    run_s = time.perf_counter() - t0
+    stats = latency_stats(done)
+    samples = sum(specs[r.stream].width * specs[r.stream].height * r.spp for r in done)
+    batches = server.batches - b0
+    log(f"serving at {rate:.1f} req/s: p50 {stats['p50_ms']} ms, p99 {stats['p99_ms']} ms, "
+        f"{len(done) / batches:.1f} requests per batch")
+    # plain per-request path: service time of each request rendered alone
+    sample_reqs = load[:min(len(load), 96)]
+    hist_cache = {}
+    service = []
+    kb = (torch.empty((0, 0, 2), dtype=torch.int32, device=dev), torch.empty((0, 0, 4), device=dev))
+    eye4 = torch.eye(4, device=dev)
+    noise = torch.empty(0, dtype=torch.uint8, device=dev)
+    torch.cuda.synchronize()
+    for r in sample_reqs:
+        sp = specs[r.stream]
+        if (sp.width, sp.height) not in hist_cache:
+            hist_cache[(sp.width, sp.height)] = tuple(torch.zeros((sp.height, sp.width, c), device=dev)
+                                                      for c in (3, 1, 1, 1))
+        hist = hist_cache[(sp.width, sp.height)]
+        w2c = torch.from_numpy(np.asarray(r.w2c, np.float32)).to(dev)
+        cp = torch.from_numpy(np.asarray(r.cam_position, np.float32)).to(dev)
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        h = hist
+        for k in range(r.spp):
+            h = _native.render(scene["means"], scene["scales"], scene["rotations"], scene["opacities"], scene["sh"],
+                               w2c, cp, sp.width, sp.height, sp.focal_x, sp.focal_y, sp.center_x, sp.center_y,
+                               sp.near_plane, sp.far_plane, False, 1, False, False, False, noise, kb[0], kb[1], *h,
+                               eye4, 2 if k == 0 else 1, frame_idx=r.frame_idx + k)
+        torch.cuda.synchronize()
+        service.append(time.perf_counter() - ts)
+    mean_service = float(np.mean(service))
+    svc = np.resize(np.array(service), len(load))
+    arr = np.array([r.arrival for r in load]) - load[0].arrival
+    done_t, lat = 0.0, []
+    for a, sv in zip(arr, svc):
+        done_t = max(done_t, a) + sv
+        lat.append(done_t - a)
+    lat = np.array(lat) * 1e3
+    del server
+    torch.cuda.empty_cache()
+    return {"streams": args.serve_streams, "inflight": args.serve_inflight,
+            "window_ms": args.serve_window_ms, "saturated_req_per_s": round(sat_rate, 1),
+            "offered_req_per_s": round(rate, 1), "completed_req_per_s": round(len(done) / run_s, 1),
+            "msamples_per_s": round(samples / run_s / 1e6, 1), "requests": len(done),
+            "requests_per_batch": round(len(done) / max(batches, 1), 2),
+            "p50_ms": stats["p50_ms"], "p99_ms": stats["p99_ms"],
+            "p99_over_p50": round(stats["p99_ms"] / max(stats["p50_ms"], 1e-9), 3),
+            "plain": {"req_per_s_capacity": round(1.0 / mean_service, 1),
+                      "p50_ms_at_offered_load": round(float(np.percentile(lat, 50)), 3),
+                      "p99_ms_at_offered_load": round(float(np.percentile(lat, 99)), 3),
+                      "service_ms_mean": round(mean_service * 1e3, 3)}}
+
+
 def main() -> int:
     args = parse()
     if args.gpus != 1:
@@ -291,6 +425,9 @@ def main() -> int:
         result["plain"], plain_out0 = bench_plain(args, scene, views, settings, k_buffers, S)
     if args.only in ("batched", "all"):
         result["batched"], acc_ms = bench_batched(args, scene, views, S, plain_out0, dev)
+    serving = None
+    if args.only == "all" and args.serve_seconds > 0:
+        serving = bench_serving(args, scene, dev)
         if "plain" in result:
             result["batched"]["speedup_vs_plain"] = round(result["batched"]["value"] / result["plain"]["value"], 3)
 
@@ -339,6 +476,7 @@ def main() -> int:
         "p99_frame_ms": result["plain"]["p99_frame_ms"] if "plain" in result else None,
         "roofline": roof,
         "roofline_plain": roof_plain,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -202,7 +202,11 @@ size_t sgs_render_batch_workspace_size(const sgs_view* views, uint32_t n_views, 
 size_t sgs_batch_view_table_bytes(uint32_t n_views);
 int sgs_batch_encode_views(const sgs_view* views, uint32_t n_views, void* host_table);
 
-/* Renders all views. view_table_device: the encoded table already in device memory, or
+/* Renders all views. A view's history may be the four output buffers of an earlier view of
+ * the same batch (consecutive frames of one camera stream): such views are resolved after
+ * the view they read, in stream order; any other overlap between one view's outputs and
+ * another view's outputs or history is rejected. view_table_device: the encoded table already
+ * in device memory, or
  * NULL to let the library copy it into the workspace (a pageable host-to-device copy,
  * not capturable into a graph). stage_events: NULL, or 5 hipEvent_t (as void*) recorded
  * on the stream before the counter and framebuffer clears, before the accumulation

@@ -3,7 +3,9 @@
 // binding of render.cu:16-165.
 #include <cmath>
 #include <cstring>
+#include <algorithm>
 #include <string>
+#include <vector>
 
 #include "sgs_internal.h"
 
@@ -411,6 +413,18 @@ BatchLayout batch_layout(const sgs_view* views, uint32_t n, uint32_t n_gaussians
     return L;
 }
 
+// resolve level of each view: 1 + the level of the earlier view whose outputs it reads
+std::vector<uint32_t> view_levels(const sgs_view* views, uint32_t n)
+{
+    std::vector<uint32_t> level(n, 0u);
+    for (uint32_t i = 0; i < n; i++) {
+        if (views[i].temporal_mode == SGS_TEMPORAL_RESET) continue;
+        for (uint32_t j = 0; j < i; j++)
+            if (views[j].out.image == views[i].history.image) level[i] = std::max(level[i], level[j] + 1u);
+    }
+    return level;
+}
+
 int check_views(const sgs_view* views, uint32_t n)
 {
     if (!views && n) return fail(SGS_ERROR_INVALID_ARGUMENT, "views is NULL");
@@ -438,6 +452,27 @@ int check_views(const sgs_view* views, uint32_t n)
         blocks += ((uint64_t)v.width * v.height + kResolveBlock - 1u) / kResolveBlock;
     }
     if (blocks >= (1ull << 31)) return fail(SGS_ERROR_INVALID_ARGUMENT, "too many output pixels in one batch");
+    // a view may read an earlier view's outputs as its history (all four buffers); no view
+    // may write buffers another view writes or reads otherwise
+    for (uint32_t i = 0; i < n; i++) {
+        const sgs_history& o = views[i].out;
+        for (uint32_t j = 0; j < n; j++) {
+            if (j == i) continue;
+            const sgs_history& oj = views[j].out;
+            if (o.image == oj.image || o.depth == oj.depth || o.alpha == oj.alpha || o.weight_mass == oj.weight_mass)
+                return fail(SGS_ERROR_INVALID_ARGUMENT,
+                            "views " + std::to_string(i) + " and " + std::to_string(j) + " write the same outputs");
+            if (views[j].temporal_mode == SGS_TEMPORAL_RESET) continue;
+            const sgs_history& h = views[j].history;
+            const int shared = (o.image == h.image) + (o.depth == h.depth) + (o.alpha == h.alpha) +
+                               (o.weight_mass == h.weight_mass);
+            if (shared == 0) continue;
+            if (shared != 4 || i > j)
+                return fail(SGS_ERROR_INVALID_ARGUMENT,
+                            "view " + std::to_string(j) + " reads buffers of view " + std::to_string(i) +
+                                "; a history may only be all outputs of an earlier view");
+        }
+    }
     uint64_t tiles = 0;
     for (uint32_t i = 0; i < n; i++) tiles += (uint64_t)view_grid_w(views[i]) * view_grid_h(views[i]);
     if (tiles >= (1ull << 31)) return fail(SGS_ERROR_INVALID_ARGUMENT, "too many framebuffer tiles in one batch");
@@ -451,6 +486,7 @@ void encode_views(const sgs_view* views, uint32_t n, void* host)
     uint32_t* tile_prefix = prefix + n + 1u;
     uint64_t fb_off = 0, smp_off = 0;
     uint32_t block = 0, tile = 0;
+    const std::vector<uint32_t> level = view_levels(views, n);
     for (uint32_t i = 0; i < n; i++) {
         const sgs_view& v = views[i];
         ViewDesc x;
@@ -476,6 +512,7 @@ void encode_views(const sgs_view* views, uint32_t n, void* host)
         x.tile_begin = tile;
         x.grid_w = view_grid_w(v);
         x.grid_h = view_grid_h(v);
+        x.level = level[i];
         x.fb_offset = fb_off;
         x.samples_offset = smp_off;
         x.hist = { v.history.image, v.history.depth, v.history.alpha, v.history.weight_mass };
@@ -616,8 +653,12 @@ int sgs_render_batch(const sgs_gaussians* gaussians, const sgs_view* views, uint
         }
         fb_off += P * v.spp;
     }
-    e = launch_batch_resolve(sc, desc, n_views, prefix, L.n_blocks, fb, samples, stream);
-    if (e != hipSuccess) return hip_fail(e, "batched resolve");
+    const std::vector<uint32_t> level = view_levels(views, n_views);
+    const uint32_t levels = 1u + *std::max_element(level.begin(), level.end());
+    for (uint32_t l = 0; l < levels; l++) {
+        e = launch_batch_resolve(sc, desc, n_views, prefix, L.n_blocks, fb, samples, l, stream);
+        if (e != hipSuccess) return hip_fail(e, "batched resolve");
+    }
     if ((e = mark(4)) != hipSuccess) return hip_fail(e, "stage event");
     return SGS_SUCCESS;
 }

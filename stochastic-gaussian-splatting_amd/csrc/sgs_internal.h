@@ -98,6 +98,7 @@ struct alignas(16) ViewDesc {
     uint32_t block_begin;       // first resolve block of this view
     uint32_t tile_begin;        // first 16x16 tile of this view in the batch's tile numbering
     uint32_t grid_w, grid_h;    // tiles per row / column of this view's framebuffer
+    uint32_t level;             // resolve order: 1 + the level of the view whose output is this view's history
     uint64_t fb_offset;         // first framebuffer word of sample 0 (samples are P words apart)
     uint64_t samples_offset;    // REPROJECT views: first float4 of the sample-0 buffer
     HistoryPtrs hist;
@@ -204,7 +205,7 @@ hipError_t run_hybrid_gps(const SceneArgs& sc, const float* w2c, const FbCamera&
 // ---- launchers (sgs_resolve.hip) ----
 hipError_t launch_batch_resolve(const SceneArgs& sc, const ViewDesc* views, uint32_t n_views,
                                 const uint32_t* block_prefix, uint32_t n_blocks, const uint64_t* fb,
-                                const float4* samples, hipStream_t stream);
+                                const float4* samples, uint32_t level, hipStream_t stream);
 hipError_t launch_sample(const SceneArgs& sc, const float* w2c, const float* cam_position, const FbCamera& cam,
                          const uint64_t* fb, float4* samples, float4* reuse_weights, uint2* k_prim, float4* k_occ,
                          bool spatial_reuse, hipStream_t stream);

@@ -526,7 +526,8 @@ __global__ __launch_bounds__(kBlockResolve) void batch_resolve_kernel(SceneArgs 
                                                                       uint32_t n_views,
                                                                       const uint32_t* __restrict__ block_prefix,
                                                                       const uint64_t* __restrict__ fb,
-                                                                      const float4* __restrict__ samples)
+                                                                      const float4* __restrict__ samples,
+                                                                      uint32_t level)
 {
     // view of this block: last v with block_prefix[v] <= blockIdx.x
     uint32_t lo = 0, hi = n_views - 1u;
@@ -536,6 +537,8 @@ __global__ __launch_bounds__(kBlockResolve) void batch_resolve_kernel(SceneArgs 
         else hi = mid - 1u;
     }
     const ViewDesc& vd = views[lo];
+    // views whose history is another view's output of the batch resolve in a later launch
+    if (vd.level != level) return;
     const uint32_t pix = (blockIdx.x - block_prefix[lo]) * kBlockResolve + threadIdx.x;
     if (pix >= vd.width * vd.height) return;
     const size_t P = (size_t)vd.cam.width_i * vd.cam.height_i;
@@ -597,14 +600,14 @@ __global__ __launch_bounds__(kBlockResolve) void batch_resolve_kernel(SceneArgs 
 
 hipError_t launch_batch_resolve(const SceneArgs& sc, const ViewDesc* views, uint32_t n_views,
                                 const uint32_t* block_prefix, uint32_t n_blocks, const uint64_t* fb,
-                                const float4* samples, hipStream_t stream)
+                                const float4* samples, uint32_t level, hipStream_t stream)
 {
     if (n_blocks == 0 || n_views == 0) return hipSuccess;
     switch (sc.sh_bases) {
-        case 1: hipLaunchKernelGGL((batch_resolve_kernel<1>), dim3(n_blocks), dim3(kBlockResolve), 0, stream, sc, views, n_views, block_prefix, fb, samples); break;
-        case 4: hipLaunchKernelGGL((batch_resolve_kernel<4>), dim3(n_blocks), dim3(kBlockResolve), 0, stream, sc, views, n_views, block_prefix, fb, samples); break;
-        case 9: hipLaunchKernelGGL((batch_resolve_kernel<9>), dim3(n_blocks), dim3(kBlockResolve), 0, stream, sc, views, n_views, block_prefix, fb, samples); break;
-        default: hipLaunchKernelGGL((batch_resolve_kernel<16>), dim3(n_blocks), dim3(kBlockResolve), 0, stream, sc, views, n_views, block_prefix, fb, samples); break;
+        case 1: hipLaunchKernelGGL((batch_resolve_kernel<1>), dim3(n_blocks), dim3(kBlockResolve), 0, stream, sc, views, n_views, block_prefix, fb, samples, level); break;
+        case 4: hipLaunchKernelGGL((batch_resolve_kernel<4>), dim3(n_blocks), dim3(kBlockResolve), 0, stream, sc, views, n_views, block_prefix, fb, samples, level); break;
+        case 9: hipLaunchKernelGGL((batch_resolve_kernel<9>), dim3(n_blocks), dim3(kBlockResolve), 0, stream, sc, views, n_views, block_prefix, fb, samples, level); break;
+        default: hipLaunchKernelGGL((batch_resolve_kernel<16>), dim3(n_blocks), dim3(kBlockResolve), 0, stream, sc, views, n_views, block_prefix, fb, samples, level); break;
     }
     return hipGetLastError();
 }

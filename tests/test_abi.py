@@ -113,7 +113,8 @@ def _batch_views(native, n=2):
         v.near_plane, v.far_plane = 0.2, 100.0
         v.supersampling_factor = 1
         v.spp = 1 + 3 * i
-        v.out = native.sgs_history(0x10000, 0x20000, 0x30000, 0x40000)
+        base = 0x10000 + 0x100000 * i
+        v.out = native.sgs_history(base, base + 0x10000, base + 0x20000, base + 0x30000)
     return views
 
 
@@ -138,3 +139,25 @@ def test_batch_workspace_and_schedule_validation(native):
     assert rc == -1 and b"schedule" in lib.sgs_last_error()
     host = (C.c_uint8 * lib.sgs_batch_view_table_bytes(2))()
     assert lib.sgs_batch_encode_views(views, 2, host) == 0
+
+
+def test_batch_output_aliasing_rules(native):
+    """Host-side checks of sgs_batch_encode_views: a view may read an earlier view's four
+    outputs as its history; shared outputs, partial overlap or reading a later view fail."""
+    lib = native.library()
+    host = (C.c_uint8 * lib.sgs_batch_view_table_bytes(2))()
+    views = _batch_views(native)
+    assert lib.sgs_batch_encode_views(views, 2, host) == 0
+    views[1].temporal_mode = 2
+    views[1].history = views[0].out  # chain: view 1 reprojects view 0's result
+    assert lib.sgs_batch_encode_views(views, 2, host) == 0
+    views[1].history = native.sgs_history(views[0].out.image, views[0].out.depth, views[0].out.alpha,
+                                          0x9990000)  # partial overlap
+    assert lib.sgs_batch_encode_views(views, 2, host) == -1
+    views[1].temporal_mode = 0
+    views[0].temporal_mode = 2
+    views[0].history = views[1].out  # reads a later view's outputs
+    assert lib.sgs_batch_encode_views(views, 2, host) == -1
+    views[0].temporal_mode = 0
+    views[1].out = views[0].out  # two views write the same buffers
+    assert lib.sgs_batch_encode_views(views, 2, host) == -1

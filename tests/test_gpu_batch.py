@@ -4,6 +4,8 @@ loop; the tile and scatter schedules, their overflow paths and graph replays giv
 same bits."""
 from __future__ import annotations
 
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -143,6 +145,49 @@ def test_batch_temporal_modes(gpu):
     torch.cuda.synchronize()
     assert_close(outs[0], plain_spp(gpu, t, cam0, 20, 2, mode=1, history=hist))
     assert_close(outs[1], plain_spp(gpu, t, cam1, 30, 3, mode=2, history=hist, v2p=v2p))
+
+
+def test_batch_history_chain(gpu):
+    """Consecutive frames of one camera in one batch: view k reads view k-1's outputs as its
+    history (resolved after it), equal to rendering the frames one after another."""
+    import torch
+    from StochasticGSCudaBackend import _native as N
+    arrays = small_scene(2500, seed=47, log_scale_mean=-3.3)
+    t = to_device(arrays)
+    cams = [small_camera(72, 52, angle=0.3 + 0.03 * k) for k in range(3)]
+    spps, frames, modes = [2, 1, 3], [40, 42, 43], [0, 2, 2]
+    opts = dict(dtype=torch.float32, device="cuda")
+    outs = [tuple(torch.empty((52, 72, c), **opts) for c in (3, 1, 1, 1)) for _ in cams]
+    views = (N.sgs_view * 3)()
+    v2ps = [None]
+    for k in range(1, 3):
+        v2ps.append((np.asarray(cams[k - 1]["w2c"], np.float64) @ np.linalg.inv(
+            np.asarray(cams[k]["w2c"], np.float64))).astype(np.float32))
+    for k, c in enumerate(cams):
+        v = views[k]
+        v.w2c[:] = np.asarray(c["w2c"], np.float32).reshape(16).tolist()
+        v.cam_position[:] = np.asarray(c["cam_position"], np.float32).tolist()
+        v.width, v.height = c["width"], c["height"]
+        v.focal_x, v.focal_y, v.center_x, v.center_y = c["focal_x"], c["focal_y"], c["center_x"], c["center_y"]
+        v.near_plane, v.far_plane = c["near_plane"], c["far_plane"]
+        v.supersampling_factor = 1
+        v.spp, v.frame_idx, v.temporal_mode = spps[k], frames[k], modes[k]
+        v.view_to_previous_view[:] = (np.eye(4, dtype=np.float32) if v2ps[k] is None else v2ps[k]).reshape(16).tolist()
+        v.out = N.sgs_history(*[o.data_ptr() for o in outs[k]])
+        if k:
+            v.history = N.sgs_history(*[o.data_ptr() for o in outs[k - 1]])
+    keep = []
+    g = N.gaussians_struct(t["means"], t["scales"], t["rotations"], t["opacities"], t["sh"], keep)
+    ws = N._workspace(N._lib.sgs_render_batch_workspace_size(views, 3, g.n_gaussians, None), torch.device("cuda"))
+    print_case(test="batch_history_chain")
+    rc = N._lib.sgs_render_batch(C.byref(g), views, 3, None, None, ws.data_ptr(), ws.numel(), None,
+                                 torch.cuda.current_stream().cuda_stream)
+    N._check(rc, "sgs_render_batch")
+    torch.cuda.synchronize()
+    hist = None
+    for k, c in enumerate(cams):
+        hist = plain_spp(gpu, t, c, frames[k], spps[k], mode=modes[k], history=hist, v2p=v2ps[k])
+        assert_close(outs[k], hist)
 
 
 def test_batch_matches_oracle(gpu, oracle_mod):
