@@ -389,7 +389,8 @@ def bench_serving(args, scene, dev):
 def main() -> int:
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (sharding is described in DESIGN.md)"}))
+        print(json.dumps({"error": "this renderer runs on one GPU; --gpus must be 1 (views and requests would shard "
+                                   "across GPUs with the scene replicated and no exchange; not built)"}))
         return 2
     import numpy as np
     import torch

@@ -317,7 +317,7 @@ class Server:
             self.pump()
         torch.cuda.synchronize(self.device)
 
-    # ---------------------------------------------------------------- driver
+    # ---------------------------------------------------------------- real-time feed
     def run(self, requests: Sequence[Request], t0: float | None = None) -> list[Request]:
         """Feeds ``requests`` (``arrival`` in seconds after ``t0``, rewritten to the absolute
         clock) in real time and returns them when all have finished; latency = completion seen
