@@ -51,7 +51,9 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--schedule", choices=["tiles", "scatter"], default="tiles",
                     help="batched accumulation: LDS tiles (default) or persistent scatter")
     ap.add_argument("--tile-samples", type=int, choices=[8, 16], default=8,
-                    help="samples per LDS pass of the tile rasteriser")
+                    help="samples per LDS pass of the tile rasteriser (with --no-autotune)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="use --schedule/--tile-samples instead of tuning them on the workload first")
     ap.add_argument("--view-major", action="store_true",
                     help="batched work items view by view instead of Gaussian chunk by chunk")
     ap.add_argument("--copy-probe", action="store_true",
@@ -524,6 +526,18 @@ def bench_batched(args, scene, views, S, plain_out0, dev):
     bviews = [BatchView(host_settings(cam), spp=cam["spp"], frame_idx=0) for cam in views]
     tuning = BatchTuning(schedule=SCHEDULE_TILES if args.schedule == "tiles" else SCHEDULE_SCATTER,
                          tile_samples=args.tile_samples, view_major=int(args.view_major))
+    tuned = None
+    if not args.no_autotune:
+        from StochasticGSCudaBackend.autotune import tune_batch
+        ts = time.perf_counter()
+        res = tune_batch(scene["means"], scene["scales"], scene["rotations"], scene["opacities"], scene["sh"], bviews,
+                         repeats=2)
+        tuning = res.best._replace(view_major=int(args.view_major))
+        args.schedule = "tiles" if tuning.schedule in (0, SCHEDULE_TILES) else "scatter"
+        args.tile_samples = tuning.tile_samples or 8
+        tuned = {"best": dict(tuning._asdict()), "seconds": round(time.perf_counter() - ts, 1),
+                 "trials": [[{k: v for k, v in c._asdict().items() if v}, round(ms, 3)] for c, ms in res.trials]}
+        log(f"autotuned batch: {tuned['best']} from {tuned['trials']}")
     br = BatchRenderer(scene["means"], scene["scales"], scene["rotations"], scene["opacities"], scene["sh"], bviews,
                        tuning)
     log(f"batched workspace {br.workspace.numel() / 2**30:.2f} GiB")
@@ -578,7 +592,7 @@ def bench_batched(args, scene, views, S, plain_out0, dev):
         g_ms.append((time.perf_counter() - ts) * 1e3)
     return {"value": round(S * args.steps / batch_s / 1e6, 2), "ms_per_step": round(batch_s * 1e3 / args.steps, 2),
             "graph_ms_per_step": round(float(np.mean(g_ms)), 2), "stages": stage,
-            "tile_instances": n_inst, "visible_pairs": n_visible, "tile_overflow": overflowed,
+            "tile_instances": n_inst, "visible_pairs": n_visible, "tile_overflow": overflowed, "autotune": tuned,
             "max_abs_diff_vs_plain": max_diff}, main_ms
 
 

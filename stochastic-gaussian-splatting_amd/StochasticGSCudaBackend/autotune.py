@@ -1,0 +1,145 @@
+"""Per-scene autotuning of the scheduling knobs.
+
+None of the knobs searched here changes a result bit (the framebuffer winners are the
+closest accepted fragment per (sub)pixel whatever the schedule), so tuning is a pure timing
+search: a coordinate descent over the candidate values of each knob, timing the stage the
+knobs act on with HIP events on the current stream, a few repetitions per trial, keeping the
+median. The max_gps_points cap does change results and is never tuned.
+
+* ``tune_render`` tunes the single-view path for one camera: tile size, binning pixel and
+  point thresholds, max_binned_gaussians and the sequential threshold (Tuning), timing the
+  accumulation stage (sgs_accumulate) of the settings' own schedule.
+* ``tune_batch`` tunes a batched render: schedule and samples per LDS pass of the tile
+  schedule (BatchTuning), timing whole renders; the instance capacity is set from the
+  measured instance count with a margin, which shrinks the workspace.
+"""
+from __future__ import annotations
+
+import statistics
+from dataclasses import dataclass, field
+from typing import Callable, Sequence
+
+import torch
+
+from . import _native
+from .render import Settings, Tuning
+
+
+@dataclass
+class TuneResult:
+    best: object
+    ms: float
+    trials: list = field(default_factory=list)  # (config, ms) in evaluation order
+
+
+def coordinate_search(evaluate: Callable[[dict], float], space: dict[str, Sequence], start: dict,
+                      passes: int = 2) -> tuple[dict, float, list]:
+    """Minimises ``evaluate`` over the product of ``space`` by coordinate descent from
+    ``start``: each pass tries every value of each knob with the others fixed and keeps the
+    best; stops after ``passes`` passes or a pass without improvement. Each configuration
+    is evaluated at most once."""
+    seen: dict[tuple, float] = {}
+    trials: list = []
+
+    def cost(cfg: dict) -> float:
+        key = tuple(sorted(cfg.items()))
+        if key not in seen:
+            seen[key] = float(evaluate(dict(cfg)))
+            trials.append((dict(cfg), seen[key]))
+        return seen[key]
+
+    best = dict(start)
+    best_ms = cost(best)
+    for _ in range(passes):
+        improved = False
+        for knob, values in space.items():
+            for v in values:
+                cand = dict(best)
+                cand[knob] = v
+                ms = cost(cand)
+                if ms < best_ms:
+                    best, best_ms, improved = cand, ms, True
+        if not improved:
+            break
+    return best, best_ms, trials
+
+
+def _time_ms(fn: Callable[[], None], repeats: int) -> float:
+    stream = torch.cuda.current_stream()
+    fn()  # warm-up (also allocates)
+    times = []
+    for _ in range(repeats):
+        start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record(stream)
+        fn()
+        end.record(stream)
+        end.synchronize()
+        times.append(start.elapsed_time(end))
+    return statistics.median(times)
+
+
+RENDER_SPACE = {
+    "tile_size": [16, 32, 64],
+    "binning_pixel_threshold": [128, 256, 512, 1024, 2048, 4096],
+    "binning_point_threshold": [128, 256, 512, 1024, 2048],
+    "max_binned_gaussians": [1 << 14, 1 << 16, 1 << 18, 1 << 20],
+    "sequential_threshold": [2, 4, 8, 16],
+}
+
+
+def tune_render(means, scales, rotations, opacities, sh_coefficients, settings: Settings, *, frame_idx: int = 0,
+                repeats: int = 3, space: dict | None = None, passes: int = 2) -> TuneResult:
+    """Fastest Tuning for the accumulation of ``settings`` (its own GPS / hybrid choice).
+    Knobs that do not act on that schedule are left at their defaults."""
+    if space is None:
+        space = dict(RENDER_SPACE)
+        if not settings.hybrid_scheduling:
+            space = {"sequential_threshold": space["sequential_threshold"]}
+        elif settings.use_gps:
+            space.pop("tile_size")
+            space.pop("binning_pixel_threshold")
+        else:
+            space.pop("binning_point_threshold")
+    n = int(means.shape[0])
+    space = {k: [v for v in vals if k != "max_binned_gaussians" or v <= max(n, 1)] or [max(n, 1)]
+             for k, vals in space.items()}
+    w2c = settings.w2c.to(device=means.device, dtype=torch.float32)
+    cam = settings.cam_position.to(device=means.device, dtype=torch.float32)
+    fb = torch.empty((settings.height * max(1, settings.supersampling_factor),
+                      settings.width * max(1, settings.supersampling_factor)), dtype=torch.int64, device=means.device)
+    args = (means, scales, rotations, opacities, sh_coefficients, w2c, cam, *settings.as_tuple()[2:])
+
+    def evaluate(cfg: dict) -> float:
+        tuning = Tuning(**cfg)
+        return _time_ms(lambda: _native.accumulate(*args, frame_idx=frame_idx, tuning=tuning, framebuffer=fb), repeats)
+
+    start = {k: 0 for k in space}  # 0 = library default
+    best, ms, trials = coordinate_search(evaluate, {k: [0] + list(v) for k, v in space.items()}, start, passes)
+    return TuneResult(Tuning(**best), ms, [(Tuning(**c), t) for c, t in trials])
+
+
+BATCH_SPACE = {"schedule": [1, 2], "tile_samples": [8, 16]}
+
+
+def tune_batch(means, scales, rotations, opacities, sh_coefficients, views, *, repeats: int = 2,
+               space: dict | None = None, capacity_margin: float = 1.25) -> TuneResult:
+    """Fastest BatchTuning for rendering ``views`` (BatchView list) in one call; the tile
+    instance capacity of the result is the measured instance count x ``capacity_margin``."""
+    from .batch import BatchRenderer, BatchTuning
+    space = dict(BATCH_SPACE if space is None else space)
+
+    def evaluate(cfg: dict) -> float:
+        r = BatchRenderer(means, scales, rotations, opacities, sh_coefficients, views, BatchTuning(**cfg))
+        ms = _time_ms(r.render, repeats)
+        del r
+        return ms
+
+    best, ms, trials = coordinate_search(evaluate, space, {k: v[0] for k, v in space.items()}, passes=1)
+    if best.get("schedule", 1) == 1:
+        probe = BatchRenderer(means, scales, rotations, opacities, sh_coefficients, views, BatchTuning(**best))
+        probe.render()
+        n_inst, _, overflowed = probe.tile_stats()
+        del probe
+        if not overflowed and n_inst > 0:
+            best["max_tile_instances"] = min(int(n_inst * capacity_margin) + 1, (1 << 31))
+    return TuneResult(BatchTuning(**best), ms, [(BatchTuning(**c), t) for c, t in trials])

@@ -1,0 +1,35 @@
+"""CPU tests of the autotuner's coordinate search (no GPU): finds the optimum of a
+separable cost, evaluates each configuration once, respects the pass limit."""
+from __future__ import annotations
+
+import pytest
+
+
+@pytest.fixture(scope="module")
+def search():
+    from StochasticGSCudaBackend.autotune import coordinate_search
+    return coordinate_search
+
+
+def test_separable_optimum(search):
+    space = {"a": [1, 2, 3, 4], "b": [10, 20, 30]}
+    cost = lambda c: (c["a"] - 3) ** 2 + (c["b"] - 20) ** 2 / 100.0
+    best, ms, trials = search(cost, space, {"a": 1, "b": 10})
+    assert best == {"a": 3, "b": 20} and ms == 0.0
+    keys = [tuple(sorted(c.items())) for c, _ in trials]
+    assert len(keys) == len(set(keys))
+
+
+def test_coupled_cost_needs_second_pass(search):
+    # optimum (2, 2) is reachable only after b moves: cost punishes a=2 unless b=2
+    def cost(c):
+        return {(1, 1): 5, (2, 1): 6, (1, 2): 4, (2, 2): 1}[(c["a"], c["b"])]
+    best, ms, _ = search(cost, {"a": [1, 2], "b": [1, 2]}, {"a": 1, "b": 1}, passes=2)
+    assert best == {"a": 2, "b": 2} and ms == 1
+    best1, ms1, _ = search(cost, {"a": [1, 2], "b": [1, 2]}, {"a": 1, "b": 1}, passes=1)
+    assert ms1 <= 4
+
+
+def test_defaults_are_candidates(search):
+    best, ms, trials = search(lambda c: 1.0, {"x": [5, 6]}, {"x": 0})
+    assert best == {"x": 0} and trials[0][0] == {"x": 0}
