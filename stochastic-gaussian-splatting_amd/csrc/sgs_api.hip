@@ -363,7 +363,7 @@ uint32_t view_grid_h(const sgs_view& v) { return (uint32_t)((v.height * view_ss(
 // cursors are contiguous so one fill clears them all.
 struct BatchLayout {
     size_t counters, ready, tile_count, tile_cursor, cleared_end, offsets, instances, keys, sorted_instances, sorted_keys,
-        centroids, cov3d, view_keys, queue, table, fb, samples, total;
+        centroids, cov3d, pair_rect, pair_key, view_keys, queue, table, fb, samples, total;
     uint64_t fb_words, n_samples, n_tiles;
     uint32_t n_blocks, capacity;
 };
@@ -405,6 +405,8 @@ BatchLayout batch_layout(const sgs_view* views, uint32_t n, uint32_t n_gaussians
     L.centroids = a.take(16u * n_chunks);
     L.view_keys = a.take(8u * (size_t)n);
     L.cov3d = a.take(tiles ? 32u * (size_t)n_gaussians : 0u);
+    L.pair_rect = a.take(tiles ? 8u * (size_t)n_gaussians * n : 0u);
+    L.pair_key = a.take(tiles ? 4u * (size_t)n_gaussians * n : 0u);
     L.queue = a.take(sizeof(BigEntry) * (size_t)t.queue_capacity);
     L.table = a.take(table_bytes(n));
     L.fb = a.take(8u * L.fb_words);
@@ -617,6 +619,8 @@ int sgs_render_batch(const sgs_gaussians* gaussians, const sgs_view* views, uint
         tb.centroids = reinterpret_cast<float4*>(ws + L.centroids);
         tb.view_keys = reinterpret_cast<uint2*>(ws + L.view_keys);
         tb.cov3d = reinterpret_cast<float4*>(ws + L.cov3d);
+        tb.pair_rect = reinterpret_cast<uint2*>(ws + L.pair_rect);
+        tb.pair_key = reinterpret_cast<uint32_t*>(ws + L.pair_key);
         tb.n_tiles = (uint32_t)L.n_tiles;
         tb.capacity = L.capacity;
         e = launch_batch_tiles(sc, desc, n_views, tile_prefix, tb, t.tile_samples, counters, fb,

@@ -346,13 +346,18 @@ struct TileRange {
     uint32_t tx0, tx1, ty0, ty1;  // inclusive
 };
 
-// tile range and depth key (float bits of the view depth) of Gaussian g in view vd (false
-// if culled or its AABB is empty)
-__device__ __forceinline__ bool gaussian_tiles(const SceneArgs& sc, const float4* __restrict__ cov3d,
-                                               const ViewDesc& vd, uint32_t g, TileRange& t, uint32_t& key)
+// tile range and depth key (float bits of the view depth) in view vd of a Gaussian whose mean
+// and cached view-independent terms are in registers (the operations of project_cached, so
+// the same bits); false if culled or its AABB is empty
+__device__ __forceinline__ bool gaussian_tiles_reg(float mx, float my, float mz, const Cov3d& cv, bool valid,
+                                                   const ViewDesc& vd, TileRange& t, uint32_t& key)
 {
+    if (!valid) return false;
+    const float4 r3 = vd.r3;
+    const float depth = r3.x * mx + r3.y * my + r3.z * mz + r3.w;
+    if (!(depth >= vd.cam.near_plane && depth <= vd.cam.far_plane)) return false;
     Projected p;
-    if (!(g < sc.n && project_cached(sc.means, cov3d, g, vd.r1, vd.r2, vd.r3, vd.cam, p))) return false;
+    if (!project_cov3d(mx, my, mz, depth, cv, vd.r1, vd.r2, r3, vd.cam, p)) return false;
     key = f2u(p.depth);
     const Aabb b = pixel_aabb(p, vd.cam);
     if (b.x1 <= b.x0 || b.y1 <= b.y0) return false;
@@ -393,7 +398,6 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t x)
     return x;
 }
 
-// Items are (view, chunk of 256 Gaussians), view by view.
 // view-independent part of every Gaussian's projection, once per render
 __global__ __launch_bounds__(kBatchBlock) void cov3d_kernel(SceneArgs sc, float4* __restrict__ cache)
 {
@@ -404,15 +408,21 @@ __global__ __launch_bounds__(kBatchBlock) void cov3d_kernel(SceneArgs sc, float4
     store_cov3d(cache, g, valid, c);
 }
 
+// tile coordinates fit 16 bits (framebuffers are at most 65535 pixels wide); a pair with no
+// tiles is marked by all ones
+constexpr uint32_t kNoTiles = 0xffffffffu;
+
 template <bool kFill>
 __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, const float4* __restrict__ cov3d,
                                                                 const ViewDesc* __restrict__ views,
-                                                                uint32_t n_views, uint32_t n_items,
+                                                                uint32_t n_views,
                                                                 uint32_t* __restrict__ tile_count,
                                                                 uint32_t* __restrict__ tile_cursor,
                                                                 const uint32_t* __restrict__ offsets,
                                                                 uint32_t* __restrict__ instances,
                                                                 uint32_t* __restrict__ keys,
+                                                                uint2* __restrict__ pair_rect,
+                                                                uint32_t* __restrict__ pair_key,
                                                                 BatchCounters* __restrict__ ctr)
 {
     __shared__ uint32_t s_wx0, s_wx1, s_wy0, s_wy1;
@@ -428,102 +438,132 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
         s_total = 0ull;
         s_visible = 0ull;
     }
-    // items are dealt round-robin: a shared claim counter would serialise ~n_items atomics
-    // on one address, and the per-item work is nearly uniform
-    for (uint32_t item = blockIdx.x; item < n_items; item += gridDim.x) {
-        if (tid == 0) {
-            s_wx0 = 0xffffffffu;
-            s_wy0 = 0xffffffffu;
-            s_wx1 = 0u;
-            s_wy1 = 0u;
-        }
-        __syncthreads();
-        const uint32_t v = item / n_chunks;
-        const uint32_t chunk = item - v * n_chunks;
-        const ViewDesc& vd = views[v];
+    // Chunks are dealt round-robin (a shared claim counter would serialise one atomic per
+    // chunk on one address; the per-chunk work is nearly uniform). A lane loads its
+    // Gaussian's parameters once and bins it for every view in turn, so the parameter
+    // stream is read from memory once per pass instead of once per view. The counting pass
+    // projects and records each (view, Gaussian) pair's tile rectangle and depth key; the
+    // fill pass reads them back instead of projecting again.
+    for (uint32_t chunk = blockIdx.x; chunk < n_chunks; chunk += gridDim.x) {
         const uint32_t g = chunk * kBatchBlock + tid;
-        TileRange tr;
-        uint32_t key = 0;
-        const bool has = gaussian_tiles(sc, cov3d, vd, g, tr, key);
-        const uint32_t tw = has ? tr.tx1 - tr.tx0 + 1u : 0u;
-        const uint32_t n_t = has ? tw * (tr.ty1 - tr.ty0 + 1u) : 0u;
-        const bool small = has && n_t <= kSmallTiles;
-        {
-            // window of the small Gaussians: wave reductions, one LDS atomic per wave
-            const uint32_t mx0 = wave_min_u32(small ? tr.tx0 : 0xffffffffu);
-            const uint32_t my0 = wave_min_u32(small ? tr.ty0 : 0xffffffffu);
-            const uint32_t mx1 = wave_max_u32(small ? tr.tx1 : 0u);
-            const uint32_t my1 = wave_max_u32(small ? tr.ty1 : 0u);
-            if (lane == 0u && mx0 <= mx1) {
-                atomicMin(&s_wx0, mx0);
-                atomicMax(&s_wx1, mx1);
-                atomicMin(&s_wy0, my0);
-                atomicMax(&s_wy1, my1);
+        float mx = 0.0f, my = 0.0f, mz = 0.0f;
+        Cov3d cv = { 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, -1.0f };
+        if (!kFill && g < sc.n) {
+            mx = sc.means[3u * g + 0u];
+            my = sc.means[3u * g + 1u];
+            mz = sc.means[3u * g + 2u];
+            const float4 a = cov3d[2u * g], b = cov3d[2u * g + 1u];
+            cv = { a.x, a.y, a.z, a.w, b.x, b.y, b.z };
+        }
+        const bool valid = g < sc.n && !(cv.opacity < 0.0f);
+        for (uint32_t v = 0; v < n_views; v++) {
+            if (tid == 0) {
+                s_wx0 = 0xffffffffu;
+                s_wy0 = 0xffffffffu;
+                s_wx1 = 0u;
+                s_wy1 = 0u;
             }
-        }
-        if constexpr (!kFill) {
-            uint32_t sum = n_t;  // a Gaussian touches < 2^24 tiles, so a wave's sum fits 32 bits
-            for (int off = 32; off > 0; off >>= 1) sum += (uint32_t)__shfl_xor((int)sum, off, 64);
-            if (lane == 0u && sum) atomicAdd(&s_total, (unsigned long long)sum);
-            const uint32_t vis = (uint32_t)__popcll(__ballot(has));
-            if (lane == 0u && vis) atomicAdd(&s_visible, (unsigned long long)vis);
-        }
-        // large Gaussians: the wave walks each one's tiles
-        uint64_t big = __ballot(has && !small);
-        while (big) {
-            const int l = __builtin_ctzll(big);
-            big &= big - 1ull;
-            const uint32_t bx0 = bcast(tr.tx0, l), by0 = bcast(tr.ty0, l), bw = bcast(tw, l), bn = bcast(n_t, l);
-            const uint32_t bg = bcast(g, l), bkey = bcast(key, l);
-            for (uint32_t k = lane; k < bn; k += 64u) {
-                const uint32_t ty = by0 + k / bw, tx = bx0 + k % bw;
-                bin_direct<kFill>(vd.tile_begin + ty * vd.grid_w + tx, bg, bkey, tile_count, tile_cursor, offsets,
-                                  instances, keys);
-            }
-        }
-        __syncthreads();
-        const uint32_t wx0 = s_wx0, wx1 = s_wx1, wy0 = s_wy0, wy1 = s_wy1;
-        if (wx0 <= wx1) {
-            const uint32_t wt = wx1 - wx0 + 1u, ht = wy1 - wy0 + 1u;
-            if (wt * ht <= kWindow) {
-                const uint32_t nw = wt * ht;
-                for (uint32_t i = tid; i < nw; i += kBatchBlock) hist[i] = 0u;
-                __syncthreads();
-                if (small)
-                    for (uint32_t ty = tr.ty0; ty <= tr.ty1; ty++)
-                        for (uint32_t tx = tr.tx0; tx <= tr.tx1; tx++) atomicAdd(&hist[(ty - wy0) * wt + tx - wx0], 1u);
-                __syncthreads();
-                for (uint32_t i = tid; i < nw; i += kBatchBlock) {
-                    const uint32_t c = hist[i];
-                    if (!c) continue;
-                    const uint32_t gt = vd.tile_begin + (wy0 + i / wt) * vd.grid_w + wx0 + i % wt;
-                    if constexpr (kFill) {
-                        hbase[i] = offsets[gt] + atomicAdd(&tile_cursor[gt], c);
-                        hist[i] = 0u;
-                    } else {
-                        atomicAdd(&tile_count[gt], c);
-                    }
+            __syncthreads();
+            const ViewDesc& vd = views[v];
+            TileRange tr = { 0u, 0u, 0u, 0u };
+            uint32_t key = 0;
+            bool has = false;
+            const size_t pair = (size_t)v * sc.n + g;
+            if constexpr (kFill) {
+                if (g < sc.n) {
+                    const uint2 r = pair_rect[pair];
+                    has = r.x != kNoTiles;
+                    tr = { r.x & 0xffffu, r.y & 0xffffu, r.x >> 16, r.y >> 16 };
+                    key = pair_key[pair];
                 }
-                if constexpr (kFill) {
+            } else {
+                has = gaussian_tiles_reg(mx, my, mz, cv, valid, vd, tr, key);
+                if (g < sc.n) {
+                    pair_rect[pair] = has ? make_uint2(tr.tx0 | (tr.ty0 << 16), tr.tx1 | (tr.ty1 << 16))
+                                          : make_uint2(kNoTiles, kNoTiles);
+                    pair_key[pair] = key;
+                }
+            }
+            const uint32_t tw = has ? tr.tx1 - tr.tx0 + 1u : 0u;
+            const uint32_t n_t = has ? tw * (tr.ty1 - tr.ty0 + 1u) : 0u;
+            const bool small = has && n_t <= kSmallTiles;
+            {
+                // window of the small Gaussians: wave reductions, one LDS atomic per wave
+                const uint32_t mx0 = wave_min_u32(small ? tr.tx0 : 0xffffffffu);
+                const uint32_t my0 = wave_min_u32(small ? tr.ty0 : 0xffffffffu);
+                const uint32_t mx1 = wave_max_u32(small ? tr.tx1 : 0u);
+                const uint32_t my1 = wave_max_u32(small ? tr.ty1 : 0u);
+                if (lane == 0u && mx0 <= mx1) {
+                    atomicMin(&s_wx0, mx0);
+                    atomicMax(&s_wx1, mx1);
+                    atomicMin(&s_wy0, my0);
+                    atomicMax(&s_wy1, my1);
+                }
+            }
+            if constexpr (!kFill) {
+                uint32_t sum = n_t;  // a Gaussian touches < 2^24 tiles, so a wave's sum fits 32 bits
+                for (int off = 32; off > 0; off >>= 1) sum += (uint32_t)__shfl_xor((int)sum, off, 64);
+                if (lane == 0u && sum) atomicAdd(&s_total, (unsigned long long)sum);
+                const uint32_t vis = (uint32_t)__popcll(__ballot(has));
+                if (lane == 0u && vis) atomicAdd(&s_visible, (unsigned long long)vis);
+            }
+            // large Gaussians: the wave walks each one's tiles
+            uint64_t big = __ballot(has && !small);
+            while (big) {
+                const int l = __builtin_ctzll(big);
+                big &= big - 1ull;
+                const uint32_t bx0 = bcast(tr.tx0, l), by0 = bcast(tr.ty0, l), bw = bcast(tw, l), bn = bcast(n_t, l);
+                const uint32_t bg = bcast(g, l), bkey = bcast(key, l);
+                for (uint32_t k = lane; k < bn; k += 64u) {
+                    const uint32_t ty = by0 + k / bw, tx = bx0 + k % bw;
+                    bin_direct<kFill>(vd.tile_begin + ty * vd.grid_w + tx, bg, bkey, tile_count, tile_cursor, offsets,
+                                      instances, keys);
+                }
+            }
+            __syncthreads();
+            const uint32_t wx0 = s_wx0, wx1 = s_wx1, wy0 = s_wy0, wy1 = s_wy1;
+            if (wx0 <= wx1) {
+                const uint32_t wt = wx1 - wx0 + 1u, ht = wy1 - wy0 + 1u;
+                if (wt * ht <= kWindow) {
+                    const uint32_t nw = wt * ht;
+                    for (uint32_t i = tid; i < nw; i += kBatchBlock) hist[i] = 0u;
                     __syncthreads();
                     if (small)
                         for (uint32_t ty = tr.ty0; ty <= tr.ty1; ty++)
-                            for (uint32_t tx = tr.tx0; tx <= tr.tx1; tx++) {
-                                const uint32_t i = (ty - wy0) * wt + tx - wx0;
-                                const uint32_t slot = hbase[i] + atomicAdd(&hist[i], 1u);
-                                instances[slot] = g;
-                                keys[slot] = key;
-                            }
+                            for (uint32_t tx = tr.tx0; tx <= tr.tx1; tx++) atomicAdd(&hist[(ty - wy0) * wt + tx - wx0], 1u);
+                    __syncthreads();
+                    for (uint32_t i = tid; i < nw; i += kBatchBlock) {
+                        const uint32_t c = hist[i];
+                        if (!c) continue;
+                        const uint32_t gt = vd.tile_begin + (wy0 + i / wt) * vd.grid_w + wx0 + i % wt;
+                        if constexpr (kFill) {
+                            hbase[i] = offsets[gt] + atomicAdd(&tile_cursor[gt], c);
+                            hist[i] = 0u;
+                        } else {
+                            atomicAdd(&tile_count[gt], c);
+                        }
+                    }
+                    if constexpr (kFill) {
+                        __syncthreads();
+                        if (small)
+                            for (uint32_t ty = tr.ty0; ty <= tr.ty1; ty++)
+                                for (uint32_t tx = tr.tx0; tx <= tr.tx1; tx++) {
+                                    const uint32_t i = (ty - wy0) * wt + tx - wx0;
+                                    const uint32_t slot = hbase[i] + atomicAdd(&hist[i], 1u);
+                                    instances[slot] = g;
+                                    keys[slot] = key;
+                                }
+                    }
+                } else if (small) {
+                    // spread-out chunk: one global atomic per instance
+                    for (uint32_t ty = tr.ty0; ty <= tr.ty1; ty++)
+                        for (uint32_t tx = tr.tx0; tx <= tr.tx1; tx++)
+                            bin_direct<kFill>(vd.tile_begin + ty * vd.grid_w + tx, g, key, tile_count, tile_cursor,
+                                              offsets, instances, keys);
                 }
-            } else if (small) {
-                // spread-out chunk: one global atomic per instance
-                for (uint32_t ty = tr.ty0; ty <= tr.ty1; ty++)
-                    for (uint32_t tx = tr.tx0; tx <= tr.tx1; tx++)
-                        bin_direct<kFill>(vd.tile_begin + ty * vd.grid_w + tx, g, key, tile_count, tile_cursor,
-                                          offsets, instances, keys);
             }
+            __syncthreads();
         }
-        __syncthreads();
     }
     if constexpr (!kFill) {
         if (tid == 0 && s_total) atomicAdd(&ctr->total_instances, s_total);
@@ -976,7 +1016,7 @@ hipError_t launch_batch_tiles(const SceneArgs& sc, const ViewDesc* views, uint32
                                                      kBatchBlock, 0);
     if (e != hipSuccess) return e;
     uint64_t grid = (uint64_t)cus * (per_cu < 1 ? 1 : per_cu);
-    if (grid > n_items64) grid = n_items64;
+    if (grid > n_chunks) grid = n_chunks;
     if (grid < 1) grid = 1;
     if (sc.n > 0) {
         hipLaunchKernelGGL(chunk_centroid_kernel, dim3(n_chunks), dim3(kBatchBlock), 0, stream, sc, tb.centroids);
@@ -984,16 +1024,16 @@ hipError_t launch_batch_tiles(const SceneArgs& sc, const ViewDesc* views, uint32
                            n_chunks, tb.view_keys);
         hipLaunchKernelGGL(cov3d_kernel, dim3(n_chunks), dim3(kBatchBlock), 0, stream, sc, tb.cov3d);
         hipLaunchKernelGGL((batch_bin_kernel<false>), dim3((unsigned)grid), dim3(kBatchBlock), 0, stream, sc, tb.cov3d,
-                           views, n_views, (uint32_t)n_items64, tb.tile_count, tb.tile_cursor, tb.offsets,
-                           tb.instances, tb.keys, counters);
+                           views, n_views, tb.tile_count, tb.tile_cursor, tb.offsets,
+                           tb.instances, tb.keys, tb.pair_rect, tb.pair_key, counters);
     }
     hipLaunchKernelGGL((exclusive_scan_kernel<CountInput>), dim3(1), dim3(kScanThreads), 0, stream,
                        CountInput{ tb.tile_count }, tb.offsets, nullptr, tb.n_tiles, tb.n_tiles, &counters->n_instances,
                        &counters->overflow, tb.capacity, &counters->total_instances);
     if (sc.n > 0) {
         hipLaunchKernelGGL((batch_bin_kernel<true>), dim3((unsigned)grid), dim3(kBatchBlock), 0, stream, sc, tb.cov3d,
-                           views, n_views, (uint32_t)n_items64, tb.tile_count, tb.tile_cursor, tb.offsets,
-                           tb.instances, tb.keys, counters);
+                           views, n_views, tb.tile_count, tb.tile_cursor, tb.offsets,
+                           tb.instances, tb.keys, tb.pair_rect, tb.pair_key, counters);
     }
     hipLaunchKernelGGL(tile_sort_kernel, dim3(tb.n_tiles), dim3(kBatchBlock), 0, stream, n_views, tile_prefix,
                        tb.view_keys, tb.offsets, tb.keys, tb.instances, tb.sorted_keys, tb.sorted_instances, counters);
