@@ -840,6 +840,7 @@ __global__ __launch_bounds__(kBatchBlock, kLayers <= 8u ? 5 : 3) void batch_tile
     __shared__ uint32_t s_pmax[kTile * kTile];  // depth bits of each pixel's largest word
     __shared__ uint16_t s_queue[kBatchBlock / 64u][128];  // per-wave (instance << 8 | pixel) queue
     __shared__ float s_msx[kBatchBlock], s_msy[kBatchBlock], s_o[kBatchBlock];
+    __shared__ float s_thr[kBatchBlock];  // exponent below which a fragment's alpha is < 1/255
     __shared__ float s_ca[kBatchBlock], s_cb[kBatchBlock], s_cc[kBatchBlock];
     __shared__ uint32_t s_rect[kBatchBlock], s_end[kBatchBlock];
     __shared__ uint32_t s_wave[kBatchBlock / 64u];
@@ -912,6 +913,9 @@ __global__ __launch_bounds__(kBatchBlock, kLayers <= 8u ? 5 : 3) void batch_tile
                     s_msx[tid] = p.mean_x - 0.5f;
                     s_msy[tid] = p.mean_y - 0.5f;
                     s_o[tid] = p.opacity;
+                    // log2(kMinAlpha / opacity) less a margin of 0.01 that dwarfs the error of
+                    // this estimate, so no fragment passing the exact alpha test is ever below it
+                    s_thr[tid] = (log_f(kMinAlpha) - log_f(p.opacity)) * kLog2e - 0.01f;
                     s_ca[tid] = ce.a;
                     s_cb[tid] = ce.b;
                     s_cc[tid] = ce.c;
@@ -932,6 +936,8 @@ __global__ __launch_bounds__(kBatchBlock, kLayers <= 8u ? 5 : 3) void batch_tile
             uint32_t k = tid * per;
             const uint32_t kend = min(k + per, total);
             uint32_t j = 0, jend = 0, rect_j = 0, rw = 1, q = 0, r = 0, fdepth = 0;
+            float msx = 0.0f, msy = 0.0f, thr = 0.0f;
+            ConicExp2 ce = { 0.0f, 0.0f, 0.0f };
             if (k < kend) {
                 uint32_t bb = kBatchBlock - 1u;
                 while (j < bb) {
@@ -946,6 +952,10 @@ __global__ __launch_bounds__(kBatchBlock, kLayers <= 8u ? 5 : 3) void batch_tile
                 q = (f * (rect_j >> 12)) >> 16;  // f / rw for f < 256, rw <= 16
                 r = f - q * rw;
                 fdepth = (uint32_t)(s_frag[j] >> 32);
+                msx = s_msx[j];
+                msy = s_msy[j];
+                thr = s_thr[j];
+                ce = { s_ca[j], s_cb[j], s_cc[j] };
             }
             uint16_t* queue = s_queue[tid / 64u];
             uint32_t count = 0;  // wave-uniform
@@ -961,14 +971,22 @@ __global__ __launch_bounds__(kBatchBlock, kLayers <= 8u ? 5 : 3) void batch_tile
                         q = 0;
                         r = 0;
                         fdepth = (uint32_t)(s_frag[j] >> 32);
+                        msx = s_msx[j];
+                        msy = s_msy[j];
+                        thr = s_thr[j];
+                        ce = { s_ca[j], s_cb[j], s_cc[j] };
                     }
-                    const uint32_t l = (((rect_j >> 4) & 0xfu) + q) * kTile + (rect_j & 0xfu) + r;
+                    const uint32_t lx = (rect_j & 0xfu) + r, ly = ((rect_j >> 4) & 0xfu) + q;
+                    const uint32_t l = ly * kTile + lx;
                     if (++r == rw) {
                         r = 0;
                         q++;
                     }
-                    // some sample word of the pixel was behind the fragment at the batch start
-                    cand = fdepth <= s_pmax[l];
+                    // some sample word of the pixel was behind the fragment at the batch start,
+                    // and the fragment lies inside the instance's alpha >= 1/255 ellipse (the
+                    // exponent is the one the alpha evaluation computes)
+                    const float e = fragment_exponent(ce, (float)(ox + lx) - msx, (float)(oy + ly) - msy);
+                    cand = fdepth <= s_pmax[l] && !(e < thr);
                     entry = (j << 8) | l;
                 }
                 const uint64_t m = __ballot(cand);

@@ -462,11 +462,16 @@ __device__ __forceinline__ ConicExp2 conic_exp2(float cx, float cy, float cz)
     return { -0.5f * kLog2e * cx, -kLog2e * cy, -0.5f * kLog2e * cz };
 }
 
-__device__ __forceinline__ float fragment_alpha(const ConicExp2 ce, float opacity, float dx, float dy)
+// exponent of a fragment at offset (dx, dy) from the mean, in log2 units
+__device__ __forceinline__ float fragment_exponent(const ConicExp2 ce, float dx, float dy)
 {
     const float t = fmaf(ce.a, dx, ce.b * dy);
-    const float e = fmaf(dx, t, ce.c * dy * dy);
-    return opacity * exp2_f(fminf(e, 0.0f));
+    return fmaf(dx, t, ce.c * dy * dy);
+}
+
+__device__ __forceinline__ float fragment_alpha(const ConicExp2 ce, float opacity, float dx, float dy)
+{
+    return opacity * exp2_f(fminf(fragment_exponent(ce, dx, dy), 0.0f));
 }
 
 __device__ __forceinline__ uint64_t fragment_key(float depth, uint32_t g)
