@@ -225,6 +225,72 @@ int sgs_render_batch(const sgs_gaussians* gaussians, const sgs_view* views, uint
 int sgs_batch_tile_stats(const void* workspace, void* stream, uint64_t* n_instances, uint64_t* visible_pairs,
                          uint32_t* overflowed);
 
+/* ---------------------------------------------------------------------------------------
+ * Scene loading: 3DGS PLY vertex records -> renderer arrays in Morton order. Replaces the
+ * model loader StochasticGSModel.build (Model.py:22-79, one directory above the package):
+ * plyfile column reads + torch.as_tensor(float32) + the SH re-layout (Model.py:24-75), and
+ * morton_encode(means) + argsort + row indexing of every array (Model.py:32-34,43,53,58,
+ * 69,74). The host reads the file (and its header); the device decodes and reorders.
+ * --------------------------------------------------------------------------------------- */
+
+/* Scalar types of PLY properties (char/uchar/short/ushort/int/uint/float/double). */
+typedef enum sgs_scalar_type {
+    SGS_SCALAR_INT8 = 1,
+    SGS_SCALAR_UINT8 = 2,
+    SGS_SCALAR_INT16 = 3,
+    SGS_SCALAR_UINT16 = 4,
+    SGS_SCALAR_INT32 = 5,
+    SGS_SCALAR_UINT32 = 6,
+    SGS_SCALAR_FLOAT32 = 7,
+    SGS_SCALAR_FLOAT64 = 8,
+} sgs_scalar_type;
+
+/* Where the loader finds each field in one vertex record. Field order: x, y, z, scale_0..2,
+ * rot_0..3, opacity, f_dc_0..2, f_rest_0..f_rest_{3(B-1)-1} (11 + 3B fields; the names of
+ * Model.py:26-73). */
+#define SGS_PLY_MAX_FIELDS 59
+typedef struct sgs_ply_layout {
+    uint32_t stride;          /* bytes per vertex record */
+    uint32_t big_endian;      /* 1 for format binary_big_endian */
+    uint32_t total_sh_bases;  /* B = 1 + (number of f_rest_* properties) / 3, in {1,4,9,16} */
+    uint32_t offset[SGS_PLY_MAX_FIELDS]; /* byte offset of field f within a record */
+    uint32_t type[SGS_PLY_MAX_FIELDS];   /* its sgs_scalar_type */
+} sgs_ply_layout;
+
+/* Writable renderer arrays (same layout as sgs_gaussians, device memory). */
+typedef struct sgs_scene_arrays {
+    float* means;
+    float* scales;
+    float* rotations;
+    float* opacities;
+    float* sh_coefficients;
+    uint64_t n_gaussians;
+    uint32_t total_sh_bases;
+} sgs_scene_arrays;
+
+/* Decodes n_records vertex records (device copy of the file's records first ..
+ * first+n_records-1) into rows first.. of out: every field converted to float32 as
+ * numpy's astype(float32) does, SH re-laid from the file's channel-major f_rest_k,
+ * k = c*(B-1) + (b-1), to sh[row][b][c] (Model.py:61-75). Rows outside the range are not
+ * touched, so a file can be decoded in chunks. */
+int sgs_ply_decode(const void* records, uint64_t n_records, const sgs_ply_layout* layout, uint64_t first,
+                   const sgs_scene_arrays* out, void* stream);
+
+/* Morton order of n means (device (n,3) float32), n < 2^31. Code: the bounding box of the
+ * finite coordinates, its largest side as the cube edge, 2^21 cells per axis, x in bit 3k,
+ * y in 3k+1, z in 3k+2 (63 bits); order = stable sort of the codes (equal codes keep their
+ * index order). Replaces morton_encode + torch.argsort (Model.py:32-33); the NeRFICG
+ * morton_encode it calls is not in the reference tree, so this code is this library's own
+ * definition. order: device (n,) uint32; sorted_codes: device (n,) uint64 or NULL. */
+size_t sgs_morton_order_workspace_size(uint64_t n);
+int sgs_morton_order(const float* means, uint64_t n, uint32_t* order, uint64_t* sorted_codes, void* workspace,
+                     size_t workspace_bytes, void* stream);
+
+/* dst row i = src row order[i] for all five arrays (Model.py:34,43,53,58,69,74: array[order]).
+ * src and dst must not overlap; dst->n_gaussians rows are written (a row whose index is not
+ * below src->n_gaussians is left untouched). */
+int sgs_gather_gaussians(const sgs_gaussians* src, const uint32_t* order, const sgs_scene_arrays* dst, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -99,6 +99,31 @@ _lib.sgs_render_batch.argtypes = [_P(sgs_gaussians), C.c_void_p, C.c_uint32, C.c
 _lib.sgs_batch_tile_stats.restype = C.c_int
 _lib.sgs_batch_tile_stats.argtypes = [C.c_void_p, C.c_void_p, _P(C.c_uint64), _P(C.c_uint64), _P(C.c_uint32)]
 
+SGS_PLY_MAX_FIELDS = 59
+
+
+class sgs_ply_layout(C.Structure):
+    _fields_ = [("stride", C.c_uint32), ("big_endian", C.c_uint32), ("total_sh_bases", C.c_uint32),
+                ("offset", C.c_uint32 * SGS_PLY_MAX_FIELDS), ("type", C.c_uint32 * SGS_PLY_MAX_FIELDS)]
+
+
+class sgs_scene_arrays(C.Structure):
+    _fields_ = [("means", C.c_void_p), ("scales", C.c_void_p), ("rotations", C.c_void_p),
+                ("opacities", C.c_void_p), ("sh_coefficients", C.c_void_p), ("n_gaussians", C.c_uint64),
+                ("total_sh_bases", C.c_uint32)]
+
+
+_lib.sgs_ply_decode.restype = C.c_int
+_lib.sgs_ply_decode.argtypes = [C.c_void_p, C.c_uint64, _P(sgs_ply_layout), C.c_uint64, _P(sgs_scene_arrays),
+                                C.c_void_p]
+_lib.sgs_morton_order_workspace_size.restype = C.c_size_t
+_lib.sgs_morton_order_workspace_size.argtypes = [C.c_uint64]
+_lib.sgs_morton_order.restype = C.c_int
+_lib.sgs_morton_order.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,
+                                  C.c_void_p]
+_lib.sgs_gather_gaussians.restype = C.c_int
+_lib.sgs_gather_gaussians.argtypes = [_P(sgs_gaussians), C.c_void_p, _P(sgs_scene_arrays), C.c_void_p]
+
 k_buffer_size: int = _lib.sgs_k_buffer_size()
 
 
@@ -294,6 +319,55 @@ def accumulate(means, scales, rotations, opacities, sh_coefficients, w2c, cam_po
                                  tp, cptr, workspace.data_ptr() if nbytes else None, nbytes, stream)
         _check(rc, "sgs_accumulate")
     return framebuffer.view(FH, FW)
+
+
+def scene_arrays(arrays: dict) -> sgs_scene_arrays:
+    """sgs_scene_arrays of a dict with means, scales, rotations, opacities, sh (device float32)."""
+    return sgs_scene_arrays(arrays["means"].data_ptr(), arrays["scales"].data_ptr(), arrays["rotations"].data_ptr(),
+                            arrays["opacities"].data_ptr(), arrays["sh"].data_ptr(), int(arrays["means"].shape[0]),
+                            int(arrays["sh"].shape[1]))
+
+
+def ply_decode(records: torch.Tensor, layout: sgs_ply_layout, first: int, out: dict) -> None:
+    """Decodes the vertex records held in ``records`` (device uint8, whole records) into rows
+    ``first``.. of ``out`` on the current stream."""
+    n = records.numel() // max(int(layout.stride), 1)
+    o = scene_arrays(out)
+    stream = torch.cuda.current_stream(records.device).cuda_stream
+    _check(_lib.sgs_ply_decode(records.data_ptr() if n else None, n, C.byref(layout), int(first), C.byref(o), stream),
+           "sgs_ply_decode")
+
+
+def morton_order(means: torch.Tensor, return_codes: bool = False):
+    """Stable Morton order of ``means`` (device (N,3) float32) as int64 indices (and the
+    sorted 63-bit codes)."""
+    n = int(means.shape[0])
+    device = means.device
+    order = torch.empty(n, dtype=torch.int32, device=device)
+    codes = torch.empty(n, dtype=torch.int64, device=device)
+    if n:
+        nbytes = _lib.sgs_morton_order_workspace_size(n)
+        if nbytes == 0:
+            raise ValueError(f"morton_order: {n} means exceed the supported 2^31 - 1")
+        ws = _workspace(nbytes, device)
+        stream = torch.cuda.current_stream(device).cuda_stream
+        _check(_lib.sgs_morton_order(means.data_ptr(), n, order.data_ptr(), codes.data_ptr(), ws.data_ptr(), nbytes,
+                                     stream), "sgs_morton_order")
+    order = order.to(torch.int64)
+    return (order, codes) if return_codes else order
+
+
+def gather_gaussians(src: dict, order: torch.Tensor) -> dict:
+    """New arrays with row i = row order[i] of ``src`` (order: device int32, values < N)."""
+    n = int(order.shape[0])
+    out = {k: torch.empty((n, *v.shape[1:]), dtype=torch.float32, device=v.device) for k, v in src.items()}
+    if n:
+        keep: list = []
+        g = gaussians_struct(src["means"], src["scales"], src["rotations"], src["opacities"], src["sh"], keep)
+        o = scene_arrays(out)
+        stream = torch.cuda.current_stream(order.device).cuda_stream
+        _check(_lib.sgs_gather_gaussians(C.byref(g), order.data_ptr(), C.byref(o), stream), "sgs_gather_gaussians")
+    return out
 
 
 def library():

@@ -14,7 +14,7 @@ CSRC = Path(__file__).resolve().parent
 PKG = CSRC.parent
 REPO = PKG.parent
 OUT = PKG / "StochasticGSCudaBackend" / "libsgs.so"
-SOURCES = ["sgs_api.hip", "sgs_accumulate.hip", "sgs_batch.hip", "sgs_hybrid.hip", "sgs_resolve.hip"]
+SOURCES = ["sgs_api.hip", "sgs_accumulate.hip", "sgs_batch.hip", "sgs_hybrid.hip", "sgs_resolve.hip", "sgs_scene.hip"]
 ARCH = os.environ.get("SGS_OFFLOAD_ARCH", "gfx950")
 
 

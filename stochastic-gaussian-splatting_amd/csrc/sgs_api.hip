@@ -684,3 +684,127 @@ int sgs_batch_tile_stats(const void* workspace, void* stream_ptr, uint64_t* n_in
 }
 
 }  // extern "C"
+
+// ---- scene loading (sgs_scene.hip) ----
+namespace {
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+constexpr uint64_t kMortonMax = 0x7fffffffull;  // the radix sort counts items in int
+
+struct MortonLayout {
+    size_t bounds, codes_in, codes_out, index_in, temp, temp_bytes, total;
+};
+
+MortonLayout morton_layout(uint64_t n)
+{
+    MortonLayout L;
+    L.bounds = 0;
+    L.codes_in = align256(L.bounds + 6 * sizeof(uint32_t));
+    L.codes_out = align256(L.codes_in + n * sizeof(uint64_t));  // used when the caller wants no codes
+    L.index_in = align256(L.codes_out + n * sizeof(uint64_t));
+    L.temp = align256(L.index_in + n * sizeof(uint32_t));
+    L.temp_bytes = sgs::morton_sort_temp_bytes(n);
+    L.total = align256(L.temp + L.temp_bytes);
+    return L;
+}
+
+bool valid_bases(uint32_t b) { return b == 1 || b == 4 || b == 9 || b == 16; }
+
+int check_scene_out(const sgs_scene_arrays* o)
+{
+    if (!o) return fail(SGS_ERROR_INVALID_ARGUMENT, "output arrays are NULL");
+    if (o->n_gaussians && (!o->means || !o->scales || !o->rotations || !o->opacities || !o->sh_coefficients))
+        return fail(SGS_ERROR_INVALID_ARGUMENT, "an output array pointer is NULL");
+    if (!valid_bases(o->total_sh_bases))
+        return fail(SGS_ERROR_INVALID_ARGUMENT, "total_sh_bases must be 1, 4, 9 or 16 (SH degree 0-3)");
+    return SGS_SUCCESS;
+}
+
+sgs::SceneOut scene_out(const sgs_scene_arrays* o)
+{
+    return {o->means, o->scales, o->rotations, o->opacities, o->sh_coefficients, o->n_gaussians, o->total_sh_bases};
+}
+
+}  // namespace
+
+extern "C" {
+
+int sgs_ply_decode(const void* records, uint64_t n_records, const sgs_ply_layout* layout, uint64_t first,
+                   const sgs_scene_arrays* out, void* stream)
+{
+    if (int rc = check_scene_out(out)) return rc;
+    if (!layout) return fail(SGS_ERROR_INVALID_ARGUMENT, "layout is NULL");
+    if (layout->total_sh_bases != out->total_sh_bases)
+        return fail(SGS_ERROR_INVALID_ARGUMENT, "layout and output disagree on total_sh_bases");
+    if (first > out->n_gaussians || n_records > out->n_gaussians - first)
+        return fail(SGS_ERROR_INVALID_ARGUMENT, "records first.." + std::to_string(first + n_records) +
+                                                    " exceed the " + std::to_string(out->n_gaussians) + " output rows");
+    if (n_records == 0) return SGS_SUCCESS;
+    if (!records) return fail(SGS_ERROR_INVALID_ARGUMENT, "records is NULL");
+    sgs::PlyDecodeTable t;
+    t.stride = layout->stride;
+    t.big_endian = layout->big_endian ? 1u : 0u;
+    const uint32_t fields = 11u + 3u * layout->total_sh_bases;
+    for (uint32_t f = 0; f < SGS_PLY_MAX_FIELDS; ++f) {
+        t.offset[f] = 0;
+        t.type[f] = SGS_SCALAR_FLOAT32;
+        if (f >= fields) continue;
+        const uint32_t type = layout->type[f];
+        if (type < SGS_SCALAR_INT8 || type > SGS_SCALAR_FLOAT64)
+            return fail(SGS_ERROR_INVALID_ARGUMENT, "field " + std::to_string(f) + ": unknown scalar type " +
+                                                        std::to_string(type));
+        static const uint32_t size[] = {0, 1, 1, 2, 2, 4, 4, 4, 8};
+        if ((uint64_t)layout->offset[f] + size[type] > layout->stride)
+            return fail(SGS_ERROR_INVALID_ARGUMENT, "field " + std::to_string(f) + " lies outside the record");
+        t.offset[f] = layout->offset[f];
+        t.type[f] = (uint8_t)type;
+    }
+    const hipError_t e =
+        sgs::launch_ply_decode(records, n_records, t, first, scene_out(out), static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? SGS_SUCCESS : hip_fail(e, "ply decode");
+}
+
+size_t sgs_morton_order_workspace_size(uint64_t n) { return n > kMortonMax ? 0 : morton_layout(n).total; }
+
+int sgs_morton_order(const float* means, uint64_t n, uint32_t* order, uint64_t* sorted_codes, void* workspace,
+                     size_t workspace_bytes, void* stream)
+{
+    if (n > kMortonMax) return fail(SGS_ERROR_INVALID_ARGUMENT, "at most 2^31 - 1 means");
+    if (n == 0) return SGS_SUCCESS;
+    if (!means || !order) return fail(SGS_ERROR_INVALID_ARGUMENT, "means and order must not be NULL");
+    const MortonLayout L = morton_layout(n);
+    if (L.temp_bytes == 0) return fail(SGS_ERROR_HIP, "radix sort size query failed");
+    if (!workspace || workspace_bytes < L.total)
+        return fail(SGS_ERROR_WORKSPACE_TOO_SMALL, "workspace needs " + std::to_string(L.total) + " bytes");
+    if ((uintptr_t)workspace & 255) return fail(SGS_ERROR_INVALID_ARGUMENT, "workspace must be 256-byte aligned");
+    char* w = static_cast<char*>(workspace);
+    uint64_t* codes_out = sorted_codes ? sorted_codes : reinterpret_cast<uint64_t*>(w + L.codes_out);
+    sgs::MortonBuffers b;
+    b.bounds = reinterpret_cast<uint32_t*>(w + L.bounds);
+    b.codes_in = reinterpret_cast<uint64_t*>(w + L.codes_in);
+    b.index_in = reinterpret_cast<uint32_t*>(w + L.index_in);
+    b.temp = w + L.temp;
+    b.temp_bytes = L.temp_bytes;
+    const hipError_t e = sgs::launch_morton_order(means, n, order, codes_out, b, static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? SGS_SUCCESS : hip_fail(e, "morton order");
+}
+
+int sgs_gather_gaussians(const sgs_gaussians* src, const uint32_t* order, const sgs_scene_arrays* dst, void* stream)
+{
+    if (int rc = check_scene_out(dst)) return rc;
+    if (!src) return fail(SGS_ERROR_INVALID_ARGUMENT, "src is NULL");
+    if (src->total_sh_bases != dst->total_sh_bases)
+        return fail(SGS_ERROR_INVALID_ARGUMENT, "src and dst disagree on total_sh_bases");
+    if (dst->n_gaussians > src->n_gaussians)
+        return fail(SGS_ERROR_INVALID_ARGUMENT, "dst has more rows than src");
+    if (dst->n_gaussians == 0) return SGS_SUCCESS;
+    if (!order || !src->means || !src->scales || !src->rotations || !src->opacities || !src->sh_coefficients)
+        return fail(SGS_ERROR_INVALID_ARGUMENT, "a source pointer or order is NULL");
+    const sgs::SceneArgs sc{src->means, src->scales, src->rotations, src->opacities, src->sh_coefficients,
+                            src->n_gaussians, src->total_sh_bases};
+    const hipError_t e = sgs::launch_gather_rows(sc, order, scene_out(dst), static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? SGS_SUCCESS : hip_fail(e, "gather");
+}
+
+}  // extern "C"

@@ -194,6 +194,35 @@ hipError_t launch_batch_tiles(const SceneArgs& sc, const ViewDesc* views, uint32
                               hipEvent_t raster_event, hipStream_t stream);
 hipError_t launch_fill_u64_if(uint64_t* p, size_t n, uint64_t value, const uint32_t* run_if_set, hipStream_t stream);
 
+// ---- launchers (sgs_scene.hip) ----
+struct SceneOut {
+    float* means;
+    float* scales;
+    float* rotations;
+    float* opacities;
+    float* sh;
+    uint64_t n;
+    uint32_t sh_bases;
+};
+struct PlyDecodeTable {
+    uint32_t stride, big_endian;
+    uint32_t offset[SGS_PLY_MAX_FIELDS];
+    uint8_t type[SGS_PLY_MAX_FIELDS];
+};
+struct MortonBuffers {
+    uint32_t* bounds;     // [6] ordered-key minima and maxima
+    uint64_t* codes_in;   // [n]
+    uint32_t* index_in;   // [n]
+    void* temp;           // radix-sort temporary storage
+    size_t temp_bytes;
+};
+hipError_t launch_ply_decode(const void* records, uint64_t n_records, const PlyDecodeTable& table, uint64_t first,
+                             const SceneOut& out, hipStream_t stream);
+hipError_t launch_gather_rows(const SceneArgs& src, const uint32_t* order, const SceneOut& dst, hipStream_t stream);
+size_t morton_sort_temp_bytes(uint64_t n);
+hipError_t launch_morton_order(const float* means, uint64_t n, uint32_t* order, uint64_t* sorted_codes,
+                               const MortonBuffers& ws, hipStream_t stream);
+
 // ---- launchers (sgs_hybrid.hip) ----
 size_t hybrid_st_scratch_bytes(uint32_t fb_width, uint32_t fb_height, const Tuning& t);
 size_t hybrid_gps_scratch_bytes(const Tuning& t);
