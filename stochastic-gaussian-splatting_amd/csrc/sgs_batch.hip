@@ -657,9 +657,12 @@ __global__ __launch_bounds__(kRangeThreads) void view_key_range_kernel(const Vie
 // ---- per-tile counting sort by depth ----
 // One workgroup per tile: the tile's instances are written bucket by bucket, nearest first.
 // Bucket 0 takes keys below the view's kmin; bucket b >= 1 takes (key - kmin) >> shift = b - 1
-// (clamped to the last bucket). keys_out holds each instance's bucket floor (0, or
-// kmin + (b - 1) << shift), a non-decreasing lower bound of the true keys from that position on.
+// (clamped to the last bucket). keys_out[begin + 256 m] holds the floor of the bucket at
+// sorted position 256 m (0, or kmin + (b - 1) << shift): a lower bound of the true keys from
+// that position on, read where the rasteriser's batches of 256 start; other positions of
+// keys_out are not written.
 constexpr uint32_t kSortBins = 1024;
+constexpr uint32_t kSortRegKeys = 16;  // entries per lane kept in registers for tiles of <= 4096
 
 __device__ __forceinline__ uint32_t sort_bucket(uint32_t key, uint2 vk)
 {
@@ -699,16 +702,36 @@ __global__ __launch_bounds__(kBatchBlock) void tile_sort_kernel(uint32_t n_views
     const uint32_t n = end - begin, tid = threadIdx.x, lane = lane_id(), wave = tid / 64u;
     for (uint32_t i = tid; i < kSortBins; i += kBatchBlock) hist[i] = 0u;
     __syncthreads();
-    // four keys in flight per lane
-    uint32_t i = tid;
-    for (; i + 3u * kBatchBlock < n; i += 4u * kBatchBlock) {
-        uint32_t k[4];
+    // Tiles of up to kSortRegKeys * 256 instances (most of them) keep each lane's buckets and
+    // instances in registers between the histogram and the scatter: every entry is read once.
+    const bool in_regs = n <= kSortRegKeys * kBatchBlock;
+    uint32_t rb[kSortRegKeys], rg[kSortRegKeys];
+    if (in_regs) {
 #pragma unroll
-        for (uint32_t u = 0; u < 4u; u++) k[u] = keys_in[begin + i + u * kBatchBlock];
+        for (uint32_t u = 0; u < kSortRegKeys; u++) {
+            const uint32_t idx = tid + u * kBatchBlock;
+            rb[u] = idx < n ? keys_in[begin + idx] : 0u;
+            rg[u] = idx < n ? inst_in[begin + idx] : 0u;
+        }
 #pragma unroll
-        for (uint32_t u = 0; u < 4u; u++) atomicAdd(&hist[sort_bucket(k[u], vk)], 1u);
+        for (uint32_t u = 0; u < kSortRegKeys; u++) {
+            if (tid + u * kBatchBlock < n) {
+                rb[u] = sort_bucket(rb[u], vk);
+                atomicAdd(&hist[rb[u]], 1u);
+            }
+        }
+    } else {
+        // four keys in flight per lane
+        uint32_t i = tid;
+        for (; i + 3u * kBatchBlock < n; i += 4u * kBatchBlock) {
+            uint32_t k[4];
+#pragma unroll
+            for (uint32_t u = 0; u < 4u; u++) k[u] = keys_in[begin + i + u * kBatchBlock];
+#pragma unroll
+            for (uint32_t u = 0; u < 4u; u++) atomicAdd(&hist[sort_bucket(k[u], vk)], 1u);
+        }
+        for (; i < n; i += kBatchBlock) atomicAdd(&hist[sort_bucket(keys_in[begin + i], vk)], 1u);
     }
-    for (; i < n; i += kBatchBlock) atomicAdd(&hist[sort_bucket(keys_in[begin + i], vk)], 1u);
     __syncthreads();
     // exclusive scan of the buckets, 4 per thread
     uint32_t v[4], local = 0;
@@ -733,7 +756,26 @@ __global__ __launch_bounds__(kBatchBlock) void tile_sort_kernel(uint32_t n_views
         run += v[q];
     }
     __syncthreads();
-    i = tid;
+    // The rasteriser reads the depth floor only where its batches of 256 start: the floor of
+    // the bucket holding sorted position 256 m is the last bucket whose start is <= 256 m.
+    for (uint32_t m = tid; m * kBatchBlock < n; m += kBatchBlock) {
+        const uint32_t p = m * kBatchBlock;
+        uint32_t b = 0, e = kSortBins - 1u;
+        while (b < e) {
+            const uint32_t mid = (b + e + 1u) >> 1;
+            if (hist[mid] <= p) b = mid;
+            else e = mid - 1u;
+        }
+        keys_out[begin + p] = bucket_floor(b, vk);
+    }
+    __syncthreads();
+    if (in_regs) {
+#pragma unroll
+        for (uint32_t u = 0; u < kSortRegKeys; u++)
+            if (tid + u * kBatchBlock < n) inst_out[begin + atomicAdd(&hist[rb[u]], 1u)] = rg[u];
+        return;
+    }
+    uint32_t i = tid;
     for (; i + 3u * kBatchBlock < n; i += 4u * kBatchBlock) {
         uint32_t k[4], g[4];
 #pragma unroll
@@ -742,19 +784,10 @@ __global__ __launch_bounds__(kBatchBlock) void tile_sort_kernel(uint32_t n_views
             g[u] = inst_in[begin + i + u * kBatchBlock];
         }
 #pragma unroll
-        for (uint32_t u = 0; u < 4u; u++) {
-            const uint32_t b = sort_bucket(k[u], vk);
-            const uint32_t slot = begin + atomicAdd(&hist[b], 1u);
-            inst_out[slot] = g[u];
-            keys_out[slot] = bucket_floor(b, vk);
-        }
+        for (uint32_t u = 0; u < 4u; u++) inst_out[begin + atomicAdd(&hist[sort_bucket(k[u], vk)], 1u)] = g[u];
     }
-    for (; i < n; i += kBatchBlock) {
-        const uint32_t b = sort_bucket(keys_in[begin + i], vk);
-        const uint32_t slot = begin + atomicAdd(&hist[b], 1u);
-        inst_out[slot] = inst_in[begin + i];
-        keys_out[slot] = bucket_floor(b, vk);
-    }
+    for (; i < n; i += kBatchBlock)
+        inst_out[begin + atomicAdd(&hist[sort_bucket(keys_in[begin + i], vk)], 1u)] = inst_in[begin + i];
 }
 
 // inclusive scan of one value per thread over the 256-thread workgroup

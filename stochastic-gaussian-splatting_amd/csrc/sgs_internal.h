@@ -181,7 +181,7 @@ struct TileBuffers {
     uint32_t* instances;        // [capacity] Gaussian index of each (view, tile, Gaussian) instance
     uint32_t* keys;             // [capacity] its depth key (float bits of the view depth)
     uint32_t* sorted_instances; // [capacity] each tile's instances, nearest depth bucket first
-    uint32_t* sorted_keys;      // [capacity] their depth bucket floors (non-decreasing per tile)
+    uint32_t* sorted_keys;      // [capacity] depth bucket floor at each tile batch start (begin + 256 m)
     float4* centroids;          // [n_chunks] mean position of each chunk of 256 Gaussians
     float4* cov3d;              // [2 * n] view-independent projection terms of each Gaussian
     uint2* pair_rect;           // [n_views * n] tile rectangle of each (view, Gaussian) pair
