@@ -845,12 +845,16 @@ __device__ __forceinline__ void tile_fragment(uint32_t entry, unsigned long long
     live &= (1u << G) - 1u;
     if (!live) return;
     const uint32_t T = alpha_threshold(alpha);
-    const uint32_t base = (uint32_t)fj ^ (px * 9277u) ^ (py * 2699u);
-    while (live) {
-        const uint32_t s = (uint32_t)__builtin_ctz(live);
-        live &= live - 1u;
-        const uint32_t h = pcg_hash(base ^ (fs_group + s * 1973u));
-        if ((h >> 9u) < T) atomicMin(&layer[s][l], (unsigned long long)fj);
+    // px, py < 2^16, so the 24-bit multiplies give the full products of rand_uniform's seed
+    const uint32_t base = (uint32_t)fj ^ __umul24(px, 9277u) ^ __umul24(py, 2699u);
+    // samples unrolled: each sample's frame term is a wave-uniform constant, and a sample no
+    // lane of the wave holds live is skipped as a whole
+#pragma unroll
+    for (uint32_t s = 0; s < kLayers; s++) {
+        if (live & (1u << s)) {
+            const uint32_t h = pcg_hash(base ^ (fs_group + s * 1973u));
+            if ((h >> 9u) < T) atomicMin(&layer[s][l], (unsigned long long)fj);
+        }
     }
 }
 
