@@ -831,17 +831,21 @@ __device__ __forceinline__ void tile_fragment(uint32_t entry, unsigned long long
     const float alpha = fragment_alpha(ce, s_o[j], (float)px - s_msx[j], (float)py - s_msy[j]);
     if (alpha < kMinAlpha) return;
     const uint64_t fj = s_frag[j];
+    const uint32_t fd = (uint32_t)(fj >> 32);
     uint32_t live = 0;
+    // only the depth halves of the words are read: a sample is live if its word's depth is not
+    // in front of the fragment (equal depths are resolved by the atomicMin), and words only
+    // decrease, so a stale read never drops a winner
 #pragma unroll
     for (uint32_t s0 = 0; s0 < kLayers; s0 += 4u) {
-        uint64_t w[4];
+        uint32_t w[4];
 #pragma unroll
         for (uint32_t i = 0; i < 4u; i++)
-            w[i] = __hip_atomic_load(&layer[s0 + i][l], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            w[i] = __hip_atomic_load(reinterpret_cast<uint32_t*>(&layer[s0 + i][l]) + 1, __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_WORKGROUP);
 #pragma unroll
-        for (uint32_t i = 0; i < 4u; i++) live |= fj < w[i] ? 1u << (s0 + i) : 0u;
+        for (uint32_t i = 0; i < 4u; i++) live |= fd <= w[i] ? 1u << (s0 + i) : 0u;
     }
-    // words only decrease, so a stale read never drops a winner
     live &= (1u << G) - 1u;
     if (!live) return;
     const uint32_t T = alpha_threshold(alpha);
@@ -931,19 +935,31 @@ __global__ __launch_bounds__(kBatchBlock, kLayers <= 8u ? 5 : 3) void batch_tile
                 Projected p;
                 if (g < sc.n && project_cached(sc.means, cov3d, g, vd.r1, vd.r2, vd.r3, vd.cam, p)) {
                     const Aabb b = pixel_aabb(p, vd.cam);
-                    const uint32_t cx0 = max(b.x0, ox), cx1 = min(b.x1, ox + kTile);
-                    const uint32_t cy0 = max(b.y0, oy), cy1 = min(b.y1, oy + kTile);
+                    uint32_t cx0 = max(b.x0, ox), cx1 = min(b.x1, ox + kTile);
+                    uint32_t cy0 = max(b.y0, oy), cy1 = min(b.y1, oy + kTile);
                     const uint64_t frag = fragment_key(p.depth, g);
                     if (cx1 > cx0 && cy1 > cy0) {
-                        const uint32_t rw = cx1 - cx0, rh = cy1 - cy0;
-                        rect = (cx0 - ox) | ((cy0 - oy) << 4) | ((rw - 1u) << 8) |
-                               (((65536u + rw - 1u) / rw) << 12);
-                        // an instance that no word of its sub-tiles exceeds cannot win
-                        uint64_t m = 0;
+                        // an instance can only win in sub-tiles holding a word behind it: the
+                        // walk covers the bounding box of those sub-tiles within its rectangle
+                        uint32_t lx0 = kTile, lx1 = 0, ly0 = kTile, ly1 = 0;
                         for (uint32_t sy = (cy0 - oy) / kSub; sy <= (cy1 - 1u - oy) / kSub; sy++)
                             for (uint32_t sx = (cx0 - ox) / kSub; sx <= (cx1 - 1u - ox) / kSub; sx++)
-                                m = max(m, s_sub[sy * (kTile / kSub) + sx]);
-                        if (frag < m) cnt = rw * rh;
+                                if (frag < s_sub[sy * (kTile / kSub) + sx]) {
+                                    lx0 = min(lx0, sx * kSub);
+                                    lx1 = max(lx1, sx * kSub + kSub);
+                                    ly0 = min(ly0, sy * kSub);
+                                    ly1 = max(ly1, sy * kSub + kSub);
+                                }
+                        if (lx0 < lx1) {
+                            cx0 = max(cx0, ox + lx0);
+                            cx1 = min(cx1, ox + lx1);
+                            cy0 = max(cy0, oy + ly0);
+                            cy1 = min(cy1, oy + ly1);
+                            const uint32_t rw = cx1 - cx0, rh = cy1 - cy0;
+                            rect = (cx0 - ox) | ((cy0 - oy) << 4) | ((rw - 1u) << 8) |
+                                   (((65536u + rw - 1u) / rw) << 12);
+                            cnt = rw * rh;
+                        }
                     }
                     const ConicExp2 ce = conic_exp2(p.conic_x, p.conic_y, p.conic_z);
                     s_frag[tid] = frag;
