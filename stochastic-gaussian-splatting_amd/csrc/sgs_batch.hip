@@ -403,7 +403,7 @@ __global__ __launch_bounds__(kBatchBlock) void cov3d_kernel(SceneArgs sc, float4
 {
     const uint32_t g = blockIdx.x * kBatchBlock + threadIdx.x;
     if (g >= sc.n) return;
-    Cov3d c = { 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f };
+    Cov3d c = { 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f };
     const bool valid = gaussian_cov3d(sc.scales, sc.rotations, sc.opacities, g, c);
     store_cov3d(cache, g, valid, c);
 }
@@ -447,13 +447,13 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
     for (uint32_t chunk = blockIdx.x; chunk < n_chunks; chunk += gridDim.x) {
         const uint32_t g = chunk * kBatchBlock + tid;
         float mx = 0.0f, my = 0.0f, mz = 0.0f;
-        Cov3d cv = { 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, -1.0f };
+        Cov3d cv = { 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, -1.0f, 0.0f };
         if (!kFill && g < sc.n) {
             mx = sc.means[3u * g + 0u];
             my = sc.means[3u * g + 1u];
             mz = sc.means[3u * g + 2u];
             const float4 a = cov3d[2u * g], b = cov3d[2u * g + 1u];
-            cv = { a.x, a.y, a.z, a.w, b.x, b.y, b.z };
+            cv = { a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w };
         }
         const bool valid = g < sc.n && !(cv.opacity < 0.0f);
         for (uint32_t v = 0; v < n_views; v++) {

@@ -313,15 +313,18 @@ __device__ __forceinline__ float view_depth(const float* __restrict__ means, uin
 
 // View-independent part of the projection (gs_utils.h:38-80): sigmoid opacity and the 3D
 // covariance R S S^T R^T of the normalised quaternion. False if the Gaussian is culled in
-// every view (opacity below 1/255 or a degenerate quaternion).
+// every view (opacity below 1/255 or a degenerate quaternion). pt: log(255 opacity), the
+// power threshold of views without proper antialiasing, when known (NaN: computed per view).
 struct Cov3d {
     float c11, c12, c13, c22, c23, c33, opacity;
+    float pt;
 };
 
 __device__ __forceinline__ bool gaussian_cov3d(const float* __restrict__ scales, const float* __restrict__ rotations,
                                                const float* __restrict__ opacities, uint32_t g, Cov3d& c)
 {
     c.opacity = 1.0f / (1.0f + exp_f(-opacities[g]));
+    c.pt = __builtin_nanf("");
     if (c.opacity < kMinAlpha) return false;
 
     const float vx = exp_f(2.0f * scales[3u * g + 0u]);
@@ -393,7 +396,8 @@ __device__ __forceinline__ bool project_cov3d(float mx, float my, float mz, floa
     }
     o.mean_x = x * cam.focal_x + cam.center_x;
     o.mean_y = y * cam.focal_y + cam.center_y;
-    const float pt = log_f(opacity * 255.0f);
+    // without antialiasing the opacity, hence the threshold, is the same in every view
+    const float pt = !cam.proper_antialiasing && cv.pt == cv.pt ? cv.pt : log_f(opacity * 255.0f);
     const float cutoff = 2.0f * pt;
     const float ex = sqrtf(cxx * cutoff);
     const float ey = sqrtf(cyy * cutoff);
@@ -429,11 +433,11 @@ __device__ __forceinline__ bool project_gaussian(const float* __restrict__ means
 }
 
 // Cached view-independent part: two float4 per Gaussian, (c11, c12, c13, c22) and
-// (c23, c33, opacity, 0); opacity < 0 marks a Gaussian culled in every view.
+// (c23, c33, opacity, log(255 opacity)); opacity < 0 marks a Gaussian culled in every view.
 __device__ __forceinline__ void store_cov3d(float4* __restrict__ cache, uint32_t g, bool valid, const Cov3d& c)
 {
     cache[2u * g] = make_float4(c.c11, c.c12, c.c13, c.c22);
-    cache[2u * g + 1u] = make_float4(c.c23, c.c33, valid ? c.opacity : -1.0f, 0.0f);
+    cache[2u * g + 1u] = make_float4(c.c23, c.c33, valid ? c.opacity : -1.0f, valid ? log_f(c.opacity * 255.0f) : 0.0f);
 }
 
 // project_gaussian with the view-independent part read from the cache (bit-identical)
@@ -446,7 +450,7 @@ __device__ __forceinline__ bool project_cached(const float* __restrict__ means, 
     if (!(depth >= cam.near_plane && depth <= cam.far_plane)) return false;
     const float4 a = cache[2u * g], b = cache[2u * g + 1u];
     if (b.z < 0.0f) return false;
-    const Cov3d cv = { a.x, a.y, a.z, a.w, b.x, b.y, b.z };
+    const Cov3d cv = { a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w };
     return project_cov3d(mx, my, mz, depth, cv, r1, r2, r3, cam, o);
 }
 
