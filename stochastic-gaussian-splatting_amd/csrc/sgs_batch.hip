@@ -387,15 +387,33 @@ __device__ __forceinline__ void bin_direct(uint32_t gt, uint32_t g, uint32_t key
     }
 }
 
+// Wave reductions through DPP lane shifts (VALU) rather than ds_bpermute, which would share
+// the LDS issue queue with the binning histograms: row_shr 1, 2, 4, 8 reduce each row of 16
+// lanes into its lane 15 (lanes without a source keep the identity), row_bcast 15 and 31 carry
+// rows 0-2 into lane 63, whose value is returned to every lane.
+template <typename Op>
+__device__ __forceinline__ uint32_t wave_reduce_dpp(uint32_t x, uint32_t identity, Op op)
+{
+    const int id = (int)identity;
+    x = op(x, (uint32_t)__builtin_amdgcn_update_dpp(id, (int)x, 0x111, 0xf, 0xf, false));  // row_shr:1
+    x = op(x, (uint32_t)__builtin_amdgcn_update_dpp(id, (int)x, 0x112, 0xf, 0xf, false));  // row_shr:2
+    x = op(x, (uint32_t)__builtin_amdgcn_update_dpp(id, (int)x, 0x114, 0xf, 0xf, false));  // row_shr:4
+    x = op(x, (uint32_t)__builtin_amdgcn_update_dpp(id, (int)x, 0x118, 0xf, 0xf, false));  // row_shr:8
+    x = op(x, (uint32_t)__builtin_amdgcn_update_dpp(id, (int)x, 0x142, 0xa, 0xf, false));  // row_bcast:15
+    x = op(x, (uint32_t)__builtin_amdgcn_update_dpp(id, (int)x, 0x143, 0xc, 0xf, false));  // row_bcast:31
+    return (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
+}
 __device__ __forceinline__ uint32_t wave_min_u32(uint32_t x)
 {
-    for (int off = 32; off > 0; off >>= 1) x = min(x, (uint32_t)__shfl_xor((int)x, off, 64));
-    return x;
+    return wave_reduce_dpp(x, 0xffffffffu, [](uint32_t a, uint32_t b) { return min(a, b); });
 }
 __device__ __forceinline__ uint32_t wave_max_u32(uint32_t x)
 {
-    for (int off = 32; off > 0; off >>= 1) x = max(x, (uint32_t)__shfl_xor((int)x, off, 64));
-    return x;
+    return wave_reduce_dpp(x, 0u, [](uint32_t a, uint32_t b) { return max(a, b); });
+}
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t x)
+{
+    return wave_reduce_dpp(x, 0u, [](uint32_t a, uint32_t b) { return a + b; });
 }
 
 // view-independent part of every Gaussian's projection, once per render
@@ -501,8 +519,8 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
                 }
             }
             if constexpr (!kFill) {
-                uint32_t sum = n_t;  // a Gaussian touches < 2^24 tiles, so a wave's sum fits 32 bits
-                for (int off = 32; off > 0; off >>= 1) sum += (uint32_t)__shfl_xor((int)sum, off, 64);
+                // a Gaussian touches < 2^24 tiles, so a wave's sum fits 32 bits
+                const uint32_t sum = wave_sum_u32(n_t);
                 if (lane == 0u && sum) atomicAdd(&s_total, (unsigned long long)sum);
                 const uint32_t vis = (uint32_t)__popcll(__ballot(has));
                 if (lane == 0u && vis) atomicAdd(&s_visible, (unsigned long long)vis);
