@@ -474,6 +474,14 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
             cv = { a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w };
         }
         const bool valid = g < sc.n && !(cv.opacity < 0.0f);
+        // the fill pass reads each view's pair record one view ahead, so the load of view v + 1
+        // is in flight while view v is binned
+        uint2 next_rect = make_uint2(kNoTiles, kNoTiles);
+        uint32_t next_key = 0u;
+        if (kFill && g < sc.n) {
+            next_rect = pair_rect[g];
+            next_key = pair_key[g];
+        }
         for (uint32_t v = 0; v < n_views; v++) {
             if (tid == 0) {
                 s_wx0 = 0xffffffffu;
@@ -488,12 +496,14 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
             bool has = false;
             const size_t pair = (size_t)v * sc.n + g;
             if constexpr (kFill) {
-                if (g < sc.n) {
-                    const uint2 r = pair_rect[pair];
-                    has = r.x != kNoTiles;
-                    tr = { r.x & 0xffffu, r.y & 0xffffu, r.x >> 16, r.y >> 16 };
-                    key = pair_key[pair];
+                const uint2 r = next_rect;
+                key = next_key;
+                if (g < sc.n && v + 1u < n_views) {
+                    next_rect = pair_rect[pair + sc.n];
+                    next_key = pair_key[pair + sc.n];
                 }
+                has = r.x != kNoTiles;
+                tr = { r.x & 0xffffu, r.y & 0xffffu, r.x >> 16, r.y >> 16 };
             } else {
                 has = gaussian_tiles_reg(mx, my, mz, cv, valid, vd, tr, key);
                 if (g < sc.n) {
