@@ -330,10 +330,11 @@ def bench_serving(args, scene, dev):
     n = max(1, int(rate * args.serve_seconds))
     times = np.cumsum(rng.exponential(1.0 / rate, n))
     load = serving_requests(specs, paths, times, seed=13)
-    b0 = server.batches
+    b0, ov0 = server.batches, server.overflowed_batches
     t0 = time.perf_counter()
     done = server.run(load)
     run_s = time.perf_counter() - t0
+    overflowed = server.overflowed_batches - ov0
     stats = latency_stats(done)
     samples = sum(specs[r.stream].width * specs[r.stream].height * r.spp for r in done)
     batches = server.batches - b0
@@ -382,6 +383,7 @@ def bench_serving(args, scene, dev):
             "requests_per_batch": round(len(done) / max(batches, 1), 2),
             "p50_ms": stats["p50_ms"], "p99_ms": stats["p99_ms"],
             "p99_over_p50": round(stats["p99_ms"] / max(stats["p50_ms"], 1e-9), 3),
+            "overflowed_batches": overflowed,
             "plain": {"req_per_s_capacity": round(1.0 / mean_service, 1),
                       "p50_ms_at_offered_load": round(float(np.percentile(lat, 50)), 3),
                       "p99_ms_at_offered_load": round(float(np.percentile(lat, 99)), 3),

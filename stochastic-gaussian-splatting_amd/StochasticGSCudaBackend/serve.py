@@ -165,14 +165,22 @@ class Server:
         if len(self.streams) > (1 << (32 - STREAM_SHIFT)):
             raise ValueError(f"at most {1 << (32 - STREAM_SHIFT)} streams")
         self.max_views = max_views
-        self.tuning = tuning.c() if tuning is not None else None
-        self._tp = C.byref(self.tuning) if self.tuning is not None else None
         self.keep_outputs = keep_outputs
         self.clock = clock
         if arena_bytes is None:
             free, _ = torch.cuda.mem_get_info(self.device)
             arena_bytes = int(0.5 * free) // inflight
         self.arena_bytes = int(arena_bytes) & ~255
+        # The tile schedule's instance list takes up to half of the arena (16 bytes per
+        # instance): the library's default of 4 per (Gaussian, view) is too small for batches
+        # of many high-resolution views, and an overflowing batch falls back to the much
+        # slower scatter schedule.
+        tuning = BatchTuning() if tuning is None else tuning
+        if tuning.max_tile_instances == 0:
+            tuning = tuning._replace(max_tile_instances=max(1 << 16, min(1 << 31, self.arena_bytes // 32)))
+        self.tuning = tuning.c()
+        self._tp = C.byref(self.tuning)
+        self.overflowed_batches = 0
         table_bytes = _native._lib.sgs_batch_view_table_bytes(max_views)
         self.slots: list[_Slot] = []
         for _ in range(inflight):
@@ -274,6 +282,11 @@ class Server:
 
     def _complete(self, slot: _Slot) -> None:
         now = self.clock()
+        n_inst, visible, overflowed = C.c_uint64(), C.c_uint64(), C.c_uint32()
+        rc = _native._lib.sgs_batch_tile_stats(slot.workspace.data_ptr(), slot.stream.cuda_stream, C.byref(n_inst),
+                                               C.byref(visible), C.byref(overflowed))
+        _native._check(rc, "sgs_batch_tile_stats")
+        self.overflowed_batches += int(overflowed.value != 0)
         for req in slot.requests:
             st = self.streams[req.stream]
             st.busy = False
