@@ -94,6 +94,14 @@ typedef struct sgs_tuning {
     uint32_t max_gps_points;          /* GPS points per Gaussian; 0 = rn(W*H/(2 ss^2)), the
                                          reference's cap (accumulate_scatter.cu:128). This one
                                          does change results when it binds. */
+    uint32_t schedule;                /* stochastic transparency without hybrid scheduling: 0 or
+                                         SGS_BATCH_TILES (1) bins the view into 16x16 tiles and
+                                         rasterises each in LDS, the batched renderer's tile
+                                         schedule for one view and one sample; SGS_BATCH_SCATTER
+                                         (2) scatters with global atomicMin as the reference
+                                         does (accumulate_scatter.cu:19-77). Calls with blue
+                                         noise or debug counters always scatter. Identical
+                                         framebuffers either way. */
 } sgs_tuning;
 
 /* Replaces _C.k_buffer_size (render.cu:164). */
@@ -178,9 +186,10 @@ typedef struct sgs_batch_tuning {
     uint32_t blocks_per_cu;           /* persistent workgroups per CU (0: occupancy limit) */
     uint32_t schedule;                /* SGS_BATCH_TILES (default) or SGS_BATCH_SCATTER */
     uint32_t max_tile_instances;      /* capacity of the tile schedule's (view, tile, Gaussian)
-                                         list, 16 bytes of workspace each (default 4 per
-                                         Gaussian and view, within [2^16, 2^31]); a batch that
-                                         needs more runs the scatter schedule */
+                                         list, 16 bytes of workspace each (default per Gaussian
+                                         and view 4 + 8 P / 2^21 for a framebuffer of P pixels,
+                                         within [2^16, 2^31]); a batch that needs more runs the
+                                         scatter schedule */
     uint32_t tile_samples;            /* samples held in LDS per pass over a tile (8 or 16;
                                          default 8) */
 } sgs_batch_tuning;

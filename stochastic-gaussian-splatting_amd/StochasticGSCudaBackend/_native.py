@@ -50,7 +50,8 @@ class sgs_history(C.Structure):
 class sgs_tuning(C.Structure):
     _fields_ = [("sequential_threshold", C.c_uint32), ("binning_pixel_threshold", C.c_uint32),
                 ("binning_point_threshold", C.c_uint32), ("max_binned_gaussians", C.c_uint32),
-                ("tile_size", C.c_uint32), ("max_tile_instances", C.c_uint32), ("max_gps_points", C.c_uint32)]
+                ("tile_size", C.c_uint32), ("max_tile_instances", C.c_uint32), ("max_gps_points", C.c_uint32),
+                ("schedule", C.c_uint32)]
 
 
 class sgs_view(C.Structure):
@@ -128,7 +129,9 @@ k_buffer_size: int = _lib.sgs_k_buffer_size()
 
 
 class Tuning(NamedTuple):
-    """Scheduling knobs (sgs_tuning); 0 keeps the library default. Only max_gps_points changes results."""
+    """Scheduling knobs (sgs_tuning); 0 keeps the library default. Only max_gps_points changes
+    results. schedule: 0 or 1 = the tile schedule for stochastic transparency without hybrid
+    scheduling (blue noise and debug counters always scatter), 2 = the reference's scatter."""
     sequential_threshold: int = 0
     binning_pixel_threshold: int = 0
     binning_point_threshold: int = 0
@@ -136,6 +139,7 @@ class Tuning(NamedTuple):
     tile_size: int = 0
     max_tile_instances: int = 0
     max_gps_points: int = 0
+    schedule: int = 0
 
     def c(self) -> sgs_tuning:
         return sgs_tuning(*[int(v) for v in self])

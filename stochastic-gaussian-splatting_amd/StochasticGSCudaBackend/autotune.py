@@ -6,9 +6,10 @@ search: a coordinate descent over the candidate values of each knob, timing the 
 knobs act on with HIP events on the current stream, a few repetitions per trial, keeping the
 median. The max_gps_points cap does change results and is never tuned.
 
-* ``tune_render`` tunes the single-view path for one camera: tile size, binning pixel and
-  point thresholds, max_binned_gaussians and the sequential threshold (Tuning), timing the
-  accumulation stage (sgs_accumulate) of the settings' own schedule.
+* ``tune_render`` tunes the single-view path for one camera: the schedule (tiles or the
+  reference's scatter) for stochastic transparency without hybrid scheduling, tile size,
+  binning pixel and point thresholds, max_binned_gaussians and the sequential threshold
+  (Tuning), timing the accumulation stage (sgs_accumulate).
 * ``tune_batch`` tunes a batched render: schedule and samples per LDS pass of the tile
   schedule (BatchTuning), timing whole renders; the instance capacity is set from the
   measured instance count with a margin, which shrinks the workspace.
@@ -84,6 +85,7 @@ RENDER_SPACE = {
     "binning_point_threshold": [128, 256, 512, 1024, 2048],
     "max_binned_gaussians": [1 << 14, 1 << 16, 1 << 18, 1 << 20],
     "sequential_threshold": [2, 4, 8, 16],
+    "schedule": [1, 2],
 }
 
 
@@ -95,6 +97,8 @@ def tune_render(means, scales, rotations, opacities, sh_coefficients, settings: 
         space = dict(RENDER_SPACE)
         if not settings.hybrid_scheduling:
             space = {"sequential_threshold": space["sequential_threshold"]}
+            if not settings.use_gps:
+                space["schedule"] = list(RENDER_SPACE["schedule"])
         elif settings.use_gps:
             space.pop("tile_size")
             space.pop("binning_pixel_threshold")

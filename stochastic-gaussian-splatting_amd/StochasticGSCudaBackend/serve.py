@@ -172,9 +172,8 @@ class Server:
             arena_bytes = int(0.5 * free) // inflight
         self.arena_bytes = int(arena_bytes) & ~255
         # The tile schedule's instance list takes up to half of the arena (16 bytes per
-        # instance): the library's default of 4 per (Gaussian, view) is too small for batches
-        # of many high-resolution views, and an overflowing batch falls back to the much
-        # slower scatter schedule.
+        # instance) rather than the library's per-view estimate: an overflowing batch falls
+        # back to the much slower scatter schedule, which would show as a latency spike.
         tuning = BatchTuning() if tuning is None else tuning
         if tuning.max_tile_instances == 0:
             tuning = tuning._replace(max_tile_instances=max(1 << 16, min(1 << 31, self.arena_bytes // 32)))

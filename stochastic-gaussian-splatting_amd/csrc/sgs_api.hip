@@ -76,6 +76,7 @@ static Tuning resolve_tuning(const sgs_tuning* t, uint32_t n_gaussians)
     r.tile_size = (t && t->tile_size) ? t->tile_size : 32u;
     r.max_tile_instances = (t && t->max_tile_instances) ? t->max_tile_instances : (1u << 24);
     r.max_gps_points = t ? t->max_gps_points : 0u;
+    r.tiles = !(t && t->schedule == SGS_BATCH_SCATTER);
     return r;
 }
 
@@ -85,8 +86,67 @@ using namespace sgs;
 
 namespace {
 
+// ---- single-view tile schedule (the batched renderer's tile pipeline for one view) ----
+struct SingleTileLayout {
+    size_t counters, ready, tile_count, tile_cursor, cleared_end, offsets, instances, keys, sorted_instances,
+        sorted_keys, centroids, view_keys, cov3d, pair_rect, pair_key, table, queue, total;
+    size_t views_bytes;
+    uint32_t n_tiles, capacity, queue_capacity;
+};
+
+constexpr uint32_t kSingleTile = 16;
+constexpr uint32_t kSingleQueue = 1u << 16;  // work queue of the scatter fallback
+
+// instance capacity: about 4 + 8 P / 2^21 per Gaussian for a framebuffer of P pixels (a
+// Gaussian of a given size covers more 16x16 tiles the finer the framebuffer); a frame that
+// needs more falls back to the scatter schedule
+uint32_t single_tile_capacity(uint64_t P, uint32_t n_gaussians)
+{
+    const uint64_t want = (uint64_t)n_gaussians * (4ull + ((8ull * P) >> 21));
+    return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(want, 1ull << 16), 1ull << 31);
+}
+
+SingleTileLayout single_tile_layout(const FbCamera& cam, uint32_t n_gaussians)
+{
+    SingleTileLayout L;
+    const uint64_t P = (uint64_t)cam.width_i * cam.height_i;
+    L.n_tiles = ((cam.width_i + kSingleTile - 1u) / kSingleTile) * ((cam.height_i + kSingleTile - 1u) / kSingleTile);
+    L.capacity = single_tile_capacity(P, n_gaussians);
+    L.queue_capacity = kSingleQueue;
+    L.views_bytes = (sizeof(ViewDesc) + 255u) & ~size_t(255u);
+    Arena a;
+    L.counters = a.take(sizeof(BatchCounters));
+    L.ready = a.offset;
+    a.offset += 4u * (size_t)L.queue_capacity;
+    L.tile_count = a.offset;
+    a.offset += 4u * (size_t)L.n_tiles;
+    L.tile_cursor = a.offset;
+    a.offset += 4u * (size_t)L.n_tiles;
+    L.cleared_end = a.offset;
+    a.offset = (a.offset + 255u) & ~size_t(255u);
+    L.offsets = a.take(4u * ((size_t)L.n_tiles + 1u));
+    L.instances = a.take(4u * (size_t)L.capacity);
+    L.keys = a.take(4u * (size_t)L.capacity);
+    L.sorted_instances = a.take(4u * (size_t)L.capacity);
+    L.sorted_keys = a.take(4u * (size_t)L.capacity);
+    L.centroids = a.take(16u * (((size_t)n_gaussians + 255u) / 256u));
+    L.view_keys = a.take(8u);
+    L.cov3d = a.take(32u * (size_t)n_gaussians);
+    L.pair_rect = a.take(8u * (size_t)n_gaussians);
+    L.pair_key = a.take(4u * (size_t)n_gaussians);
+    L.table = a.take(L.views_bytes + 16u);
+    L.queue = a.take(sizeof(BigEntry) * (size_t)L.queue_capacity);
+    L.total = a.offset;
+    return L;
+}
+
+bool single_tiles_allowed(const sgs_settings* s, const Tuning& t)
+{
+    return t.tiles && !s->use_gps && !s->hybrid_scheduling;
+}
+
 struct RenderLayout {
-    size_t fb, samples, reuse, hybrid, total;
+    size_t fb, samples, reuse, hybrid, tiles, total;
 };
 
 size_t hybrid_bytes(const sgs_settings* s, const FbCamera& cam, const Tuning& t)
@@ -95,7 +155,7 @@ size_t hybrid_bytes(const sgs_settings* s, const FbCamera& cam, const Tuning& t)
     return s->use_gps ? hybrid_gps_scratch_bytes(t) : hybrid_st_scratch_bytes(cam.width_i, cam.height_i, t);
 }
 
-RenderLayout render_layout(const sgs_settings* s, const FbCamera& cam, const Tuning& t)
+RenderLayout render_layout(const sgs_settings* s, const FbCamera& cam, const Tuning& t, uint32_t n_gaussians)
 {
     const size_t P = (size_t)cam.width_i * cam.height_i;
     Arena a;
@@ -105,6 +165,7 @@ RenderLayout render_layout(const sgs_settings* s, const FbCamera& cam, const Tun
     L.reuse = s->spatial_reuse ? a.take(16u * P) : 0;
     const size_t hb = hybrid_bytes(s, cam, t);
     L.hybrid = hb ? a.take(hb) : 0;
+    L.tiles = single_tiles_allowed(s, t) ? a.take(single_tile_layout(cam, n_gaussians).total) : 0;
     L.total = a.offset;
     return L;
 }
@@ -141,19 +202,29 @@ int check_tuning(const sgs_tuning* t)
 {
     if (t && t->tile_size && t->tile_size != 16u && t->tile_size != 32u && t->tile_size != 64u)
         return fail(SGS_ERROR_INVALID_ARGUMENT, "tuning.tile_size must be 16, 32 or 64");
+    if (t && t->schedule > SGS_BATCH_SCATTER)
+        return fail(SGS_ERROR_INVALID_ARGUMENT, "tuning.schedule must be 0, 1 (tiles) or 2 (scatter)");
     return SGS_SUCCESS;
 }
+
+// the tile schedule for one view and one sample into fb (defined with the batched renderer)
+int accumulate_tiles_single(const sgs_gaussians* g, const sgs_settings* s, const FbCamera& cam, uint64_t* fb,
+                            uint32_t frame_idx, char* tile_ws, hipStream_t stream);
 
 SceneArgs scene_args(const sgs_gaussians* g)
 {
     return { g->means, g->scales, g->rotations, g->opacities, g->sh_coefficients, g->n_gaussians, g->total_sh_bases };
 }
 
-// clear + accumulate into fb
+// clear + accumulate into fb (tile_ws: the single-view tile section of the workspace, or NULL)
 int accumulate_into(const sgs_gaussians* g, const sgs_settings* s, const FbCamera& cam, const uint8_t* blue_noise,
                     uint64_t blue_noise_numel, uint64_t* fb, uint32_t frame_idx, const Tuning& t, uint64_t* counters,
-                    void* hybrid_scratch, hipStream_t stream)
+                    void* hybrid_scratch, void* tile_ws, hipStream_t stream)
 {
+    // the tile schedule writes every framebuffer word (no clear); blue noise and the debug
+    // counters exist in the scatter kernels only
+    if (tile_ws && !(blue_noise && blue_noise_numel > 0) && !counters)
+        return accumulate_tiles_single(g, s, cam, fb, frame_idx, static_cast<char*>(tile_ws), stream);
     const size_t P = (size_t)cam.width_i * cam.height_i;
     hipError_t e = launch_fill_u64(fb, P, kEmpty, stream);
     if (e != hipSuccess) return hip_fail(e, "framebuffer clear");
@@ -210,7 +281,7 @@ size_t sgs_render_workspace_size(const sgs_settings* settings, uint32_t n_gaussi
                                         settings->center_x, settings->center_y, settings->near_plane,
                                         settings->far_plane, settings->proper_antialiasing,
                                         settings->supersampling_factor);
-    return render_layout(settings, cam, resolve_tuning(tuning, n_gaussians)).total;
+    return render_layout(settings, cam, resolve_tuning(tuning, n_gaussians), n_gaussians).total;
 }
 
 int sgs_render(const sgs_gaussians* gaussians, const sgs_settings* s, const uint8_t* blue_noise,
@@ -242,7 +313,7 @@ int sgs_render(const sgs_gaussians* gaussians, const sgs_settings* s, const uint
     const Tuning t = resolve_tuning(tuning, gaussians->n_gaussians);
     const FbCamera cam = make_fb_camera(s->width, s->height, s->focal_x, s->focal_y, s->center_x, s->center_y,
                                         s->near_plane, s->far_plane, s->proper_antialiasing, s->supersampling_factor);
-    const RenderLayout L = render_layout(s, cam, t);
+    const RenderLayout L = render_layout(s, cam, t, gaussians->n_gaussians);
     if (!workspace || workspace_bytes < L.total)
         return fail(SGS_ERROR_WORKSPACE_TOO_SMALL,
                     "workspace of " + std::to_string(L.total) + " bytes needed, got " + std::to_string(workspace_bytes));
@@ -253,7 +324,7 @@ int sgs_render(const sgs_gaussians* gaussians, const sgs_settings* s, const uint
     char* ws = static_cast<char*>(workspace);
     uint64_t* fb = reinterpret_cast<uint64_t*>(ws + L.fb);
     if ((rc = accumulate_into(gaussians, s, cam, blue_noise, blue_noise_numel, fb, frame_idx, t, nullptr,
-                              L.hybrid ? ws + L.hybrid : nullptr, stream)))
+                              L.hybrid ? ws + L.hybrid : nullptr, L.tiles ? ws + L.tiles : nullptr, stream)))
         return rc;
 
     const SceneArgs sc = scene_args(gaussians);
@@ -284,7 +355,9 @@ size_t sgs_accumulate_workspace_size(const sgs_settings* settings, uint32_t n_ga
                                         settings->center_x, settings->center_y, settings->near_plane,
                                         settings->far_plane, settings->proper_antialiasing,
                                         settings->supersampling_factor);
-    const size_t hb = hybrid_bytes(settings, cam, resolve_tuning(tuning, n_gaussians));
+    const Tuning t = resolve_tuning(tuning, n_gaussians);
+    const size_t hb = hybrid_bytes(settings, cam, t);
+    if (single_tiles_allowed(settings, t)) return single_tile_layout(cam, n_gaussians).total;
     return hb ? ((hb + 255u) & ~size_t(255u)) : 0;
 }
 
@@ -302,14 +375,16 @@ int sgs_accumulate(const sgs_gaussians* gaussians, const sgs_settings* s, const 
     const Tuning t = resolve_tuning(tuning, gaussians->n_gaussians);
     const FbCamera cam = make_fb_camera(s->width, s->height, s->focal_x, s->focal_y, s->center_x, s->center_y,
                                         s->near_plane, s->far_plane, s->proper_antialiasing, s->supersampling_factor);
-    const size_t need = hybrid_bytes(s, cam, t);
+    const bool tiles = single_tiles_allowed(s, t);
+    const size_t need = tiles ? single_tile_layout(cam, gaussians->n_gaussians).total : hybrid_bytes(s, cam, t);
     if (need && (!workspace || workspace_bytes < need))
         return fail(SGS_ERROR_WORKSPACE_TOO_SMALL,
                     "workspace of " + std::to_string(need) + " bytes needed, got " + std::to_string(workspace_bytes));
     if (need && reinterpret_cast<uintptr_t>(workspace) % 256u != 0)
         return fail(SGS_ERROR_INVALID_ARGUMENT, "workspace must be 256-byte aligned");
     return accumulate_into(gaussians, s, cam, blue_noise, blue_noise_numel, framebuffer, frame_idx, t, counters,
-                           need ? workspace : nullptr, static_cast<hipStream_t>(stream_ptr));
+                           need && !tiles ? workspace : nullptr, tiles ? workspace : nullptr,
+                           static_cast<hipStream_t>(stream_ptr));
 }
 
 }  // extern "C"
@@ -334,11 +409,16 @@ BatchTuning resolve_batch_tuning(const sgs_batch_tuning* t)
     return r;
 }
 
-// default capacity of the tile schedule's instance list (16 bytes of workspace each): 4 per
-// (Gaussian, view) pair
-uint32_t default_tile_instances(uint32_t n_gaussians, uint32_t n_views)
+uint64_t view_fb_pixels(const sgs_view& v);
+
+// default capacity of the tile schedule's instance list (16 bytes of workspace each): per
+// view, 4 + 8 P / 2^21 per Gaussian for a framebuffer of P pixels (a Gaussian covers more
+// 16x16 tiles the finer the framebuffer), as for a single view
+uint32_t default_tile_instances(const sgs_view* views, uint32_t n_views, uint32_t n_gaussians)
 {
-    const uint64_t want = 4ull * n_gaussians * n_views;
+    uint64_t per = 0;
+    for (uint32_t v = 0; v < n_views; v++) per += 4ull + ((8ull * view_fb_pixels(views[v])) >> 21);
+    const uint64_t want = per >= (1ull << 32) ? (1ull << 32) : per * n_gaussians;
     return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(want, 1ull << 16), 1ull << 31);
 }
 
@@ -384,7 +464,7 @@ BatchLayout batch_layout(const sgs_view* views, uint32_t n, uint32_t n_gaussians
     }
     L.n_blocks = (uint32_t)blocks;
     const bool tiles = t.schedule == 0u;
-    L.capacity = !tiles ? 0u : (t.max_tile_instances ? t.max_tile_instances : default_tile_instances(n_gaussians, n));
+    L.capacity = !tiles ? 0u : (t.max_tile_instances ? t.max_tile_instances : default_tile_instances(views, n, n_gaussians));
     const size_t tiles_n = tiles ? (size_t)L.n_tiles : 0u;
     Arena a;
     L.counters = a.take(sizeof(BatchCounters));
@@ -531,6 +611,68 @@ void encode_views(const sgs_view* views, uint32_t n, void* host)
     }
     prefix[n] = block;
     tile_prefix[n] = tile;
+}
+
+int accumulate_tiles_single(const sgs_gaussians* g, const sgs_settings* s, const FbCamera& cam, uint64_t* fb,
+                            uint32_t frame_idx, char* tile_ws, hipStream_t stream)
+{
+    const SingleTileLayout L = single_tile_layout(cam, g->n_gaussians);
+    // view table: everything but the camera pose from the host, the pose from the device
+    // pointers (no host synchronisation)
+    sgs_view v;
+    std::memset(&v, 0, sizeof v);
+    v.width = s->width;
+    v.height = s->height;
+    v.focal_x = s->focal_x;
+    v.focal_y = s->focal_y;
+    v.center_x = s->center_x;
+    v.center_y = s->center_y;
+    v.near_plane = s->near_plane;
+    v.far_plane = s->far_plane;
+    v.proper_antialiasing = s->proper_antialiasing;
+    v.supersampling_factor = s->supersampling_factor;
+    v.spp = 1;
+    v.frame_idx = frame_idx;
+    v.temporal_mode = SGS_TEMPORAL_RESET;
+    std::vector<char> host(table_bytes(1));
+    encode_views(&v, 1, host.data());
+    const ViewDesc& d = *reinterpret_cast<const ViewDesc*>(host.data());
+    const uint32_t* prefix = reinterpret_cast<const uint32_t*>(host.data() + table_views_bytes(1));
+    ViewDesc* desc = reinterpret_cast<ViewDesc*>(tile_ws + L.table);
+    hipError_t e = launch_single_view_table(d, L.views_bytes, prefix[1], prefix[3], s->w2c, s->cam_position, desc,
+                                            stream);
+    if (e != hipSuccess) return hip_fail(e, "view table");
+    BatchCounters* counters = reinterpret_cast<BatchCounters*>(tile_ws + L.counters);
+    e = launch_zero_u32(reinterpret_cast<uint32_t*>(counters), (L.cleared_end - L.counters) / 4u, stream);
+    if (e != hipSuccess) return hip_fail(e, "counter clear");
+    TileBuffers tb;
+    tb.tile_count = reinterpret_cast<uint32_t*>(tile_ws + L.tile_count);
+    tb.tile_cursor = reinterpret_cast<uint32_t*>(tile_ws + L.tile_cursor);
+    tb.offsets = reinterpret_cast<uint32_t*>(tile_ws + L.offsets);
+    tb.instances = reinterpret_cast<uint32_t*>(tile_ws + L.instances);
+    tb.keys = reinterpret_cast<uint32_t*>(tile_ws + L.keys);
+    tb.sorted_instances = reinterpret_cast<uint32_t*>(tile_ws + L.sorted_instances);
+    tb.sorted_keys = reinterpret_cast<uint32_t*>(tile_ws + L.sorted_keys);
+    tb.centroids = reinterpret_cast<float4*>(tile_ws + L.centroids);
+    tb.view_keys = reinterpret_cast<uint2*>(tile_ws + L.view_keys);
+    tb.cov3d = reinterpret_cast<float4*>(tile_ws + L.cov3d);
+    tb.pair_rect = reinterpret_cast<uint2*>(tile_ws + L.pair_rect);
+    tb.pair_key = reinterpret_cast<uint32_t*>(tile_ws + L.pair_key);
+    tb.n_tiles = L.n_tiles;
+    tb.capacity = L.capacity;
+    const SceneArgs sc = scene_args(g);
+    const uint32_t* tile_prefix = reinterpret_cast<const uint32_t*>(tile_ws + L.table + L.views_bytes) + 2;
+    e = launch_batch_tiles(sc, desc, 1, tile_prefix, tb, 8u, counters, fb, nullptr, stream);
+    if (e != hipSuccess) return hip_fail(e, "tile schedule");
+    // instance list overflow: the scatter schedule on a cleared framebuffer, gated on the flag
+    e = launch_fill_u64_if(fb, (size_t)cam.width_i * cam.height_i, kEmpty, &counters->overflow, stream);
+    if (e != hipSuccess) return hip_fail(e, "framebuffer clear");
+    BatchTuning bt = resolve_batch_tuning(nullptr);
+    bt.queue_capacity = L.queue_capacity;
+    e = launch_batch_accumulate(sc, desc, 1, fb, counters, reinterpret_cast<BigEntry*>(tile_ws + L.queue),
+                                reinterpret_cast<uint32_t*>(tile_ws + L.ready), bt, &counters->overflow, stream);
+    if (e != hipSuccess) return hip_fail(e, "scatter fallback");
+    return SGS_SUCCESS;
 }
 
 }  // namespace

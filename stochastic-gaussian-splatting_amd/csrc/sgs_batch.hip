@@ -1098,6 +1098,33 @@ __global__ __launch_bounds__(kBatchBlock, kLayers <= 8u ? 5 : 3) void batch_tile
     }
 }
 
+__global__ void single_view_table_kernel(ViewDesc d, uint32_t views_bytes, uint32_t n_blocks, uint32_t n_tiles,
+                                         const float* __restrict__ w2c, const float* __restrict__ cam_position,
+                                         char* __restrict__ table)
+{
+    if (threadIdx.x != 0) return;
+    d.r1 = make_float4(w2c[0], w2c[1], w2c[2], w2c[3]);
+    d.r2 = make_float4(w2c[4], w2c[5], w2c[6], w2c[7]);
+    d.r3 = make_float4(w2c[8], w2c[9], w2c[10], w2c[11]);
+    d.cam_x = cam_position[0];
+    d.cam_y = cam_position[1];
+    d.cam_z = cam_position[2];
+    *reinterpret_cast<ViewDesc*>(table) = d;
+    uint32_t* prefix = reinterpret_cast<uint32_t*>(table + views_bytes);
+    prefix[0] = 0u;
+    prefix[1] = n_blocks;
+    prefix[2] = 0u;
+    prefix[3] = n_tiles;
+}
+
+hipError_t launch_single_view_table(const ViewDesc& desc, size_t views_bytes, uint32_t n_blocks, uint32_t n_tiles,
+                                    const float* w2c, const float* cam_position, void* table, hipStream_t stream)
+{
+    hipLaunchKernelGGL(single_view_table_kernel, dim3(1), dim3(64), 0, stream, desc, (uint32_t)views_bytes, n_blocks,
+                       n_tiles, w2c, cam_position, static_cast<char*>(table));
+    return hipGetLastError();
+}
+
 hipError_t launch_batch_tiles(const SceneArgs& sc, const ViewDesc* views, uint32_t n_views, const uint32_t* tile_prefix,
                               const TileBuffers& tb, uint32_t layers, BatchCounters* counters, uint64_t* fb,
                               hipEvent_t raster_event, hipStream_t stream)

@@ -30,6 +30,7 @@ struct Tuning {
     uint32_t tile_size;
     uint32_t max_tile_instances;
     uint32_t max_gps_points;  // 0 = reference heuristic
+    bool tiles;               // single view: tile schedule when the settings allow it
 };
 
 // One binned large Gaussian of the hybrid ST path (64 bytes, 16-byte aligned).
@@ -193,6 +194,12 @@ hipError_t launch_batch_tiles(const SceneArgs& sc, const ViewDesc* views, uint32
                               const TileBuffers& tb, uint32_t layers, BatchCounters* counters, uint64_t* fb,
                               hipEvent_t raster_event, hipStream_t stream);
 hipError_t launch_fill_u64_if(uint64_t* p, size_t n, uint64_t value, const uint32_t* run_if_set, hipStream_t stream);
+// device view table of a single view whose world-to-view matrix and camera position are in
+// device memory: `desc` with its rows and position taken from w2c / cam_position, then the
+// resolve-block prefix {0, n_blocks} and the tile prefix {0, n_tiles} (the table layout of
+// the batched renderer for one view, whose ViewDesc part is views_bytes long)
+hipError_t launch_single_view_table(const ViewDesc& desc, size_t views_bytes, uint32_t n_blocks, uint32_t n_tiles,
+                                    const float* w2c, const float* cam_position, void* table, hipStream_t stream);
 
 // ---- launchers (sgs_scene.hip) ----
 struct SceneOut {

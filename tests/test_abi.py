@@ -88,7 +88,18 @@ def test_workspace_sizes(native):
     assert base >= 64 * 48 * (8 + 16)
     s2 = _settings(native, supersampling_factor=2, spatial_reuse=1)
     assert lib.sgs_render_workspace_size(C.byref(s2), 10, None) >= 4 * 64 * 48 * (8 + 16 + 16)
-    assert lib.sgs_accumulate_workspace_size(C.byref(s), 10, None) == 0
+    # the reference's scatter needs no workspace; the default tile schedule holds at least its
+    # instance list (2^16 entries of 16 bytes) and the per-Gaussian caches
+    scatter = native.sgs_tuning(schedule=2)
+    assert lib.sgs_accumulate_workspace_size(C.byref(s), 10, C.byref(scatter)) == 0
+    assert lib.sgs_accumulate_workspace_size(C.byref(s), 10, None) >= 16 * (1 << 16)
+    assert lib.sgs_render_workspace_size(C.byref(s), 10, C.byref(scatter)) < base
+    gps = _settings(native, use_gps=1)
+    assert lib.sgs_accumulate_workspace_size(C.byref(gps), 10, None) == 0
+    bad = native.sgs_tuning(schedule=3)
+    g = native.sgs_gaussians(0x1000, 0x2000, 0x3000, 0x4000, 0x5000, 10, 16)
+    assert lib.sgs_accumulate(C.byref(g), C.byref(s), None, 0, 0x6000, 0, C.byref(bad), None, None, 0, None) == -1
+    assert b"schedule" in lib.sgs_last_error()
 
 
 def test_python_api_mirrors_reference():

@@ -49,16 +49,47 @@ def print_case(**kw):
     print("case:", " ".join(f"{k}={v}" for k, v in kw.items()), flush=True)
 
 
-@pytest.mark.parametrize("ss,aa,seq", [(1, False, 8), (2, False, 8), (1, True, 8), (1, False, 1), (1, False, 64)])
-def test_st_framebuffer_bit_exact(gpu, oracle_mod, ss, aa, seq):
+SCATTER, TILES = 2, 1  # sgs_tuning.schedule
+
+
+@pytest.mark.parametrize("ss,aa,seq,schedule", [
+    (1, False, 8, SCATTER), (2, False, 8, SCATTER), (1, True, 8, SCATTER), (1, False, 1, SCATTER),
+    (1, False, 64, SCATTER),
+    (1, False, 0, TILES), (2, False, 0, TILES), (1, True, 0, TILES),
+])
+def test_st_framebuffer_bit_exact(gpu, oracle_mod, ss, aa, seq, schedule):
     arrays = small_scene(3000, seed=21, log_scale_mean=-3.6, log_scale_std=0.8)
     sc, t = oracle_scene(arrays), to_device(arrays)
     cam = small_camera(160, 120, supersampling_factor=ss, proper_antialiasing=aa)
     for frame in (0, 5, 1 << 20):
-        print_case(test="st", ss=ss, aa=aa, seq=seq, frame=frame)
+        print_case(test="st", ss=ss, aa=aa, seq=seq, schedule=schedule, frame=frame)
         ref, st = oracle_mod.accumulate(sc, cam, frame)
-        got = gpu_fb(gpu, t, cam, frame, tuning=gpu.Tuning(sequential_threshold=seq))
+        got = gpu_fb(gpu, t, cam, frame, tuning=gpu.Tuning(sequential_threshold=seq, schedule=schedule))
         assert got.shape == ref.shape
+        assert np.array_equal(got, ref), f"{np.count_nonzero(got != ref)} of {ref.size} pixels differ"
+
+
+def test_st_tiles_overflow_falls_back_to_scatter(gpu, oracle_mod):
+    # 400 large Gaussians over a 512x384 view bin into far more than the single-view
+    # instance capacity (2^16 entries for so few Gaussians): the gated scatter runs instead
+    import math
+    arrays = small_scene(400, seed=23, log_scale_mean=-1.2, log_scale_std=0.3)
+    sc, t = oracle_scene(arrays), to_device(arrays)
+    cam = small_camera(512, 384, distance=2.2)
+    # the (Gaussian, tile) instances the binning would need, from the oracle's AABBs
+    need = 0
+    for i in range(400):
+        p = oracle_mod.project(sc, i, cam)
+        if p.active:
+            x0, x1 = max(math.floor(p.bounds[0]), 0), min(math.ceil(p.bounds[1]), 512)
+            y0, y1 = max(math.floor(p.bounds[2]), 0), min(math.ceil(p.bounds[3]), 384)
+            if x1 > x0 and y1 > y0:
+                need += ((x1 - 1) // 16 - x0 // 16 + 1) * ((y1 - 1) // 16 - y0 // 16 + 1)
+    assert need > (1 << 16), need
+    for frame in (1, 9):
+        print_case(test="st_tiles_overflow", frame=frame)
+        ref, _ = oracle_mod.accumulate(sc, cam, frame)
+        got = gpu_fb(gpu, t, cam, frame, tuning=gpu.Tuning(schedule=TILES))
         assert np.array_equal(got, ref), f"{np.count_nonzero(got != ref)} of {ref.size} pixels differ"
 
 
