@@ -6,6 +6,11 @@ is disabled), STATIC accumulation while the camera stands still, REPROJECT (temp
 or RESET when it moves; ``SAMPLES_PER_PIXEL`` render calls per image, the first in that mode
 and the rest STATIC; the premultiplied history composited over the background.
 
+When the settings allow it (stochastic transparency without GPS, hybrid scheduling, spatial
+reuse or blue noise), the SAMPLES_PER_PIXEL samples of an image are rendered by one batched
+call (sgs_render_batch: the view binned once, its samples rasterised together), which gives
+the same history bit for bit as the per-sample loop.
+
 ``PerspectiveCamera`` and ``View`` carry the fields of the framework's camera and view
 objects that the renderer reads (Renderer.py:51-71,80-89).
 """
@@ -17,7 +22,7 @@ from dataclasses import dataclass, field
 import numpy as np
 import torch
 
-from .render import History, Settings, TemporalMode, create_k_buffer, render
+from .render import History, Settings, TemporalMode, blue_noise, create_k_buffer, render
 
 
 @dataclass
@@ -111,6 +116,31 @@ class StochasticGSRenderer:
         self.history = render(m.means, m.scales, m.rotations, m.opacities, m.sh_coefficients, settings, self.k_buffer,
                               self.history, view_to_previous_view, temporal_mode, frame_idx=frame)
 
+    def _batched(self, settings: Settings) -> bool:
+        return (not settings.use_gps and not settings.hybrid_scheduling and not settings.spatial_reuse
+                and blue_noise().numel() == 0)
+
+    def _render_samples(self, settings: Settings, view_to_previous_view: torch.Tensor, temporal_mode: TemporalMode,
+                        spp: int) -> None:
+        """The SPP loop of Renderer.py:85-87 (first sample in temporal_mode, the rest STATIC),
+        as one batched render when the settings allow it."""
+        if spp > 1 and self._batched(settings):
+            from .batch import BatchView, render_batch
+            frame = None
+            if self.frame_idx is not None:
+                frame, self.frame_idx = self.frame_idx, self.frame_idx + spp
+            reset = temporal_mode == TemporalMode.RESET
+            bv = BatchView(settings, spp=spp, frame_idx=frame, temporal_mode=temporal_mode,
+                           history=None if reset else self.history,
+                           view_to_previous_view=view_to_previous_view if temporal_mode == TemporalMode.REPROJECT
+                           else None)
+            m = self.model
+            self.history = render_batch(m.means, m.scales, m.rotations, m.opacities, m.sh_coefficients, [bv])[0]
+            return
+        for _ in range(spp):
+            self._render_once(settings, view_to_previous_view, temporal_mode)
+            temporal_mode = TemporalMode.STATIC
+
     def settings_for(self, view: View) -> Settings:
         cam = view.camera
         return Settings(view.w2c, view.position, cam.width, cam.height, cam.focal_x, cam.focal_y, cam.center_x,
@@ -139,9 +169,7 @@ class StochasticGSRenderer:
         settings = self.settings_for(view)
         temporal_mode = self.next_temporal_mode(view, settings, benchmark)
         view_to_previous_view = self.history_w2c @ view.c2w if temporal_mode == TemporalMode.REPROJECT else view.c2w
-        for _ in range(max(self.SAMPLES_PER_PIXEL, 1)):
-            self._render_once(settings, view_to_previous_view, temporal_mode)
-            temporal_mode = TemporalMode.STATIC
+        self._render_samples(settings, view_to_previous_view, temporal_mode, max(self.SAMPLES_PER_PIXEL, 1))
         self.history_c2w = view.c2w_numpy
         self.history_w2c = view.w2c
 
