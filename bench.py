@@ -180,6 +180,57 @@ def roofline_scatter(scene_n: int, views, ms, work):
     }
 
 
+def roofline_render_samples(scene_np, views, scene, step_ms, work, atomics_per_s):
+    """The per-render-sample roofline of a whole step (every view, every sample): bytes
+    B = 12 N + 32 N_z + 16 P + 204 U + 24 Q (1 + [not RESET]) per (view, sample), N_z the
+    Gaussians inside [near, far], P framebuffer words, U distinct winners, Q output pixels, and
+    A = accepted fragments, one 64-bit atomicMin each in a scatter; t_roof = max(B / 8 TB/s,
+    A / R_atomic) with R_atomic the scatter's measured atomic rate. U comes from one
+    tile-schedule framebuffer per view and A from the scatter's counters of one sample per view,
+    both taken for each of the view's samples. Outside the timed region."""
+    import numpy as np
+    import torch
+    from StochasticGSCudaBackend import _native
+    means = scene_np["means"].astype(np.float64)
+    n = means.shape[0]
+    dev = scene["means"].device
+    bytes_total, atomics = 0.0, 0.0
+    for v, cam in enumerate(views):
+        w2c = np.asarray(cam["w2c"], np.float64)
+        depth = means @ w2c[2, :3] + w2c[2, 3]
+        n_z = int(np.count_nonzero((depth >= cam["near_plane"]) & (depth <= cam["far_plane"])))
+        fb = _native.accumulate(scene["means"], scene["scales"], scene["rotations"], scene["opacities"], scene["sh"],
+                                torch.from_numpy(np.asarray(cam["w2c"], np.float32)).to(dev),
+                                torch.from_numpy(np.asarray(cam["cam_position"], np.float32)).to(dev),
+                                cam["width"], cam["height"], cam["focal_x"], cam["focal_y"], cam["center_x"],
+                                cam["center_y"], cam["near_plane"], cam["far_plane"], False, 1, False, False, False,
+                                frame_idx=7)
+        won = fb[(fb >> 32) != 0xFFFFFFFF] & 0xFFFFFFFF
+        u = int(torch.unique(won).numel())
+        P = Q = cam["width"] * cam["height"]
+        spp = int(cam["spp"])
+        per = 12.0 * n + 32.0 * n_z + 16.0 * P + 204.0 * u
+        bytes_total += spp * per + 24.0 * Q * (1 + 2 * (spp - 1))
+        atomics += spp * work[v][1]
+    t = step_ms * 1e-3
+    t_hbm = bytes_total / (HBM_PEAK_GBS * 1e9)
+    t_atomic = atomics / atomics_per_s
+    return {
+        "definition": "per (view, sample): B = 12N + 32N_z + 16P + 204U + 24Q(1 + [mode != RESET]) bytes, "
+                      "A = accepted fragments (atomicMin each in a scatter); t_roof = max(B / 8 TB/s, A / R_atomic)",
+        "bytes_per_step": round(bytes_total),
+        "atomics_per_step": round(atomics),
+        "r_atomic_per_s": round(atomics_per_s, 1),
+        "t_hbm_ms": round(t_hbm * 1e3, 3),
+        "t_atomic_ms": round(t_atomic * 1e3, 3),
+        "step_ms": round(step_ms, 3),
+        "achieved_gbs": round(bytes_total / t / 1e9, 1),
+        "hbm_frac": round(t_hbm / t, 4),
+        "atomic_frac": round(t_atomic / t, 4),
+        "frac": round(max(t_hbm, t_atomic) / t, 4),
+    }
+
+
 # HBM bytes per launch of the tile rasteriser on the default workload (6M Gaussians, 64 views,
 # 8 samples per LDS pass), from rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0 (calibrated on a
 # 1 GiB device copy); the commands and raw counters are in profiles/tile_raster_c4_6M_64views.md.
@@ -451,6 +502,12 @@ def main() -> int:
         else:
             roof = roofline_batch(spec.n, views, acc_ms, work)
             log(f"batched accumulate: {roof['avg_launch_ms']} ms, {roof['achieved']} GB/s algorithmic")
+    roof_samples = None
+    if "batched" in result:
+        roof_samples = roofline_render_samples(scene_np, views, scene, result["batched"]["ms_per_step"], work,
+                                               roof_plain["atomics_per_s"])
+        log(f"render-sample roofline: HBM {roof_samples['hbm_frac']}, atomics {roof_samples['atomic_frac']} "
+            f"of the {roof_samples['step_ms']} ms step")
 
     cpu = None
     if not args.no_cpu:
@@ -482,6 +539,7 @@ def main() -> int:
         "p99_frame_ms": result["plain"]["p99_frame_ms"] if "plain" in result else None,
         "roofline": roof,
         "roofline_scatter": roof_plain,
+        "roofline_render_samples": roof_samples,
         "serving": serving,
         "cpu_baseline": cpu,
     }
