@@ -690,7 +690,7 @@ __global__ __launch_bounds__(kRangeThreads) void view_key_range_kernel(const Vie
 // that position on, read where the rasteriser's batches of 256 start; other positions of
 // keys_out are not written.
 constexpr uint32_t kSortBins = 1024;
-constexpr uint32_t kSortRegKeys = 16;  // entries per lane kept in registers for tiles of <= 4096
+constexpr uint32_t kSortRegKeys = 32;  // entries per lane kept in registers for tiles of <= 8192
 
 __device__ __forceinline__ uint32_t sort_bucket(uint32_t key, uint2 vk)
 {
