@@ -94,7 +94,7 @@ struct SingleTileLayout {
     uint32_t n_tiles, capacity, queue_capacity;
 };
 
-constexpr uint32_t kSingleTile = 16;
+constexpr uint32_t kSingleTile = kBinPixels;
 constexpr uint32_t kSingleQueue = 1u << 16;  // work queue of the scatter fallback
 
 // instance capacity: about 4 + 8 P / 2^21 per Gaussian for a framebuffer of P pixels (a
@@ -424,7 +424,7 @@ uint32_t default_tile_instances(const sgs_view* views, uint32_t n_views, uint32_
 
 constexpr uint32_t kResolveBlock = 256;
 
-constexpr uint32_t kBatchTile = 16;
+constexpr uint32_t kBatchTile = kBinPixels;
 
 // device view table: ViewDesc[n], then resolve block prefix[n+1] and tile prefix[n+1]
 size_t table_views_bytes(uint32_t n) { return ((size_t)n * sizeof(ViewDesc) + 255u) & ~size_t(255u); }
@@ -557,7 +557,9 @@ int check_views(const sgs_view* views, uint32_t n)
     }
     uint64_t tiles = 0;
     for (uint32_t i = 0; i < n; i++) tiles += (uint64_t)view_grid_w(views[i]) * view_grid_h(views[i]);
-    if (tiles >= (1ull << 31)) return fail(SGS_ERROR_INVALID_ARGUMENT, "too many framebuffer tiles in one batch");
+    // one rasteriser workgroup per 16x16 tile: kBinScale^2 per bin
+    if (tiles * kBinScale * kBinScale >= (1ull << 31))
+        return fail(SGS_ERROR_INVALID_ARGUMENT, "too many framebuffer tiles in one batch");
     return SGS_SUCCESS;
 }
 

@@ -339,7 +339,7 @@ hipError_t launch_batch_accumulate(const SceneArgs& sc, const ViewDesc* views, u
 // because every tile writes all of its pixels. The tested pixels, alpha and thresholds are
 // those of the scatter path, so the framebuffers are bit-identical to it.
 // ============================================================================================
-constexpr uint32_t kTile = 16;
+constexpr uint32_t kTile = kRasterPixels;
 constexpr uint32_t kWindow = 1024;    // tiles of a workgroup's LDS histogram window
 
 struct TileRange {
@@ -361,10 +361,10 @@ __device__ __forceinline__ bool gaussian_tiles_reg(float mx, float my, float mz,
     key = f2u(p.depth);
     const Aabb b = pixel_aabb(p, vd.cam);
     if (b.x1 <= b.x0 || b.y1 <= b.y0) return false;
-    t.tx0 = b.x0 / kTile;
-    t.tx1 = (b.x1 - 1u) / kTile;
-    t.ty0 = b.y0 / kTile;
-    t.ty1 = (b.y1 - 1u) / kTile;
+    t.tx0 = b.x0 / kBinPixels;
+    t.tx1 = (b.x1 - 1u) / kBinPixels;
+    t.ty0 = b.y0 / kBinPixels;
+    t.ty1 = (b.y1 - 1u) / kBinPixels;
     return true;
 }
 
@@ -914,7 +914,8 @@ __global__ __launch_bounds__(kBatchBlock, kLayers <= 8u ? 5 : 3) void batch_tile
     __shared__ uint32_t s_rect[kBatchBlock], s_end[kBatchBlock];
     __shared__ uint32_t s_wave[kBatchBlock / 64u];
     if (ctr->overflow) return;
-    const uint32_t gt = blockIdx.x;
+    // workgroup -> (bin gt, 16x16 tile q of the bin's kBinScale x kBinScale)
+    const uint32_t gt = blockIdx.x / (kBinScale * kBinScale), q = blockIdx.x % (kBinScale * kBinScale);
     uint32_t lo = 0, hi = n_views - 1u;
     while (lo < hi) {
         const uint32_t mid = (lo + hi + 1u) >> 1;
@@ -923,8 +924,10 @@ __global__ __launch_bounds__(kBatchBlock, kLayers <= 8u ? 5 : 3) void batch_tile
     }
     const ViewDesc& vd = views[lo];
     const uint32_t lt = gt - tile_prefix[lo];
-    const uint32_t ox = (lt % vd.grid_w) * kTile, oy = (lt / vd.grid_w) * kTile;
+    const uint32_t ox = (lt % vd.grid_w) * kBinPixels + (q % kBinScale) * kTile;
+    const uint32_t oy = (lt / vd.grid_w) * kBinPixels + (q / kBinScale) * kTile;
     const uint32_t W = vd.cam.width_i, H = vd.cam.height_i;
+    if (ox >= W || oy >= H) return;  // a tile of an edge bin beyond the framebuffer
     const size_t P = (size_t)W * H;
     const uint32_t begin = offsets[gt], end = offsets[gt + 1u];
     const uint32_t tid = threadIdx.x, lane = lane_id();
@@ -1168,10 +1171,12 @@ hipError_t launch_batch_tiles(const SceneArgs& sc, const ViewDesc* views, uint32
         if (e != hipSuccess) return e;
     }
     if (layers == 8u)
-        hipLaunchKernelGGL(batch_tile_raster_kernel<8>, dim3(tb.n_tiles), dim3(kBatchBlock), 0, stream, sc, tb.cov3d,
+        hipLaunchKernelGGL(batch_tile_raster_kernel<8>, dim3(tb.n_tiles * kBinScale * kBinScale), dim3(kBatchBlock), 0,
+                           stream, sc, tb.cov3d,
                            views, n_views, tile_prefix, tb.offsets, tb.sorted_instances, tb.sorted_keys, counters, fb);
     else
-        hipLaunchKernelGGL(batch_tile_raster_kernel<16>, dim3(tb.n_tiles), dim3(kBatchBlock), 0, stream, sc, tb.cov3d,
+        hipLaunchKernelGGL(batch_tile_raster_kernel<16>, dim3(tb.n_tiles * kBinScale * kBinScale), dim3(kBatchBlock),
+                           0, stream, sc, tb.cov3d,
                            views, n_views, tile_prefix, tb.offsets, tb.sorted_instances, tb.sorted_keys, counters, fb);
     return hipGetLastError();
 }

@@ -10,6 +10,13 @@
 
 namespace sgs {
 
+// Tile schedule geometry: instances are binned per kBinPixels x kBinPixels framebuffer bin;
+// the rasteriser keeps 16x16-pixel tiles in LDS, kBinScale^2 of them per bin, each reading
+// its bin's depth-sorted list.
+constexpr uint32_t kRasterPixels = 16;
+constexpr uint32_t kBinScale = 2;
+constexpr uint32_t kBinPixels = kRasterPixels * kBinScale;
+
 // Gaussian arrays as the kernels read them.
 struct SceneArgs {
     const float* means;
@@ -97,8 +104,8 @@ struct alignas(16) ViewDesc {
     uint32_t frame0;            // RNG frame of sample 0; sample s uses frame0 + s
     int32_t mode;               // temporal mode of sample 0; later samples are STATIC
     uint32_t block_begin;       // first resolve block of this view
-    uint32_t tile_begin;        // first 16x16 tile of this view in the batch's tile numbering
-    uint32_t grid_w, grid_h;    // tiles per row / column of this view's framebuffer
+    uint32_t tile_begin;        // first bin of this view in the batch's bin numbering
+    uint32_t grid_w, grid_h;    // bins per row / column of this view's framebuffer
     uint32_t level;             // resolve order: 1 + the level of the view whose output is this view's history
     uint64_t fb_offset;         // first framebuffer word of sample 0 (samples are P words apart)
     uint64_t samples_offset;    // REPROJECT views: first float4 of the sample-0 buffer
