@@ -95,13 +95,16 @@ typedef struct sgs_tuning {
                                          reference's cap (accumulate_scatter.cu:128). This one
                                          does change results when it binds. */
     uint32_t schedule;                /* stochastic transparency without hybrid scheduling: 0 or
-                                         SGS_BATCH_TILES (1) bins the view into 16x16 tiles and
-                                         rasterises each in LDS, the batched renderer's tile
+                                         SGS_BATCH_TILES (1) bins the view into framebuffer bins
+                                         and rasterises their 16x16 tiles in LDS, the batched renderer's tile
                                          schedule for one view and one sample; SGS_BATCH_SCATTER
                                          (2) scatters with global atomicMin as the reference
                                          does (accumulate_scatter.cu:19-77). Calls with blue
                                          noise or debug counters always scatter. Identical
                                          framebuffers either way. */
+    uint32_t bin_size;                /* tile schedule: edge in pixels of the framebuffer bins
+                                         whose depth-sorted instance lists the 16x16 raster tiles
+                                         read: 16, 32 or 64 (default 16 for one view) */
 } sgs_tuning;
 
 /* Replaces _C.k_buffer_size (render.cu:164). */
@@ -192,11 +195,14 @@ typedef struct sgs_batch_tuning {
                                          scatter schedule */
     uint32_t tile_samples;            /* samples held in LDS per pass over a tile (8 or 16;
                                          default 8) */
+    uint32_t bin_size;                /* edge in pixels of the framebuffer bins the instances are
+                                         listed and depth-sorted per: 16, 32 or 64 (default 32);
+                                         each bin's 16x16 raster tiles read its list */
 } sgs_batch_tuning;
 
-/* SGS_BATCH_TILES: every (view, Gaussian) is binned into the 16x16 framebuffer tiles its
- * bounding box overlaps; one workgroup per (view, tile) keeps the tile's words of up to 16
- * samples in LDS and writes them out once. SGS_BATCH_SCATTER: one persistent launch over
+/* SGS_BATCH_TILES: every (view, Gaussian) is binned into the framebuffer bins (bin_size) its
+ * bounding box overlaps and each bin's list is depth-sorted; one workgroup per 16x16 tile of
+ * a bin keeps the tile's words of up to 16 samples in LDS and writes them out once. SGS_BATCH_SCATTER: one persistent launch over
  * (Gaussian chunk, view) items with 64-bit atomicMin into the framebuffer in device memory
  * and a work queue for large Gaussians. */
 #define SGS_BATCH_TILES 1u

@@ -51,7 +51,7 @@ class sgs_tuning(C.Structure):
     _fields_ = [("sequential_threshold", C.c_uint32), ("binning_pixel_threshold", C.c_uint32),
                 ("binning_point_threshold", C.c_uint32), ("max_binned_gaussians", C.c_uint32),
                 ("tile_size", C.c_uint32), ("max_tile_instances", C.c_uint32), ("max_gps_points", C.c_uint32),
-                ("schedule", C.c_uint32)]
+                ("schedule", C.c_uint32), ("bin_size", C.c_uint32)]
 
 
 class sgs_view(C.Structure):
@@ -67,7 +67,7 @@ class sgs_batch_tuning(C.Structure):
     _fields_ = [("sequential_threshold", C.c_uint32), ("big_threshold", C.c_uint32),
                 ("entry_fragments", C.c_uint32), ("queue_capacity", C.c_uint32), ("view_major", C.c_uint32),
                 ("blocks_per_cu", C.c_uint32), ("schedule", C.c_uint32), ("max_tile_instances", C.c_uint32),
-                ("tile_samples", C.c_uint32)]
+                ("tile_samples", C.c_uint32), ("bin_size", C.c_uint32)]
 
 
 _P = C.POINTER
@@ -131,7 +131,8 @@ k_buffer_size: int = _lib.sgs_k_buffer_size()
 class Tuning(NamedTuple):
     """Scheduling knobs (sgs_tuning); 0 keeps the library default. Only max_gps_points changes
     results. schedule: 0 or 1 = the tile schedule for stochastic transparency without hybrid
-    scheduling (blue noise and debug counters always scatter), 2 = the reference's scatter."""
+    scheduling (blue noise and debug counters always scatter), 2 = the reference's scatter.
+    bin_size: the tile schedule's bin edge in pixels (16, 32 or 64; 0 = 16)."""
     sequential_threshold: int = 0
     binning_pixel_threshold: int = 0
     binning_point_threshold: int = 0
@@ -140,6 +141,7 @@ class Tuning(NamedTuple):
     max_tile_instances: int = 0
     max_gps_points: int = 0
     schedule: int = 0
+    bin_size: int = 0
 
     def c(self) -> sgs_tuning:
         return sgs_tuning(*[int(v) for v in self])

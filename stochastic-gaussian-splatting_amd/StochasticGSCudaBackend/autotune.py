@@ -86,6 +86,7 @@ RENDER_SPACE = {
     "max_binned_gaussians": [1 << 14, 1 << 16, 1 << 18, 1 << 20],
     "sequential_threshold": [2, 4, 8, 16],
     "schedule": [1, 2],
+    "bin_size": [16, 32],
 }
 
 
@@ -94,11 +95,13 @@ def tune_render(means, scales, rotations, opacities, sh_coefficients, settings: 
     """Fastest Tuning for the accumulation of ``settings`` (its own GPS / hybrid choice).
     Knobs that do not act on that schedule are left at their defaults."""
     if space is None:
-        space = dict(RENDER_SPACE)
+        # schedule and bin_size act only on stochastic transparency without hybrid scheduling
+        space = {k: v for k, v in RENDER_SPACE.items() if k not in ("schedule", "bin_size")}
         if not settings.hybrid_scheduling:
             space = {"sequential_threshold": space["sequential_threshold"]}
             if not settings.use_gps:
                 space["schedule"] = list(RENDER_SPACE["schedule"])
+                space["bin_size"] = list(RENDER_SPACE["bin_size"])
         elif settings.use_gps:
             space.pop("tile_size")
             space.pop("binning_pixel_threshold")
@@ -122,7 +125,7 @@ def tune_render(means, scales, rotations, opacities, sh_coefficients, settings: 
     return TuneResult(Tuning(**best), ms, [(Tuning(**c), t) for c, t in trials])
 
 
-BATCH_SPACE = {"schedule": [1, 2], "tile_samples": [8, 16]}
+BATCH_SPACE = {"schedule": [1, 2], "tile_samples": [8, 16], "bin_size": [32, 16, 64]}
 
 
 def tune_batch(means, scales, rotations, opacities, sh_coefficients, views, *, repeats: int = 2,

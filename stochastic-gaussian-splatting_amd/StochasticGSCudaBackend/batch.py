@@ -25,7 +25,9 @@ SCHEDULE_SCATTER = 2  # persistent scatter with atomicMin into device memory
 
 
 class BatchTuning(NamedTuple):
-    """sgs_batch_tuning; 0 keeps the library default. No field changes a result bit."""
+    """sgs_batch_tuning; 0 keeps the library default. No field changes a result bit.
+    bin_size: edge in pixels of the bins whose depth-sorted lists the 16x16 raster tiles
+    read (16, 32 or 64; 0 = 32)."""
     sequential_threshold: int = 0
     big_threshold: int = 0
     entry_fragments: int = 0
@@ -35,6 +37,7 @@ class BatchTuning(NamedTuple):
     schedule: int = 0
     max_tile_instances: int = 0
     tile_samples: int = 0
+    bin_size: int = 0
 
     def c(self) -> _native.sgs_batch_tuning:
         return _native.sgs_batch_tuning(*[int(v) for v in self])

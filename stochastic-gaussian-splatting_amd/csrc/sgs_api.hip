@@ -77,6 +77,8 @@ static Tuning resolve_tuning(const sgs_tuning* t, uint32_t n_gaussians)
     r.max_tile_instances = (t && t->max_tile_instances) ? t->max_tile_instances : (1u << 24);
     r.max_gps_points = t ? t->max_gps_points : 0u;
     r.tiles = !(t && t->schedule == SGS_BATCH_SCATTER);
+    // one sample of one view: 16x16 bins (larger bins leave small frames with few workgroups)
+    r.bin_log = (t && t->bin_size) ? bin_log2(t->bin_size) : 0u;
     return r;
 }
 
@@ -94,7 +96,6 @@ struct SingleTileLayout {
     uint32_t n_tiles, capacity, queue_capacity;
 };
 
-constexpr uint32_t kSingleTile = kBinPixels;
 constexpr uint32_t kSingleQueue = 1u << 16;  // work queue of the scatter fallback
 
 // instance capacity: about 4 + 8 P / 2^21 per Gaussian for a framebuffer of P pixels (a
@@ -106,11 +107,12 @@ uint32_t single_tile_capacity(uint64_t P, uint32_t n_gaussians)
     return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(want, 1ull << 16), 1ull << 31);
 }
 
-SingleTileLayout single_tile_layout(const FbCamera& cam, uint32_t n_gaussians)
+SingleTileLayout single_tile_layout(const FbCamera& cam, uint32_t n_gaussians, uint32_t bin_log)
 {
     SingleTileLayout L;
     const uint64_t P = (uint64_t)cam.width_i * cam.height_i;
-    L.n_tiles = ((cam.width_i + kSingleTile - 1u) / kSingleTile) * ((cam.height_i + kSingleTile - 1u) / kSingleTile);
+    const uint32_t bin = kRasterPixels << bin_log;
+    L.n_tiles = ((cam.width_i + bin - 1u) / bin) * ((cam.height_i + bin - 1u) / bin);
     L.capacity = single_tile_capacity(P, n_gaussians);
     L.queue_capacity = kSingleQueue;
     L.views_bytes = (sizeof(ViewDesc) + 255u) & ~size_t(255u);
@@ -134,7 +136,7 @@ SingleTileLayout single_tile_layout(const FbCamera& cam, uint32_t n_gaussians)
     L.cov3d = a.take(32u * (size_t)n_gaussians);
     L.pair_rect = a.take(8u * (size_t)n_gaussians);
     L.pair_key = a.take(4u * (size_t)n_gaussians);
-    L.table = a.take(L.views_bytes + 16u);
+    L.table = a.take(L.views_bytes + sizeof(SingleViewPrefixes));
     L.queue = a.take(sizeof(BigEntry) * (size_t)L.queue_capacity);
     L.total = a.offset;
     return L;
@@ -165,7 +167,7 @@ RenderLayout render_layout(const sgs_settings* s, const FbCamera& cam, const Tun
     L.reuse = s->spatial_reuse ? a.take(16u * P) : 0;
     const size_t hb = hybrid_bytes(s, cam, t);
     L.hybrid = hb ? a.take(hb) : 0;
-    L.tiles = single_tiles_allowed(s, t) ? a.take(single_tile_layout(cam, n_gaussians).total) : 0;
+    L.tiles = single_tiles_allowed(s, t) ? a.take(single_tile_layout(cam, n_gaussians, t.bin_log).total) : 0;
     L.total = a.offset;
     return L;
 }
@@ -204,12 +206,14 @@ int check_tuning(const sgs_tuning* t)
         return fail(SGS_ERROR_INVALID_ARGUMENT, "tuning.tile_size must be 16, 32 or 64");
     if (t && t->schedule > SGS_BATCH_SCATTER)
         return fail(SGS_ERROR_INVALID_ARGUMENT, "tuning.schedule must be 0, 1 (tiles) or 2 (scatter)");
+    if (t && t->bin_size && bin_log2(t->bin_size) >= kBinSizes)
+        return fail(SGS_ERROR_INVALID_ARGUMENT, "tuning.bin_size must be 16, 32 or 64");
     return SGS_SUCCESS;
 }
 
 // the tile schedule for one view and one sample into fb (defined with the batched renderer)
 int accumulate_tiles_single(const sgs_gaussians* g, const sgs_settings* s, const FbCamera& cam, uint64_t* fb,
-                            uint32_t frame_idx, char* tile_ws, hipStream_t stream);
+                            uint32_t frame_idx, uint32_t bin_log, char* tile_ws, hipStream_t stream);
 
 SceneArgs scene_args(const sgs_gaussians* g)
 {
@@ -224,7 +228,7 @@ int accumulate_into(const sgs_gaussians* g, const sgs_settings* s, const FbCamer
     // the tile schedule writes every framebuffer word (no clear); blue noise and the debug
     // counters exist in the scatter kernels only
     if (tile_ws && !(blue_noise && blue_noise_numel > 0) && !counters)
-        return accumulate_tiles_single(g, s, cam, fb, frame_idx, static_cast<char*>(tile_ws), stream);
+        return accumulate_tiles_single(g, s, cam, fb, frame_idx, t.bin_log, static_cast<char*>(tile_ws), stream);
     const size_t P = (size_t)cam.width_i * cam.height_i;
     hipError_t e = launch_fill_u64(fb, P, kEmpty, stream);
     if (e != hipSuccess) return hip_fail(e, "framebuffer clear");
@@ -357,7 +361,7 @@ size_t sgs_accumulate_workspace_size(const sgs_settings* settings, uint32_t n_ga
                                         settings->supersampling_factor);
     const Tuning t = resolve_tuning(tuning, n_gaussians);
     const size_t hb = hybrid_bytes(settings, cam, t);
-    if (single_tiles_allowed(settings, t)) return single_tile_layout(cam, n_gaussians).total;
+    if (single_tiles_allowed(settings, t)) return single_tile_layout(cam, n_gaussians, t.bin_log).total;
     return hb ? ((hb + 255u) & ~size_t(255u)) : 0;
 }
 
@@ -376,7 +380,7 @@ int sgs_accumulate(const sgs_gaussians* gaussians, const sgs_settings* s, const 
     const FbCamera cam = make_fb_camera(s->width, s->height, s->focal_x, s->focal_y, s->center_x, s->center_y,
                                         s->near_plane, s->far_plane, s->proper_antialiasing, s->supersampling_factor);
     const bool tiles = single_tiles_allowed(s, t);
-    const size_t need = tiles ? single_tile_layout(cam, gaussians->n_gaussians).total : hybrid_bytes(s, cam, t);
+    const size_t need = tiles ? single_tile_layout(cam, gaussians->n_gaussians, t.bin_log).total : hybrid_bytes(s, cam, t);
     if (need && (!workspace || workspace_bytes < need))
         return fail(SGS_ERROR_WORKSPACE_TOO_SMALL,
                     "workspace of " + std::to_string(need) + " bytes needed, got " + std::to_string(workspace_bytes));
@@ -406,6 +410,7 @@ BatchTuning resolve_batch_tuning(const sgs_batch_tuning* t)
     r.schedule = (t && t->schedule == SGS_BATCH_SCATTER) ? 1u : 0u;
     r.max_tile_instances = t ? t->max_tile_instances : 0u;
     r.tile_samples = (t && t->tile_samples == 16u) ? 16u : 8u;
+    r.bin_log = (t && t->bin_size) ? bin_log2(t->bin_size) : 1u;  // default 32x32-pixel bins
     return r;
 }
 
@@ -424,11 +429,10 @@ uint32_t default_tile_instances(const sgs_view* views, uint32_t n_views, uint32_
 
 constexpr uint32_t kResolveBlock = 256;
 
-constexpr uint32_t kBatchTile = kBinPixels;
-
-// device view table: ViewDesc[n], then resolve block prefix[n+1] and tile prefix[n+1]
+// device view table: ViewDesc[n], then the resolve block prefix[n+1] and, for each bin size
+// b, the bin prefix[n+1]
 size_t table_views_bytes(uint32_t n) { return ((size_t)n * sizeof(ViewDesc) + 255u) & ~size_t(255u); }
-size_t table_bytes(uint32_t n) { return table_views_bytes(n) + 8u * ((size_t)n + 1u); }
+size_t table_bytes(uint32_t n) { return table_views_bytes(n) + 4u * (1u + kBinSizes) * ((size_t)n + 1u); }
 
 uint32_t view_ss(const sgs_view& v) { return v.supersampling_factor < 1 ? 1u : (uint32_t)v.supersampling_factor; }
 uint64_t view_fb_pixels(const sgs_view& v)
@@ -436,8 +440,16 @@ uint64_t view_fb_pixels(const sgs_view& v)
     const uint64_t ss = view_ss(v);
     return (uint64_t)v.width * ss * v.height * ss;
 }
-uint32_t view_grid_w(const sgs_view& v) { return (uint32_t)((v.width * view_ss(v) + kBatchTile - 1u) / kBatchTile); }
-uint32_t view_grid_h(const sgs_view& v) { return (uint32_t)((v.height * view_ss(v) + kBatchTile - 1u) / kBatchTile); }
+uint32_t view_grid_w(const sgs_view& v, uint32_t bl)
+{
+    const uint32_t bin = kRasterPixels << bl;
+    return (uint32_t)((v.width * view_ss(v) + bin - 1u) / bin);
+}
+uint32_t view_grid_h(const sgs_view& v, uint32_t bl)
+{
+    const uint32_t bin = kRasterPixels << bl;
+    return (uint32_t)((v.height * view_ss(v) + bin - 1u) / bin);
+}
 
 // Workspace of a batch. The counters, the queue's ready flags and the tile counts and
 // cursors are contiguous so one fill clears them all.
@@ -460,7 +472,7 @@ BatchLayout batch_layout(const sgs_view* views, uint32_t n, uint32_t n_gaussians
         L.fb_words += P * views[v].spp;
         if (views[v].temporal_mode == SGS_TEMPORAL_REPROJECT) L.n_samples += P;
         blocks += ((uint64_t)views[v].width * views[v].height + kResolveBlock - 1u) / kResolveBlock;
-        L.n_tiles += (uint64_t)view_grid_w(views[v]) * view_grid_h(views[v]);
+        L.n_tiles += (uint64_t)view_grid_w(views[v], t.bin_log) * view_grid_h(views[v], t.bin_log);
     }
     L.n_blocks = (uint32_t)blocks;
     const bool tiles = t.schedule == 0u;
@@ -555,10 +567,12 @@ int check_views(const sgs_view* views, uint32_t n)
                                 "; a history may only be all outputs of an earlier view");
         }
     }
+    // one rasteriser workgroup per 16x16 tile (the same count at every bin size up to the
+    // partial tiles of edge bins, which return at once); bin prefixes of every bin size fit 32 bits
     uint64_t tiles = 0;
-    for (uint32_t i = 0; i < n; i++) tiles += (uint64_t)view_grid_w(views[i]) * view_grid_h(views[i]);
-    // one rasteriser workgroup per 16x16 tile: kBinScale^2 per bin
-    if (tiles * kBinScale * kBinScale >= (1ull << 31))
+    for (uint32_t i = 0; i < n; i++)
+        tiles += (uint64_t)view_grid_w(views[i], kBinSizes - 1u) * view_grid_h(views[i], kBinSizes - 1u);
+    if (tiles << (2u * (kBinSizes - 1u)) >= (1ull << 31))
         return fail(SGS_ERROR_INVALID_ARGUMENT, "too many framebuffer tiles in one batch");
     return SGS_SUCCESS;
 }
@@ -567,9 +581,9 @@ void encode_views(const sgs_view* views, uint32_t n, void* host)
 {
     ViewDesc* d = static_cast<ViewDesc*>(host);
     uint32_t* prefix = reinterpret_cast<uint32_t*>(static_cast<char*>(host) + table_views_bytes(n));
-    uint32_t* tile_prefix = prefix + n + 1u;
+    uint32_t* tile_prefix = prefix + n + 1u;  // [kBinSizes][n + 1]
     uint64_t fb_off = 0, smp_off = 0;
-    uint32_t block = 0, tile = 0;
+    uint32_t block = 0, tile[kBinSizes] = {};
     const std::vector<uint32_t> level = view_levels(views, n);
     for (uint32_t i = 0; i < n; i++) {
         const sgs_view& v = views[i];
@@ -593,9 +607,11 @@ void encode_views(const sgs_view* views, uint32_t n, void* host)
         x.frame0 = v.frame_idx;
         x.mode = v.temporal_mode;
         x.block_begin = block;
-        x.tile_begin = tile;
-        x.grid_w = view_grid_w(v);
-        x.grid_h = view_grid_h(v);
+        for (uint32_t b = 0; b < kBinSizes; b++) {
+            x.tile_begin[b] = tile[b];
+            x.grid_w[b] = view_grid_w(v, b);
+            x.grid_h[b] = view_grid_h(v, b);
+        }
         x.level = level[i];
         x.fb_offset = fb_off;
         x.samples_offset = smp_off;
@@ -604,21 +620,23 @@ void encode_views(const sgs_view* views, uint32_t n, void* host)
         std::memcpy(x.v2p, v.view_to_previous_view, sizeof x.v2p);
         d[i] = x;
         prefix[i] = block;
-        tile_prefix[i] = tile;
-        tile += x.grid_w * x.grid_h;
+        for (uint32_t b = 0; b < kBinSizes; b++) {
+            tile_prefix[b * (n + 1u) + i] = tile[b];
+            tile[b] += x.grid_w[b] * x.grid_h[b];
+        }
         const uint64_t P = view_fb_pixels(v);
         fb_off += P * v.spp;
         if (v.temporal_mode == SGS_TEMPORAL_REPROJECT) smp_off += P;
         block += (uint32_t)(((uint64_t)v.width * v.height + kResolveBlock - 1u) / kResolveBlock);
     }
     prefix[n] = block;
-    tile_prefix[n] = tile;
+    for (uint32_t b = 0; b < kBinSizes; b++) tile_prefix[b * (n + 1u) + n] = tile[b];
 }
 
 int accumulate_tiles_single(const sgs_gaussians* g, const sgs_settings* s, const FbCamera& cam, uint64_t* fb,
-                            uint32_t frame_idx, char* tile_ws, hipStream_t stream)
+                            uint32_t frame_idx, uint32_t bin_log, char* tile_ws, hipStream_t stream)
 {
-    const SingleTileLayout L = single_tile_layout(cam, g->n_gaussians);
+    const SingleTileLayout L = single_tile_layout(cam, g->n_gaussians, bin_log);
     // view table: everything but the camera pose from the host, the pose from the device
     // pointers (no host synchronisation)
     sgs_view v;
@@ -639,10 +657,10 @@ int accumulate_tiles_single(const sgs_gaussians* g, const sgs_settings* s, const
     std::vector<char> host(table_bytes(1));
     encode_views(&v, 1, host.data());
     const ViewDesc& d = *reinterpret_cast<const ViewDesc*>(host.data());
-    const uint32_t* prefix = reinterpret_cast<const uint32_t*>(host.data() + table_views_bytes(1));
+    SingleViewPrefixes prefixes;
+    std::memcpy(prefixes.w, host.data() + table_views_bytes(1), sizeof prefixes.w);
     ViewDesc* desc = reinterpret_cast<ViewDesc*>(tile_ws + L.table);
-    hipError_t e = launch_single_view_table(d, L.views_bytes, prefix[1], prefix[3], s->w2c, s->cam_position, desc,
-                                            stream);
+    hipError_t e = launch_single_view_table(d, L.views_bytes, prefixes, s->w2c, s->cam_position, desc, stream);
     if (e != hipSuccess) return hip_fail(e, "view table");
     BatchCounters* counters = reinterpret_cast<BatchCounters*>(tile_ws + L.counters);
     e = launch_zero_u32(reinterpret_cast<uint32_t*>(counters), (L.cleared_end - L.counters) / 4u, stream);
@@ -662,8 +680,9 @@ int accumulate_tiles_single(const sgs_gaussians* g, const sgs_settings* s, const
     tb.pair_key = reinterpret_cast<uint32_t*>(tile_ws + L.pair_key);
     tb.n_tiles = L.n_tiles;
     tb.capacity = L.capacity;
+    tb.bin_log = bin_log;
     const SceneArgs sc = scene_args(g);
-    const uint32_t* tile_prefix = reinterpret_cast<const uint32_t*>(tile_ws + L.table + L.views_bytes) + 2;
+    const uint32_t* tile_prefix = reinterpret_cast<const uint32_t*>(tile_ws + L.table + L.views_bytes) + 2u * (1u + bin_log);
     e = launch_batch_tiles(sc, desc, 1, tile_prefix, tb, 8u, counters, fb, nullptr, stream);
     if (e != hipSuccess) return hip_fail(e, "tile schedule");
     // instance list overflow: the scatter schedule on a cleared framebuffer, gated on the flag
@@ -712,6 +731,7 @@ int sgs_render_batch(const sgs_gaussians* gaussians, const sgs_view* views, uint
         return fail(SGS_ERROR_INVALID_ARGUMENT, "unknown batch schedule " + std::to_string(tuning->schedule));
     if (tuning && tuning->tile_samples && tuning->tile_samples != 8u && tuning->tile_samples != 16u)
         return fail(SGS_ERROR_INVALID_ARGUMENT, "tile_samples must be 8 or 16");
+    if (t.bin_log >= kBinSizes) return fail(SGS_ERROR_INVALID_ARGUMENT, "bin_size must be 16, 32 or 64");
     const BatchLayout L = batch_layout(views, n_views, gaussians->n_gaussians, t);
     if (!workspace || workspace_bytes < L.total)
         return fail(SGS_ERROR_WORKSPACE_TOO_SMALL,
@@ -734,7 +754,7 @@ int sgs_render_batch(const sgs_gaussians* gaussians, const sgs_view* views, uint
     }
     const ViewDesc* desc = reinterpret_cast<const ViewDesc*>(table);
     const uint32_t* prefix = reinterpret_cast<const uint32_t*>(table + table_views_bytes(n_views));
-    const uint32_t* tile_prefix = prefix + n_views + 1u;
+    const uint32_t* tile_prefix = prefix + (size_t)(n_views + 1u) * (1u + t.bin_log);
     BatchCounters* counters = reinterpret_cast<BatchCounters*>(ws + L.counters);
     uint32_t* ready = reinterpret_cast<uint32_t*>(ws + L.ready);
     BigEntry* queue = reinterpret_cast<BigEntry*>(ws + L.queue);
@@ -767,6 +787,7 @@ int sgs_render_batch(const sgs_gaussians* gaussians, const sgs_view* views, uint
         tb.pair_key = reinterpret_cast<uint32_t*>(ws + L.pair_key);
         tb.n_tiles = (uint32_t)L.n_tiles;
         tb.capacity = L.capacity;
+        tb.bin_log = t.bin_log;
         e = launch_batch_tiles(sc, desc, n_views, tile_prefix, tb, t.tile_samples, counters, fb,
                                stage_events ? static_cast<hipEvent_t>(stage_events[2]) : nullptr, stream);
         if (e != hipSuccess) return hip_fail(e, "batched tiles");

@@ -350,7 +350,7 @@ struct TileRange {
 // and cached view-independent terms are in registers (the operations of project_cached, so
 // the same bits); false if culled or its AABB is empty
 __device__ __forceinline__ bool gaussian_tiles_reg(float mx, float my, float mz, const Cov3d& cv, bool valid,
-                                                   const ViewDesc& vd, TileRange& t, uint32_t& key)
+                                                   const ViewDesc& vd, uint32_t bin_shift, TileRange& t, uint32_t& key)
 {
     if (!valid) return false;
     const float4 r3 = vd.r3;
@@ -361,10 +361,10 @@ __device__ __forceinline__ bool gaussian_tiles_reg(float mx, float my, float mz,
     key = f2u(p.depth);
     const Aabb b = pixel_aabb(p, vd.cam);
     if (b.x1 <= b.x0 || b.y1 <= b.y0) return false;
-    t.tx0 = b.x0 / kBinPixels;
-    t.tx1 = (b.x1 - 1u) / kBinPixels;
-    t.ty0 = b.y0 / kBinPixels;
-    t.ty1 = (b.y1 - 1u) / kBinPixels;
+    t.tx0 = b.x0 >> bin_shift;
+    t.tx1 = (b.x1 - 1u) >> bin_shift;
+    t.ty0 = b.y0 >> bin_shift;
+    t.ty1 = (b.y1 - 1u) >> bin_shift;
     return true;
 }
 
@@ -433,7 +433,7 @@ constexpr uint32_t kNoTiles = 0xffffffffu;
 template <bool kFill>
 __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, const float4* __restrict__ cov3d,
                                                                 const ViewDesc* __restrict__ views,
-                                                                uint32_t n_views,
+                                                                uint32_t n_views, uint32_t bl,
                                                                 uint32_t* __restrict__ tile_count,
                                                                 uint32_t* __restrict__ tile_cursor,
                                                                 const uint32_t* __restrict__ offsets,
@@ -491,6 +491,7 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
             }
             __syncthreads();
             const ViewDesc& vd = views[v];
+            const uint32_t bin0 = vd.tile_begin[bl], gw = vd.grid_w[bl];
             TileRange tr = { 0u, 0u, 0u, 0u };
             uint32_t key = 0;
             bool has = false;
@@ -505,7 +506,7 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
                 has = r.x != kNoTiles;
                 tr = { r.x & 0xffffu, r.y & 0xffffu, r.x >> 16, r.y >> 16 };
             } else {
-                has = gaussian_tiles_reg(mx, my, mz, cv, valid, vd, tr, key);
+                has = gaussian_tiles_reg(mx, my, mz, cv, valid, vd, kRasterLog2 + bl, tr, key);
                 if (g < sc.n) {
                     pair_rect[pair] = has ? make_uint2(tr.tx0 | (tr.ty0 << 16), tr.tx1 | (tr.ty1 << 16))
                                           : make_uint2(kNoTiles, kNoTiles);
@@ -544,7 +545,7 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
                 const uint32_t bg = bcast(g, l), bkey = bcast(key, l);
                 for (uint32_t k = lane; k < bn; k += 64u) {
                     const uint32_t ty = by0 + k / bw, tx = bx0 + k % bw;
-                    bin_direct<kFill>(vd.tile_begin + ty * vd.grid_w + tx, bg, bkey, tile_count, tile_cursor, offsets,
+                    bin_direct<kFill>(bin0 + ty * gw + tx, bg, bkey, tile_count, tile_cursor, offsets,
                                       instances, keys);
                 }
             }
@@ -563,7 +564,7 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
                     for (uint32_t i = tid; i < nw; i += kBatchBlock) {
                         const uint32_t c = hist[i];
                         if (!c) continue;
-                        const uint32_t gt = vd.tile_begin + (wy0 + i / wt) * vd.grid_w + wx0 + i % wt;
+                        const uint32_t gt = bin0 + (wy0 + i / wt) * gw + wx0 + i % wt;
                         if constexpr (kFill) {
                             hbase[i] = offsets[gt] + atomicAdd(&tile_cursor[gt], c);
                             hist[i] = 0u;
@@ -586,7 +587,7 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
                     // spread-out chunk: one global atomic per instance
                     for (uint32_t ty = tr.ty0; ty <= tr.ty1; ty++)
                         for (uint32_t tx = tr.tx0; tx <= tr.tx1; tx++)
-                            bin_direct<kFill>(vd.tile_begin + ty * vd.grid_w + tx, g, key, tile_count, tile_cursor,
+                            bin_direct<kFill>(bin0 + ty * gw + tx, g, key, tile_count, tile_cursor,
                                               offsets, instances, keys);
                 }
             }
@@ -895,7 +896,7 @@ template <uint32_t kLayers>
 __global__ __launch_bounds__(kBatchBlock, kLayers <= 8u ? 5 : 3) void batch_tile_raster_kernel(SceneArgs sc,
                                                                         const float4* __restrict__ cov3d,
                                                                         const ViewDesc* __restrict__ views,
-                                                                        uint32_t n_views,
+                                                                        uint32_t n_views, uint32_t bl,
                                                                         const uint32_t* __restrict__ tile_prefix,
                                                                         const uint32_t* __restrict__ offsets,
                                                                         const uint32_t* __restrict__ instances,
@@ -914,8 +915,8 @@ __global__ __launch_bounds__(kBatchBlock, kLayers <= 8u ? 5 : 3) void batch_tile
     __shared__ uint32_t s_rect[kBatchBlock], s_end[kBatchBlock];
     __shared__ uint32_t s_wave[kBatchBlock / 64u];
     if (ctr->overflow) return;
-    // workgroup -> (bin gt, 16x16 tile q of the bin's kBinScale x kBinScale)
-    const uint32_t gt = blockIdx.x / (kBinScale * kBinScale), q = blockIdx.x % (kBinScale * kBinScale);
+    // workgroup -> (bin gt, 16x16 tile q of the bin's 2^bl x 2^bl)
+    const uint32_t gt = blockIdx.x >> (2u * bl), q = blockIdx.x & ((1u << (2u * bl)) - 1u);
     uint32_t lo = 0, hi = n_views - 1u;
     while (lo < hi) {
         const uint32_t mid = (lo + hi + 1u) >> 1;
@@ -924,8 +925,9 @@ __global__ __launch_bounds__(kBatchBlock, kLayers <= 8u ? 5 : 3) void batch_tile
     }
     const ViewDesc& vd = views[lo];
     const uint32_t lt = gt - tile_prefix[lo];
-    const uint32_t ox = (lt % vd.grid_w) * kBinPixels + (q % kBinScale) * kTile;
-    const uint32_t oy = (lt / vd.grid_w) * kBinPixels + (q / kBinScale) * kTile;
+    const uint32_t gw = vd.grid_w[bl];
+    const uint32_t ox = ((lt % gw) << (kRasterLog2 + bl)) + (q & ((1u << bl) - 1u)) * kTile;
+    const uint32_t oy = ((lt / gw) << (kRasterLog2 + bl)) + (q >> bl) * kTile;
     const uint32_t W = vd.cam.width_i, H = vd.cam.height_i;
     if (ox >= W || oy >= H) return;  // a tile of an edge bin beyond the framebuffer
     const size_t P = (size_t)W * H;
@@ -1101,7 +1103,7 @@ __global__ __launch_bounds__(kBatchBlock, kLayers <= 8u ? 5 : 3) void batch_tile
     }
 }
 
-__global__ void single_view_table_kernel(ViewDesc d, uint32_t views_bytes, uint32_t n_blocks, uint32_t n_tiles,
+__global__ void single_view_table_kernel(ViewDesc d, uint32_t views_bytes, SingleViewPrefixes prefixes,
                                          const float* __restrict__ w2c, const float* __restrict__ cam_position,
                                          char* __restrict__ table)
 {
@@ -1114,17 +1116,14 @@ __global__ void single_view_table_kernel(ViewDesc d, uint32_t views_bytes, uint3
     d.cam_z = cam_position[2];
     *reinterpret_cast<ViewDesc*>(table) = d;
     uint32_t* prefix = reinterpret_cast<uint32_t*>(table + views_bytes);
-    prefix[0] = 0u;
-    prefix[1] = n_blocks;
-    prefix[2] = 0u;
-    prefix[3] = n_tiles;
+    for (uint32_t i = 0; i < 2u * (1u + kBinSizes); i++) prefix[i] = prefixes.w[i];
 }
 
-hipError_t launch_single_view_table(const ViewDesc& desc, size_t views_bytes, uint32_t n_blocks, uint32_t n_tiles,
+hipError_t launch_single_view_table(const ViewDesc& desc, size_t views_bytes, const SingleViewPrefixes& prefixes,
                                     const float* w2c, const float* cam_position, void* table, hipStream_t stream)
 {
-    hipLaunchKernelGGL(single_view_table_kernel, dim3(1), dim3(64), 0, stream, desc, (uint32_t)views_bytes, n_blocks,
-                       n_tiles, w2c, cam_position, static_cast<char*>(table));
+    hipLaunchKernelGGL(single_view_table_kernel, dim3(1), dim3(64), 0, stream, desc, (uint32_t)views_bytes, prefixes,
+                       w2c, cam_position, static_cast<char*>(table));
     return hipGetLastError();
 }
 
@@ -1133,6 +1132,7 @@ hipError_t launch_batch_tiles(const SceneArgs& sc, const ViewDesc* views, uint32
                               hipEvent_t raster_event, hipStream_t stream)
 {
     if (n_views == 0 || tb.n_tiles == 0) return hipSuccess;
+    if (tb.bin_log >= kBinSizes) return hipErrorInvalidValue;
     const uint32_t n_chunks = (sc.n + kBatchBlock - 1u) / kBatchBlock;
     const uint64_t n_items64 = (uint64_t)n_chunks * n_views;
     if (n_items64 >= 0xffffffffull) return hipErrorInvalidValue;
@@ -1153,7 +1153,7 @@ hipError_t launch_batch_tiles(const SceneArgs& sc, const ViewDesc* views, uint32
                            n_chunks, tb.view_keys);
         hipLaunchKernelGGL(cov3d_kernel, dim3(n_chunks), dim3(kBatchBlock), 0, stream, sc, tb.cov3d);
         hipLaunchKernelGGL((batch_bin_kernel<false>), dim3((unsigned)grid), dim3(kBatchBlock), 0, stream, sc, tb.cov3d,
-                           views, n_views, tb.tile_count, tb.tile_cursor, tb.offsets,
+                           views, n_views, tb.bin_log, tb.tile_count, tb.tile_cursor, tb.offsets,
                            tb.instances, tb.keys, tb.pair_rect, tb.pair_key, counters);
     }
     hipLaunchKernelGGL((exclusive_scan_kernel<CountInput>), dim3(1), dim3(kScanThreads), 0, stream,
@@ -1161,7 +1161,7 @@ hipError_t launch_batch_tiles(const SceneArgs& sc, const ViewDesc* views, uint32
                        &counters->overflow, tb.capacity, &counters->total_instances);
     if (sc.n > 0) {
         hipLaunchKernelGGL((batch_bin_kernel<true>), dim3((unsigned)grid), dim3(kBatchBlock), 0, stream, sc, tb.cov3d,
-                           views, n_views, tb.tile_count, tb.tile_cursor, tb.offsets,
+                           views, n_views, tb.bin_log, tb.tile_count, tb.tile_cursor, tb.offsets,
                            tb.instances, tb.keys, tb.pair_rect, tb.pair_key, counters);
     }
     hipLaunchKernelGGL(tile_sort_kernel, dim3(tb.n_tiles), dim3(kBatchBlock), 0, stream, n_views, tile_prefix,
@@ -1170,14 +1170,16 @@ hipError_t launch_batch_tiles(const SceneArgs& sc, const ViewDesc* views, uint32
         e = hipEventRecord(raster_event, stream);
         if (e != hipSuccess) return e;
     }
+    // one workgroup per 16x16 tile: 4^bin_log per bin (the caller keeps the total below 2^31)
+    const dim3 raster_grid(tb.n_tiles << (2u * tb.bin_log));
     if (layers == 8u)
-        hipLaunchKernelGGL(batch_tile_raster_kernel<8>, dim3(tb.n_tiles * kBinScale * kBinScale), dim3(kBatchBlock), 0,
-                           stream, sc, tb.cov3d,
-                           views, n_views, tile_prefix, tb.offsets, tb.sorted_instances, tb.sorted_keys, counters, fb);
+        hipLaunchKernelGGL(batch_tile_raster_kernel<8>, raster_grid, dim3(kBatchBlock), 0, stream, sc, tb.cov3d,
+                           views, n_views, tb.bin_log, tile_prefix, tb.offsets, tb.sorted_instances, tb.sorted_keys,
+                           counters, fb);
     else
-        hipLaunchKernelGGL(batch_tile_raster_kernel<16>, dim3(tb.n_tiles * kBinScale * kBinScale), dim3(kBatchBlock),
-                           0, stream, sc, tb.cov3d,
-                           views, n_views, tile_prefix, tb.offsets, tb.sorted_instances, tb.sorted_keys, counters, fb);
+        hipLaunchKernelGGL(batch_tile_raster_kernel<16>, raster_grid, dim3(kBatchBlock), 0, stream, sc, tb.cov3d,
+                           views, n_views, tb.bin_log, tile_prefix, tb.offsets, tb.sorted_instances, tb.sorted_keys,
+                           counters, fb);
     return hipGetLastError();
 }
 

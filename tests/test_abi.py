@@ -100,6 +100,15 @@ def test_workspace_sizes(native):
     g = native.sgs_gaussians(0x1000, 0x2000, 0x3000, 0x4000, 0x5000, 10, 16)
     assert lib.sgs_accumulate(C.byref(g), C.byref(s), None, 0, 0x6000, 0, C.byref(bad), None, None, 0, None) == -1
     assert b"schedule" in lib.sgs_last_error()
+    # bins of 16 pixels need more per-bin counters than bins of 64
+    fine, coarse = native.sgs_tuning(bin_size=16), native.sgs_tuning(bin_size=64)
+    big = _settings(native, width=1920, height=1080)
+    assert lib.sgs_accumulate_workspace_size(C.byref(big), 10, C.byref(fine)) > lib.sgs_accumulate_workspace_size(
+        C.byref(big), 10, C.byref(coarse))
+    bad_bin = native.sgs_tuning(bin_size=48)
+    assert lib.sgs_accumulate(C.byref(g), C.byref(s), None, 0, 0x6000, 0, C.byref(bad_bin), None, None, 0,
+                              None) == -1
+    assert b"bin_size" in lib.sgs_last_error()
 
 
 def test_python_api_mirrors_reference():
@@ -148,6 +157,14 @@ def test_batch_workspace_and_schedule_validation(native):
     g = _gaussians(native)
     rc = lib.sgs_render_batch(C.byref(g), views, 2, None, C.byref(bad), 0x100000, 1 << 40, None, None)
     assert rc == -1 and b"schedule" in lib.sgs_last_error()
+    bad_bin = native.sgs_batch_tuning(bin_size=8)
+    rc = lib.sgs_render_batch(C.byref(g), views, 2, None, C.byref(bad_bin), 0x100000, 1 << 40, None, None)
+    assert rc == -1 and b"bin_size" in lib.sgs_last_error()
+    # the tile counters scale with the bin count: 16-pixel bins need more than 64-pixel ones
+    fine = native.sgs_batch_tuning(bin_size=16, max_tile_instances=1 << 16)
+    coarse = native.sgs_batch_tuning(bin_size=64, max_tile_instances=1 << 16)
+    assert lib.sgs_render_batch_workspace_size(views, 2, 1000, C.byref(fine)) > lib.sgs_render_batch_workspace_size(
+        views, 2, 1000, C.byref(coarse))
     host = (C.c_uint8 * lib.sgs_batch_view_table_bytes(2))()
     assert lib.sgs_batch_encode_views(views, 2, host) == 0
 

@@ -75,6 +75,8 @@ def make_views():
     dict(),                                                   # defaults (tile schedule)
     dict(max_tile_instances=16),                              # tile list overflow: scatter fallback
     dict(tile_samples=8),                                     # 8 samples per LDS pass
+    dict(bin_size=16),                                        # one raster tile per bin
+    dict(bin_size=64, tile_samples=16),                       # 4x4 raster tiles per bin
     dict(schedule=2),                                         # scatter schedule
     dict(schedule=2, big_threshold=64, entry_fragments=64),   # most Gaussians through the work queue
     dict(schedule=2, big_threshold=64, entry_fragments=64, queue_capacity=3),  # queue overflow: in place
@@ -97,7 +99,7 @@ def test_batch_equals_plain_spp_loop(gpu, tuning):
         assert_close(outs[v], ref)
 
 
-@pytest.mark.parametrize("case", ["many_samples", "many_samples_8", "wide_views"])
+@pytest.mark.parametrize("case", ["many_samples", "many_samples_8", "wide_views", "wide_views_16"])
 def test_batch_tile_schedule_edges(gpu, case):
     """More than 16 samples (several LDS passes per tile), and views wider than a
     workgroup's 1024-tile binning window with large Gaussians (direct binning), each
@@ -115,7 +117,8 @@ def test_batch_tile_schedule_edges(gpu, case):
     views = [BatchView(settings_of(c), spp=n, frame_idx=f) for c, n, f in zip(cams, spps, frames)]
     print_case(test="batch_tile_edges", case=case)
     tiles = BatchRenderer(t["means"], t["scales"], t["rotations"], t["opacities"], t["sh"], views,
-                          BatchTuning(tile_samples=8 if case.endswith("_8") else 0))
+                          BatchTuning(tile_samples=8 if case.endswith("_8") else 0,
+                                      bin_size=16 if case.endswith("_16") else 0))
     scatter = BatchRenderer(t["means"], t["scales"], t["rotations"], t["opacities"], t["sh"], views,
                             BatchTuning(schedule=2))
     tiles.render()

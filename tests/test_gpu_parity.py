@@ -52,19 +52,21 @@ def print_case(**kw):
 SCATTER, TILES = 2, 1  # sgs_tuning.schedule
 
 
-@pytest.mark.parametrize("ss,aa,seq,schedule", [
-    (1, False, 8, SCATTER), (2, False, 8, SCATTER), (1, True, 8, SCATTER), (1, False, 1, SCATTER),
-    (1, False, 64, SCATTER),
-    (1, False, 0, TILES), (2, False, 0, TILES), (1, True, 0, TILES),
+@pytest.mark.parametrize("ss,aa,seq,schedule,bin_size", [
+    (1, False, 8, SCATTER, 0), (2, False, 8, SCATTER, 0), (1, True, 8, SCATTER, 0), (1, False, 1, SCATTER, 0),
+    (1, False, 64, SCATTER, 0),
+    (1, False, 0, TILES, 0), (2, False, 0, TILES, 0), (1, True, 0, TILES, 0),
+    (1, False, 0, TILES, 32), (2, False, 0, TILES, 64),   # bins of 2x2 and 4x4 raster tiles
 ])
-def test_st_framebuffer_bit_exact(gpu, oracle_mod, ss, aa, seq, schedule):
+def test_st_framebuffer_bit_exact(gpu, oracle_mod, ss, aa, seq, schedule, bin_size):
     arrays = small_scene(3000, seed=21, log_scale_mean=-3.6, log_scale_std=0.8)
     sc, t = oracle_scene(arrays), to_device(arrays)
     cam = small_camera(160, 120, supersampling_factor=ss, proper_antialiasing=aa)
     for frame in (0, 5, 1 << 20):
-        print_case(test="st", ss=ss, aa=aa, seq=seq, schedule=schedule, frame=frame)
+        print_case(test="st", ss=ss, aa=aa, seq=seq, schedule=schedule, bin_size=bin_size, frame=frame)
         ref, st = oracle_mod.accumulate(sc, cam, frame)
-        got = gpu_fb(gpu, t, cam, frame, tuning=gpu.Tuning(sequential_threshold=seq, schedule=schedule))
+        got = gpu_fb(gpu, t, cam, frame,
+                     tuning=gpu.Tuning(sequential_threshold=seq, schedule=schedule, bin_size=bin_size))
         assert got.shape == ref.shape
         assert np.array_equal(got, ref), f"{np.count_nonzero(got != ref)} of {ref.size} pixels differ"
 
