@@ -908,7 +908,7 @@ __global__ __launch_bounds__(kBatchBlock, kLayers <= 8u ? 5 : 3) void batch_tile
     __shared__ uint64_t s_frag[kBatchBlock];
     __shared__ uint64_t s_sub[kSubtiles];
     __shared__ uint32_t s_pmax[kTile * kTile];  // depth bits of each pixel's largest word
-    __shared__ uint16_t s_queue[kBatchBlock / 64u][128];  // per-wave (instance << 8 | pixel) queue
+    __shared__ uint16_t s_queue[kBatchBlock / 64u][192];  // per-wave (instance << 8 | pixel) queue
     __shared__ float s_msx[kBatchBlock], s_msy[kBatchBlock], s_o[kBatchBlock];
     __shared__ float s_thr[kBatchBlock];  // exponent below which a fragment's alpha is < 1/255
     __shared__ float s_ca[kBatchBlock], s_cb[kBatchBlock], s_cc[kBatchBlock];
@@ -988,10 +988,11 @@ __global__ __launch_bounds__(kBatchBlock, kLayers <= 8u ? 5 : 3) void batch_tile
                             cx1 = min(cx1, ox + lx1);
                             cy0 = max(cy0, oy + ly0);
                             cy1 = min(cy1, oy + ly1);
-                            const uint32_t rw = cx1 - cx0, rh = cy1 - cy0;
+                            // the walk visits horizontal pixel pairs: rp per row of rw pixels
+                            const uint32_t rw = cx1 - cx0, rh = cy1 - cy0, rp = (rw + 1u) >> 1;
                             rect = (cx0 - ox) | ((cy0 - oy) << 4) | ((rw - 1u) << 8) |
-                                   (((65536u + rw - 1u) / rw) << 12);
-                            cnt = rw * rh;
+                                   (((65536u + rp - 1u) / rp) << 12);
+                            cnt = rp * rh;
                         }
                     }
                     const ConicExp2 ce = conic_exp2(p.conic_x, p.conic_y, p.conic_z);
@@ -1008,20 +1009,20 @@ __global__ __launch_bounds__(kBatchBlock, kLayers <= 8u ? 5 : 3) void batch_tile
                 }
             }
             s_rect[tid] = rect;
-            // fragments of the batch's live instances, flattened over the workgroup's lanes
+            // pixel pairs of the batch's live instances, flattened over the workgroup's lanes
             const uint32_t incl = block_inclusive_scan(cnt, s_wave);
             s_end[tid] = incl;
             __syncthreads();
             const uint32_t total = s_end[kBatchBlock - 1u];
-            // Each lane walks a contiguous range of the batch's fragments (an instance's
-            // rectangle advances without division). Fragments that some sample word of
-            // their pixel still lies behind go to the wave's LDS queue; alpha and the
-            // samples are evaluated on full queues of 64, so no lane idles on a rejected
-            // fragment.
+            // Each lane walks a contiguous range of the batch's pixel pairs (an instance's
+            // rectangle advances without division; the second pixel of a pair lies beyond an
+            // odd-width row's end and is skipped). Fragments that some sample word of their
+            // pixel still lies behind go to the wave's LDS queue; alpha and the samples are
+            // evaluated on full queues of 64, so no lane idles on a rejected fragment.
             const uint32_t per = (total + kBatchBlock - 1u) / kBatchBlock;
             uint32_t k = tid * per;
             const uint32_t kend = min(k + per, total);
-            uint32_t j = 0, jend = 0, rect_j = 0, rw = 1, q = 0, r = 0, fdepth = 0;
+            uint32_t j = 0, jend = 0, rect_j = 0, rw = 1, rp = 1, q = 0, r = 0, fdepth = 0;
             float msx = 0.0f, msy = 0.0f, thr = 0.0f;
             ConicExp2 ce = { 0.0f, 0.0f, 0.0f };
             if (k < kend) {
@@ -1035,8 +1036,9 @@ __global__ __launch_bounds__(kBatchBlock, kLayers <= 8u ? 5 : 3) void batch_tile
                 const uint32_t f = k - (j ? s_end[j - 1u] : 0u);
                 rect_j = s_rect[j];
                 rw = ((rect_j >> 8) & 0xfu) + 1u;
-                q = (f * (rect_j >> 12)) >> 16;  // f / rw for f < 256, rw <= 16
-                r = f - q * rw;
+                rp = (rw + 1u) >> 1;
+                q = (f * (rect_j >> 12)) >> 16;  // f / rp for f < 128, rp <= 8
+                r = f - q * rp;
                 fdepth = (uint32_t)(s_frag[j] >> 32);
                 msx = s_msx[j];
                 msy = s_msy[j];
@@ -1046,7 +1048,7 @@ __global__ __launch_bounds__(kBatchBlock, kLayers <= 8u ? 5 : 3) void batch_tile
             uint16_t* queue = s_queue[tid / 64u];
             uint32_t count = 0;  // wave-uniform
             for (uint32_t it = 0; it < per; it++, k++) {
-                bool cand = false;
+                bool cand = false, cand2 = false;
                 uint32_t entry = 0;
                 if (k < kend) {
                     if (k == jend) {
@@ -1054,6 +1056,7 @@ __global__ __launch_bounds__(kBatchBlock, kLayers <= 8u ? 5 : 3) void batch_tile
                         jend = s_end[j];
                         rect_j = s_rect[j];
                         rw = ((rect_j >> 8) & 0xfu) + 1u;
+                        rp = (rw + 1u) >> 1;
                         q = 0;
                         r = 0;
                         fdepth = (uint32_t)(s_frag[j] >> 32);
@@ -1062,30 +1065,47 @@ __global__ __launch_bounds__(kBatchBlock, kLayers <= 8u ? 5 : 3) void batch_tile
                         thr = s_thr[j];
                         ce = { s_ca[j], s_cb[j], s_cc[j] };
                     }
-                    const uint32_t lx = (rect_j & 0xfu) + r, ly = ((rect_j >> 4) & 0xfu) + q;
+                    const uint32_t lx = (rect_j & 0xfu) + 2u * r, ly = ((rect_j >> 4) & 0xfu) + q;
                     const uint32_t l = ly * kTile + lx;
-                    if (++r == rw) {
+                    const bool two = 2u * r + 1u < rw;
+                    if (++r == rp) {
                         r = 0;
                         q++;
                     }
                     // some sample word of the pixel was behind the fragment at the batch start,
                     // and the fragment lies inside the instance's alpha >= 1/255 ellipse (the
                     // exponent is the one the alpha evaluation computes)
-                    const float e = fragment_exponent(ce, (float)(ox + lx) - msx, (float)(oy + ly) - msy);
+                    const float dy = (float)(oy + ly) - msy;
+                    const float e = fragment_exponent(ce, (float)(ox + lx) - msx, dy);
                     cand = fdepth <= s_pmax[l] && !(e < thr);
+                    if (two) {
+                        const float e2 = fragment_exponent(ce, (float)(ox + lx + 1u) - msx, dy);
+                        cand2 = fdepth <= s_pmax[l + 1u] && !(e2 < thr);
+                    }
                     entry = (j << 8) | l;
                 }
-                const uint64_t m = __ballot(cand);
-                if (!m) continue;
-                if (cand) queue[count + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = (uint16_t)entry;
-                count += (uint32_t)__popcll(m);
+                const uint64_t m = __ballot(cand), m2 = __ballot(cand2);
+                if (!(m | m2)) continue;
+                const uint64_t below = (1ull << lane) - 1ull;
+                const uint32_t n1 = (uint32_t)__popcll(m);
+                if (cand) queue[count + (uint32_t)__popcll(m & below)] = (uint16_t)entry;
+                if (cand2) queue[count + n1 + (uint32_t)__popcll(m2 & below)] = (uint16_t)(entry + 1u);
+                count += n1 + (uint32_t)__popcll(m2);
+                // count < 64 before this step and at most 128 entries were added: at most two
+                // full queues to evaluate, then fewer than 64 entries move to the front
                 if (count >= 64u) {
                     __builtin_amdgcn_wave_barrier();
                     tile_fragment<kLayers>(queue[lane], layer, s_frag, s_msx, s_msy, s_o, s_ca, s_cb, s_cc, ox, oy,
                                            G, fs_group);
-                    count -= 64u;
+                    uint32_t head = 64u;
+                    if (count >= 128u) {
+                        tile_fragment<kLayers>(queue[64u + lane], layer, s_frag, s_msx, s_msy, s_o, s_ca, s_cb, s_cc,
+                                               ox, oy, G, fs_group);
+                        head = 128u;
+                    }
+                    count -= head;
                     __builtin_amdgcn_wave_barrier();
-                    const uint32_t rest = lane < count ? queue[64u + lane] : 0u;
+                    const uint32_t rest = lane < count ? queue[head + lane] : 0u;
                     __builtin_amdgcn_wave_barrier();
                     if (lane < count) queue[lane] = rest;
                     __builtin_amdgcn_wave_barrier();
