@@ -340,7 +340,11 @@ hipError_t launch_batch_accumulate(const SceneArgs& sc, const ViewDesc* views, u
 // those of the scatter path, so the framebuffers are bit-identical to it.
 // ============================================================================================
 constexpr uint32_t kTile = kRasterPixels;
-constexpr uint32_t kWindow = 1024;    // tiles of a workgroup's LDS histogram window
+constexpr uint32_t kWindow = 512;    // tiles of a workgroup's LDS histogram window
+// Each window cell has kHistCopies adjacent counters (distinct LDS banks); lane l counts into
+// copy l % kHistCopies, so the many lanes of a spatially compact chunk that hit one bin
+// spread over that many addresses instead of serialising on one.
+constexpr uint32_t kHistCopies = 2;
 
 struct TileRange {
     uint32_t tx0, tx1, ty0, ty1;  // inclusive
@@ -445,12 +449,13 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
 {
     __shared__ uint32_t s_wx0, s_wx1, s_wy0, s_wy1;
     __shared__ unsigned long long s_total, s_visible;
-    __shared__ uint32_t hist[kWindow];
-    __shared__ uint32_t hbase[kWindow];
+    __shared__ uint32_t hist[kWindow * kHistCopies];
+    __shared__ uint32_t hbase[kWindow * kHistCopies];
     if constexpr (kFill) {
         if (ctr->overflow) return;
     }
     const uint32_t tid = threadIdx.x, lane = lane_id();
+    const uint32_t hc = lane & (kHistCopies - 1u);
     const uint32_t n_chunks = (sc.n + kBatchBlock - 1u) / kBatchBlock;
     if (tid == 0) {
         s_total = 0ull;
@@ -555,19 +560,31 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
                 const uint32_t wt = wx1 - wx0 + 1u, ht = wy1 - wy0 + 1u;
                 if (wt * ht <= kWindow) {
                     const uint32_t nw = wt * ht;
-                    for (uint32_t i = tid; i < nw; i += kBatchBlock) hist[i] = 0u;
+                    for (uint32_t i = tid; i < nw * kHistCopies; i += kBatchBlock) hist[i] = 0u;
                     __syncthreads();
                     if (small)
                         for (uint32_t ty = tr.ty0; ty <= tr.ty1; ty++)
-                            for (uint32_t tx = tr.tx0; tx <= tr.tx1; tx++) atomicAdd(&hist[(ty - wy0) * wt + tx - wx0], 1u);
+                            for (uint32_t tx = tr.tx0; tx <= tr.tx1; tx++)
+                                atomicAdd(&hist[((ty - wy0) * wt + tx - wx0) * kHistCopies + hc], 1u);
                     __syncthreads();
                     for (uint32_t i = tid; i < nw; i += kBatchBlock) {
-                        const uint32_t c = hist[i];
+                        uint32_t cc[kHistCopies], c = 0;
+#pragma unroll
+                        for (uint32_t q = 0; q < kHistCopies; q++) {
+                            cc[q] = hist[i * kHistCopies + q];
+                            c += cc[q];
+                        }
                         if (!c) continue;
                         const uint32_t gt = bin0 + (wy0 + i / wt) * gw + wx0 + i % wt;
                         if constexpr (kFill) {
-                            hbase[i] = offsets[gt] + atomicAdd(&tile_cursor[gt], c);
-                            hist[i] = 0u;
+                            // copy q's instances take the slots after copies 0 .. q-1
+                            uint32_t b = offsets[gt] + atomicAdd(&tile_cursor[gt], c);
+#pragma unroll
+                            for (uint32_t q = 0; q < kHistCopies; q++) {
+                                hbase[i * kHistCopies + q] = b;
+                                b += cc[q];
+                                hist[i * kHistCopies + q] = 0u;
+                            }
                         } else {
                             atomicAdd(&tile_count[gt], c);
                         }
@@ -577,7 +594,7 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
                         if (small)
                             for (uint32_t ty = tr.ty0; ty <= tr.ty1; ty++)
                                 for (uint32_t tx = tr.tx0; tx <= tr.tx1; tx++) {
-                                    const uint32_t i = (ty - wy0) * wt + tx - wx0;
+                                    const uint32_t i = ((ty - wy0) * wt + tx - wx0) * kHistCopies + hc;
                                     const uint32_t slot = hbase[i] + atomicAdd(&hist[i], 1u);
                                     instances[slot] = g;
                                     keys[slot] = key;
