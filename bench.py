@@ -594,10 +594,12 @@ def bench_batched(args, scene, views, S, plain_out0, dev):
         res = tune_batch(scene["means"], scene["scales"], scene["rotations"], scene["opacities"], scene["sh"], bviews,
                          repeats=2)
         tuning = res.best._replace(view_major=int(args.view_major))
+        bviews = res.views
         args.schedule = "tiles" if tuning.schedule in (0, SCHEDULE_TILES) else "scatter"
         args.tile_samples = tuning.tile_samples or 8
-        tuned = {"best": dict(tuning._asdict()), "seconds": round(time.perf_counter() - ts, 1),
-                 "trials": [[{k: v for k, v in c._asdict().items() if v}, round(ms, 3)] for c, ms in res.trials]}
+        tuned = {"best": dict(tuning._asdict()), "view_bin_sizes": sorted({v.bin_size for v in bviews}),
+                 "seconds": round(time.perf_counter() - ts, 1),
+                 "trials": [[dict(c), round(ms, 3)] for c, ms in res.trials]}
         log(f"autotuned batch: {tuned['best']} from {tuned['trials']}")
     br = BatchRenderer(scene["means"], scene["scales"], scene["rotations"], scene["opacities"], scene["sh"], bviews,
                        tuning)

@@ -174,6 +174,11 @@ typedef struct sgs_view {
     float view_to_previous_view[16];  /* host copy, read in REPROJECT mode */
     sgs_history history;              /* device, read unless RESET */
     sgs_history out;                  /* device outputs (H,W,*) */
+    uint32_t bin_size;                /* tile schedule: edge in pixels of the framebuffer bins
+                                         this view's instances are listed and depth-sorted per,
+                                         read by the bins' 16x16 raster tiles: 16, 32 or 64; 0 =
+                                         16 for framebuffers (W ss x H ss) of at most 2^19
+                                         pixels, else 32. No result bit depends on it. */
 } sgs_view;
 
 /* Scheduling of the batched accumulation; zero fields take defaults. None of them changes
@@ -195,13 +200,10 @@ typedef struct sgs_batch_tuning {
                                          scatter schedule */
     uint32_t tile_samples;            /* samples held in LDS per pass over a tile (8 or 16;
                                          default 8) */
-    uint32_t bin_size;                /* edge in pixels of the framebuffer bins the instances are
-                                         listed and depth-sorted per: 16, 32 or 64 (default 32);
-                                         each bin's 16x16 raster tiles read its list */
 } sgs_batch_tuning;
 
-/* SGS_BATCH_TILES: every (view, Gaussian) is binned into the framebuffer bins (bin_size) its
- * bounding box overlaps and each bin's list is depth-sorted; one workgroup per 16x16 tile of
+/* SGS_BATCH_TILES: every (view, Gaussian) is binned into the framebuffer bins (the view's
+ * bin_size) its bounding box overlaps and each bin's list is depth-sorted; one workgroup per 16x16 tile of
  * a bin keeps the tile's words of up to 16 samples in LDS and writes them out once. SGS_BATCH_SCATTER: one persistent launch over
  * (Gaussian chunk, view) items with 64-bit atomicMin into the framebuffer in device memory
  * and a work queue for large Gaussians. */

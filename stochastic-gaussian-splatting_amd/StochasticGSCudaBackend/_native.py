@@ -60,14 +60,14 @@ class sgs_view(C.Structure):
                 ("center_y", C.c_float), ("near_plane", C.c_float), ("far_plane", C.c_float),
                 ("proper_antialiasing", C.c_int32), ("supersampling_factor", C.c_int32), ("spp", C.c_uint32),
                 ("frame_idx", C.c_uint32), ("temporal_mode", C.c_int32), ("view_to_previous_view", C.c_float * 16),
-                ("history", sgs_history), ("out", sgs_history)]
+                ("history", sgs_history), ("out", sgs_history), ("bin_size", C.c_uint32)]
 
 
 class sgs_batch_tuning(C.Structure):
     _fields_ = [("sequential_threshold", C.c_uint32), ("big_threshold", C.c_uint32),
                 ("entry_fragments", C.c_uint32), ("queue_capacity", C.c_uint32), ("view_major", C.c_uint32),
                 ("blocks_per_cu", C.c_uint32), ("schedule", C.c_uint32), ("max_tile_instances", C.c_uint32),
-                ("tile_samples", C.c_uint32), ("bin_size", C.c_uint32)]
+                ("tile_samples", C.c_uint32)]
 
 
 _P = C.POINTER

@@ -11,8 +11,10 @@ median. The max_gps_points cap does change results and is never tuned.
   binning pixel and point thresholds, max_binned_gaussians and the sequential threshold
   (Tuning), timing the accumulation stage (sgs_accumulate).
 * ``tune_batch`` tunes a batched render: schedule and samples per LDS pass of the tile
-  schedule (BatchTuning), timing whole renders; the instance capacity is set from the
-  measured instance count with a margin, which shrinks the workspace.
+  schedule (BatchTuning) and the views' bin sizes (16-pixel bins for framebuffers up to a
+  pixel-count threshold, 32 above; ``TuneResult.views`` holds the views with the chosen bin
+  sizes), timing whole renders; the instance capacity is set from the measured instance
+  count with a margin, which shrinks the workspace.
 """
 from __future__ import annotations
 
@@ -31,6 +33,7 @@ class TuneResult:
     best: object
     ms: float
     trials: list = field(default_factory=list)  # (config, ms) in evaluation order
+    views: list | None = None  # tune_batch: the views with the chosen bin sizes
 
 
 def coordinate_search(evaluate: Callable[[dict], float], space: dict[str, Sequence], start: dict,
@@ -125,7 +128,23 @@ def tune_render(means, scales, rotations, opacities, sh_coefficients, settings: 
     return TuneResult(Tuning(**best), ms, [(Tuning(**c), t) for c, t in trials])
 
 
-BATCH_SPACE = {"schedule": [1, 2], "tile_samples": [8, 16], "bin_size": [32, 16, 64]}
+# small_bin_max_pixels: views whose framebuffer has at most this many pixels get 16-pixel
+# bins, the others 32 (-1: each view's own bin_size, by default the library's choice)
+BATCH_SPACE = {"schedule": [1, 2], "tile_samples": [8, 16], "small_bin_max_pixels": [-1, 0, 1 << 18, 1 << 21]}
+
+
+def views_with_bins(views, small_bin_max_pixels: int) -> list:
+    """The views with 16-pixel bins for framebuffers of at most ``small_bin_max_pixels``
+    pixels and 32-pixel bins above (-1: unchanged)."""
+    import dataclasses
+    if small_bin_max_pixels < 0:
+        return list(views)
+    out = []
+    for v in views:
+        ss = max(1, int(v.settings.supersampling_factor))
+        px = int(v.settings.width) * ss * int(v.settings.height) * ss
+        out.append(dataclasses.replace(v, bin_size=16 if px <= small_bin_max_pixels else 32))
+    return out
 
 
 def tune_batch(means, scales, rotations, opacities, sh_coefficients, views, *, repeats: int = 2,
@@ -135,18 +154,26 @@ def tune_batch(means, scales, rotations, opacities, sh_coefficients, views, *, r
     from .batch import BatchRenderer, BatchTuning
     space = dict(BATCH_SPACE if space is None else space)
 
+    def split(cfg: dict) -> tuple:
+        cfg = dict(cfg)
+        vs = views_with_bins(views, cfg.pop("small_bin_max_pixels", -1))
+        return BatchTuning(**cfg), vs
+
     def evaluate(cfg: dict) -> float:
-        r = BatchRenderer(means, scales, rotations, opacities, sh_coefficients, views, BatchTuning(**cfg))
+        tuning, vs = split(cfg)
+        r = BatchRenderer(means, scales, rotations, opacities, sh_coefficients, vs, tuning)
         ms = _time_ms(r.render, repeats)
         del r
         return ms
 
     best, ms, trials = coordinate_search(evaluate, space, {k: v[0] for k, v in space.items()}, passes=1)
+    best_views = views_with_bins(views, best.get("small_bin_max_pixels", -1))
+    best = {k: v for k, v in best.items() if k != "small_bin_max_pixels"}
     if best.get("schedule", 1) == 1:
-        probe = BatchRenderer(means, scales, rotations, opacities, sh_coefficients, views, BatchTuning(**best))
+        probe = BatchRenderer(means, scales, rotations, opacities, sh_coefficients, best_views, BatchTuning(**best))
         probe.render()
         n_inst, _, overflowed = probe.tile_stats()
         del probe
         if not overflowed and n_inst > 0:
             best["max_tile_instances"] = min(int(n_inst * capacity_margin) + 1, (1 << 31))
-    return TuneResult(BatchTuning(**best), ms, [(BatchTuning(**c), t) for c, t in trials])
+    return TuneResult(BatchTuning(**best), ms, [(dict(c), t) for c, t in trials], best_views)

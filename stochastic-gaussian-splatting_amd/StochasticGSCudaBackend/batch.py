@@ -25,9 +25,7 @@ SCHEDULE_SCATTER = 2  # persistent scatter with atomicMin into device memory
 
 
 class BatchTuning(NamedTuple):
-    """sgs_batch_tuning; 0 keeps the library default. No field changes a result bit.
-    bin_size: edge in pixels of the bins whose depth-sorted lists the 16x16 raster tiles
-    read (16, 32 or 64; 0 = 32)."""
+    """sgs_batch_tuning; 0 keeps the library default. No field changes a result bit."""
     sequential_threshold: int = 0
     big_threshold: int = 0
     entry_fragments: int = 0
@@ -37,7 +35,6 @@ class BatchTuning(NamedTuple):
     schedule: int = 0
     max_tile_instances: int = 0
     tile_samples: int = 0
-    bin_size: int = 0
 
     def c(self) -> _native.sgs_batch_tuning:
         return _native.sgs_batch_tuning(*[int(v) for v in self])
@@ -51,6 +48,9 @@ class BatchView:
     temporal_mode: TemporalMode = TemporalMode.RESET
     history: History | None = None
     view_to_previous_view: torch.Tensor | np.ndarray | None = None
+    # tile schedule bins of this view (16, 32 or 64 pixels; 0 = 16 for framebuffers of at most
+    # 2^19 pixels, else 32); no result bit depends on it
+    bin_size: int = 0
 
 
 def _host(t, shape) -> np.ndarray:
@@ -99,6 +99,7 @@ class BatchRenderer:
             cv.supersampling_factor = int(s.supersampling_factor)
             cv.spp = int(v.spp)
             cv.temporal_mode = int(v.temporal_mode)
+            cv.bin_size = int(v.bin_size)
             v2p = np.eye(4, dtype=np.float32) if v.view_to_previous_view is None else _host(v.view_to_previous_view, 16)
             cv.view_to_previous_view[:] = np.asarray(v2p, np.float32).reshape(16).tolist()
             if int(v.temporal_mode) != 0:

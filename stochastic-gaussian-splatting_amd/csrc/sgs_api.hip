@@ -410,7 +410,6 @@ BatchTuning resolve_batch_tuning(const sgs_batch_tuning* t)
     r.schedule = (t && t->schedule == SGS_BATCH_SCATTER) ? 1u : 0u;
     r.max_tile_instances = t ? t->max_tile_instances : 0u;
     r.tile_samples = (t && t->tile_samples == 16u) ? 16u : 8u;
-    r.bin_log = (t && t->bin_size) ? bin_log2(t->bin_size) : 1u;  // default 32x32-pixel bins
     return r;
 }
 
@@ -429,10 +428,10 @@ uint32_t default_tile_instances(const sgs_view* views, uint32_t n_views, uint32_
 
 constexpr uint32_t kResolveBlock = 256;
 
-// device view table: ViewDesc[n], then the resolve block prefix[n+1] and, for each bin size
-// b, the bin prefix[n+1]
+// device view table: ViewDesc[n], then the resolve block prefix[n+1], the bin prefix[n+1] and
+// the rasteriser workgroup prefix[n+1]
 size_t table_views_bytes(uint32_t n) { return ((size_t)n * sizeof(ViewDesc) + 255u) & ~size_t(255u); }
-size_t table_bytes(uint32_t n) { return table_views_bytes(n) + 4u * (1u + kBinSizes) * ((size_t)n + 1u); }
+size_t table_bytes(uint32_t n) { return table_views_bytes(n) + 12u * ((size_t)n + 1u); }
 
 uint32_t view_ss(const sgs_view& v) { return v.supersampling_factor < 1 ? 1u : (uint32_t)v.supersampling_factor; }
 uint64_t view_fb_pixels(const sgs_view& v)
@@ -440,15 +439,27 @@ uint64_t view_fb_pixels(const sgs_view& v)
     const uint64_t ss = view_ss(v);
     return (uint64_t)v.width * ss * v.height * ss;
 }
-uint32_t view_grid_w(const sgs_view& v, uint32_t bl)
+// bins of (16 << b) pixels: the view's bin_size, else 16 for framebuffers of at most
+// kSmallBinMaxPixels pixels and 32 above (check_views rejects other sizes)
+uint32_t view_bin_log(const sgs_view& v)
 {
-    const uint32_t bin = kRasterPixels << bl;
+    if (v.bin_size) return bin_log2(v.bin_size);
+    return view_fb_pixels(v) <= kSmallBinMaxPixels ? 0u : 1u;
+}
+uint32_t view_grid_w(const sgs_view& v)
+{
+    const uint32_t bin = kRasterPixels << view_bin_log(v);
     return (uint32_t)((v.width * view_ss(v) + bin - 1u) / bin);
 }
-uint32_t view_grid_h(const sgs_view& v, uint32_t bl)
+uint32_t view_grid_h(const sgs_view& v)
 {
-    const uint32_t bin = kRasterPixels << bl;
+    const uint32_t bin = kRasterPixels << view_bin_log(v);
     return (uint32_t)((v.height * view_ss(v) + bin - 1u) / bin);
+}
+// rasteriser workgroups of a view: 16x16 tiles of its bins, 4^b per bin
+uint64_t view_raster_groups(const sgs_view& v)
+{
+    return ((uint64_t)view_grid_w(v) * view_grid_h(v)) << (2u * view_bin_log(v));
 }
 
 // Workspace of a batch. The counters, the queue's ready flags and the tile counts and
@@ -456,7 +467,7 @@ uint32_t view_grid_h(const sgs_view& v, uint32_t bl)
 struct BatchLayout {
     size_t counters, ready, tile_count, tile_cursor, cleared_end, offsets, instances, keys, sorted_instances, sorted_keys,
         centroids, cov3d, pair_rect, pair_key, view_keys, queue, table, fb, samples, total;
-    uint64_t fb_words, n_samples, n_tiles;
+    uint64_t fb_words, n_samples, n_tiles, n_raster;
     uint32_t n_blocks, capacity;
 };
 
@@ -466,13 +477,15 @@ BatchLayout batch_layout(const sgs_view* views, uint32_t n, uint32_t n_gaussians
     L.fb_words = 0;
     L.n_samples = 0;
     L.n_tiles = 0;
+    L.n_raster = 0;
     uint64_t blocks = 0;
     for (uint32_t v = 0; v < n; v++) {
         const uint64_t P = view_fb_pixels(views[v]);
         L.fb_words += P * views[v].spp;
         if (views[v].temporal_mode == SGS_TEMPORAL_REPROJECT) L.n_samples += P;
         blocks += ((uint64_t)views[v].width * views[v].height + kResolveBlock - 1u) / kResolveBlock;
-        L.n_tiles += (uint64_t)view_grid_w(views[v], t.bin_log) * view_grid_h(views[v], t.bin_log);
+        L.n_tiles += (uint64_t)view_grid_w(views[v]) * view_grid_h(views[v]);
+        L.n_raster += view_raster_groups(views[v]);
     }
     L.n_blocks = (uint32_t)blocks;
     const bool tiles = t.schedule == 0u;
@@ -531,6 +544,8 @@ int check_views(const sgs_view* views, uint32_t n)
         if ((uint64_t)v.width * ss > 65535u || (uint64_t)v.height * ss > 65535u || view_fb_pixels(v) >= (1ull << 32))
             return fail(SGS_ERROR_INVALID_ARGUMENT, at + "framebuffer too large");
         if (v.spp < 1 || v.spp > 4096) return fail(SGS_ERROR_INVALID_ARGUMENT, at + "spp must be in [1, 4096]");
+        if (v.bin_size && bin_log2(v.bin_size) >= kBinSizes)
+            return fail(SGS_ERROR_INVALID_ARGUMENT, at + "bin_size must be 0, 16, 32 or 64");
         if (!(v.focal_x != 0.0f) || !(v.focal_y != 0.0f)) return fail(SGS_ERROR_INVALID_ARGUMENT, at + "focal length is 0");
         if (v.temporal_mode < SGS_TEMPORAL_RESET || v.temporal_mode > SGS_TEMPORAL_REPROJECT)
             return fail(SGS_ERROR_INVALID_ARGUMENT, at + "invalid temporal mode " + std::to_string(v.temporal_mode));
@@ -567,13 +582,10 @@ int check_views(const sgs_view* views, uint32_t n)
                                 "; a history may only be all outputs of an earlier view");
         }
     }
-    // one rasteriser workgroup per 16x16 tile (the same count at every bin size up to the
-    // partial tiles of edge bins, which return at once); bin prefixes of every bin size fit 32 bits
-    uint64_t tiles = 0;
-    for (uint32_t i = 0; i < n; i++)
-        tiles += (uint64_t)view_grid_w(views[i], kBinSizes - 1u) * view_grid_h(views[i], kBinSizes - 1u);
-    if (tiles << (2u * (kBinSizes - 1u)) >= (1ull << 31))
-        return fail(SGS_ERROR_INVALID_ARGUMENT, "too many framebuffer tiles in one batch");
+    // one rasteriser workgroup per 16x16 tile of every view's bins (at least as many as bins)
+    uint64_t groups = 0;
+    for (uint32_t i = 0; i < n; i++) groups += view_raster_groups(views[i]);
+    if (groups >= (1ull << 31)) return fail(SGS_ERROR_INVALID_ARGUMENT, "too many framebuffer tiles in one batch");
     return SGS_SUCCESS;
 }
 
@@ -581,9 +593,10 @@ void encode_views(const sgs_view* views, uint32_t n, void* host)
 {
     ViewDesc* d = static_cast<ViewDesc*>(host);
     uint32_t* prefix = reinterpret_cast<uint32_t*>(static_cast<char*>(host) + table_views_bytes(n));
-    uint32_t* tile_prefix = prefix + n + 1u;  // [kBinSizes][n + 1]
+    uint32_t* tile_prefix = prefix + n + 1u;
+    uint32_t* raster_prefix = tile_prefix + n + 1u;
     uint64_t fb_off = 0, smp_off = 0;
-    uint32_t block = 0, tile[kBinSizes] = {};
+    uint32_t block = 0, tile = 0, group = 0;
     const std::vector<uint32_t> level = view_levels(views, n);
     for (uint32_t i = 0; i < n; i++) {
         const sgs_view& v = views[i];
@@ -607,11 +620,11 @@ void encode_views(const sgs_view* views, uint32_t n, void* host)
         x.frame0 = v.frame_idx;
         x.mode = v.temporal_mode;
         x.block_begin = block;
-        for (uint32_t b = 0; b < kBinSizes; b++) {
-            x.tile_begin[b] = tile[b];
-            x.grid_w[b] = view_grid_w(v, b);
-            x.grid_h[b] = view_grid_h(v, b);
-        }
+        x.tile_begin = tile;
+        x.grid_w = view_grid_w(v);
+        x.grid_h = view_grid_h(v);
+        x.bin_log = view_bin_log(v);
+        x.raster_begin = group;
         x.level = level[i];
         x.fb_offset = fb_off;
         x.samples_offset = smp_off;
@@ -620,17 +633,18 @@ void encode_views(const sgs_view* views, uint32_t n, void* host)
         std::memcpy(x.v2p, v.view_to_previous_view, sizeof x.v2p);
         d[i] = x;
         prefix[i] = block;
-        for (uint32_t b = 0; b < kBinSizes; b++) {
-            tile_prefix[b * (n + 1u) + i] = tile[b];
-            tile[b] += x.grid_w[b] * x.grid_h[b];
-        }
+        tile_prefix[i] = tile;
+        raster_prefix[i] = group;
+        tile += x.grid_w * x.grid_h;
+        group += (uint32_t)view_raster_groups(v);
         const uint64_t P = view_fb_pixels(v);
         fb_off += P * v.spp;
         if (v.temporal_mode == SGS_TEMPORAL_REPROJECT) smp_off += P;
         block += (uint32_t)(((uint64_t)v.width * v.height + kResolveBlock - 1u) / kResolveBlock);
     }
     prefix[n] = block;
-    for (uint32_t b = 0; b < kBinSizes; b++) tile_prefix[b * (n + 1u) + n] = tile[b];
+    tile_prefix[n] = tile;
+    raster_prefix[n] = group;
 }
 
 int accumulate_tiles_single(const sgs_gaussians* g, const sgs_settings* s, const FbCamera& cam, uint64_t* fb,
@@ -654,6 +668,7 @@ int accumulate_tiles_single(const sgs_gaussians* g, const sgs_settings* s, const
     v.spp = 1;
     v.frame_idx = frame_idx;
     v.temporal_mode = SGS_TEMPORAL_RESET;
+    v.bin_size = kRasterPixels << bin_log;
     std::vector<char> host(table_bytes(1));
     encode_views(&v, 1, host.data());
     const ViewDesc& d = *reinterpret_cast<const ViewDesc*>(host.data());
@@ -680,10 +695,10 @@ int accumulate_tiles_single(const sgs_gaussians* g, const sgs_settings* s, const
     tb.pair_key = reinterpret_cast<uint32_t*>(tile_ws + L.pair_key);
     tb.n_tiles = L.n_tiles;
     tb.capacity = L.capacity;
-    tb.bin_log = bin_log;
+    tb.n_raster = prefixes.w[5];
     const SceneArgs sc = scene_args(g);
-    const uint32_t* tile_prefix = reinterpret_cast<const uint32_t*>(tile_ws + L.table + L.views_bytes) + 2u * (1u + bin_log);
-    e = launch_batch_tiles(sc, desc, 1, tile_prefix, tb, 8u, counters, fb, nullptr, stream);
+    const uint32_t* dev_prefix = reinterpret_cast<const uint32_t*>(tile_ws + L.table + L.views_bytes);
+    e = launch_batch_tiles(sc, desc, 1, dev_prefix + 2, dev_prefix + 4, tb, 8u, counters, fb, nullptr, stream);
     if (e != hipSuccess) return hip_fail(e, "tile schedule");
     // instance list overflow: the scatter schedule on a cleared framebuffer, gated on the flag
     e = launch_fill_u64_if(fb, (size_t)cam.width_i * cam.height_i, kEmpty, &counters->overflow, stream);
@@ -731,7 +746,6 @@ int sgs_render_batch(const sgs_gaussians* gaussians, const sgs_view* views, uint
         return fail(SGS_ERROR_INVALID_ARGUMENT, "unknown batch schedule " + std::to_string(tuning->schedule));
     if (tuning && tuning->tile_samples && tuning->tile_samples != 8u && tuning->tile_samples != 16u)
         return fail(SGS_ERROR_INVALID_ARGUMENT, "tile_samples must be 8 or 16");
-    if (t.bin_log >= kBinSizes) return fail(SGS_ERROR_INVALID_ARGUMENT, "bin_size must be 16, 32 or 64");
     const BatchLayout L = batch_layout(views, n_views, gaussians->n_gaussians, t);
     if (!workspace || workspace_bytes < L.total)
         return fail(SGS_ERROR_WORKSPACE_TOO_SMALL,
@@ -754,7 +768,8 @@ int sgs_render_batch(const sgs_gaussians* gaussians, const sgs_view* views, uint
     }
     const ViewDesc* desc = reinterpret_cast<const ViewDesc*>(table);
     const uint32_t* prefix = reinterpret_cast<const uint32_t*>(table + table_views_bytes(n_views));
-    const uint32_t* tile_prefix = prefix + (size_t)(n_views + 1u) * (1u + t.bin_log);
+    const uint32_t* tile_prefix = prefix + n_views + 1u;
+    const uint32_t* raster_prefix = tile_prefix + n_views + 1u;
     BatchCounters* counters = reinterpret_cast<BatchCounters*>(ws + L.counters);
     uint32_t* ready = reinterpret_cast<uint32_t*>(ws + L.ready);
     BigEntry* queue = reinterpret_cast<BigEntry*>(ws + L.queue);
@@ -787,8 +802,8 @@ int sgs_render_batch(const sgs_gaussians* gaussians, const sgs_view* views, uint
         tb.pair_key = reinterpret_cast<uint32_t*>(ws + L.pair_key);
         tb.n_tiles = (uint32_t)L.n_tiles;
         tb.capacity = L.capacity;
-        tb.bin_log = t.bin_log;
-        e = launch_batch_tiles(sc, desc, n_views, tile_prefix, tb, t.tile_samples, counters, fb,
+        tb.n_raster = (uint32_t)L.n_raster;
+        e = launch_batch_tiles(sc, desc, n_views, tile_prefix, raster_prefix, tb, t.tile_samples, counters, fb,
                                stage_events ? static_cast<hipEvent_t>(stage_events[2]) : nullptr, stream);
         if (e != hipSuccess) return hip_fail(e, "batched tiles");
         e = launch_fill_u64_if(fb, L.fb_words, kEmpty, &counters->overflow, stream);

@@ -437,7 +437,7 @@ constexpr uint32_t kNoTiles = 0xffffffffu;
 template <bool kFill>
 __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, const float4* __restrict__ cov3d,
                                                                 const ViewDesc* __restrict__ views,
-                                                                uint32_t n_views, uint32_t bl,
+                                                                uint32_t n_views,
                                                                 uint32_t* __restrict__ tile_count,
                                                                 uint32_t* __restrict__ tile_cursor,
                                                                 const uint32_t* __restrict__ offsets,
@@ -496,7 +496,7 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
             }
             __syncthreads();
             const ViewDesc& vd = views[v];
-            const uint32_t bin0 = vd.tile_begin[bl], gw = vd.grid_w[bl];
+            const uint32_t bin0 = vd.tile_begin, gw = vd.grid_w;
             TileRange tr = { 0u, 0u, 0u, 0u };
             uint32_t key = 0;
             bool has = false;
@@ -511,7 +511,7 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
                 has = r.x != kNoTiles;
                 tr = { r.x & 0xffffu, r.y & 0xffffu, r.x >> 16, r.y >> 16 };
             } else {
-                has = gaussian_tiles_reg(mx, my, mz, cv, valid, vd, kRasterLog2 + bl, tr, key);
+                has = gaussian_tiles_reg(mx, my, mz, cv, valid, vd, kRasterLog2 + vd.bin_log, tr, key);
                 if (g < sc.n) {
                     pair_rect[pair] = has ? make_uint2(tr.tx0 | (tr.ty0 << 16), tr.tx1 | (tr.ty1 << 16))
                                           : make_uint2(kNoTiles, kNoTiles);
@@ -708,6 +708,7 @@ __global__ __launch_bounds__(kRangeThreads) void view_key_range_kernel(const Vie
 // that position on, read where the rasteriser's batches of 256 start; other positions of
 // keys_out are not written.
 constexpr uint32_t kSortBins = 1024;
+constexpr uint32_t kSortInFlight = 16;  // entries per lane in flight in the two-pass path of larger tiles
 constexpr uint32_t kSortRegKeys = 32;  // entries per lane kept in registers for tiles of <= 8192
 
 __device__ __forceinline__ uint32_t sort_bucket(uint32_t key, uint2 vk)
@@ -767,14 +768,14 @@ __global__ __launch_bounds__(kBatchBlock) void tile_sort_kernel(uint32_t n_views
             }
         }
     } else {
-        // four keys in flight per lane
+        // kSortInFlight keys in flight per lane
         uint32_t i = tid;
-        for (; i + 3u * kBatchBlock < n; i += 4u * kBatchBlock) {
-            uint32_t k[4];
+        for (; i + (kSortInFlight - 1u) * kBatchBlock < n; i += kSortInFlight * kBatchBlock) {
+            uint32_t k[kSortInFlight];
 #pragma unroll
-            for (uint32_t u = 0; u < 4u; u++) k[u] = keys_in[begin + i + u * kBatchBlock];
+            for (uint32_t u = 0; u < kSortInFlight; u++) k[u] = keys_in[begin + i + u * kBatchBlock];
 #pragma unroll
-            for (uint32_t u = 0; u < 4u; u++) atomicAdd(&hist[sort_bucket(k[u], vk)], 1u);
+            for (uint32_t u = 0; u < kSortInFlight; u++) atomicAdd(&hist[sort_bucket(k[u], vk)], 1u);
         }
         for (; i < n; i += kBatchBlock) atomicAdd(&hist[sort_bucket(keys_in[begin + i], vk)], 1u);
     }
@@ -822,15 +823,16 @@ __global__ __launch_bounds__(kBatchBlock) void tile_sort_kernel(uint32_t n_views
         return;
     }
     uint32_t i = tid;
-    for (; i + 3u * kBatchBlock < n; i += 4u * kBatchBlock) {
-        uint32_t k[4], g[4];
+    for (; i + (kSortInFlight - 1u) * kBatchBlock < n; i += kSortInFlight * kBatchBlock) {
+        uint32_t k[kSortInFlight], g[kSortInFlight];
 #pragma unroll
-        for (uint32_t u = 0; u < 4u; u++) {
+        for (uint32_t u = 0; u < kSortInFlight; u++) {
             k[u] = keys_in[begin + i + u * kBatchBlock];
             g[u] = inst_in[begin + i + u * kBatchBlock];
         }
 #pragma unroll
-        for (uint32_t u = 0; u < 4u; u++) inst_out[begin + atomicAdd(&hist[sort_bucket(k[u], vk)], 1u)] = g[u];
+        for (uint32_t u = 0; u < kSortInFlight; u++)
+            inst_out[begin + atomicAdd(&hist[sort_bucket(k[u], vk)], 1u)] = g[u];
     }
     for (; i < n; i += kBatchBlock)
         inst_out[begin + atomicAdd(&hist[sort_bucket(keys_in[begin + i], vk)], 1u)] = inst_in[begin + i];
@@ -913,8 +915,8 @@ template <uint32_t kLayers>
 __global__ __launch_bounds__(kBatchBlock, kLayers <= 8u ? 5 : 3) void batch_tile_raster_kernel(SceneArgs sc,
                                                                         const float4* __restrict__ cov3d,
                                                                         const ViewDesc* __restrict__ views,
-                                                                        uint32_t n_views, uint32_t bl,
-                                                                        const uint32_t* __restrict__ tile_prefix,
+                                                                        uint32_t n_views,
+                                                                        const uint32_t* __restrict__ raster_prefix,
                                                                         const uint32_t* __restrict__ offsets,
                                                                         const uint32_t* __restrict__ instances,
                                                                         const uint32_t* __restrict__ keys,
@@ -932,17 +934,19 @@ __global__ __launch_bounds__(kBatchBlock, kLayers <= 8u ? 5 : 3) void batch_tile
     __shared__ uint32_t s_rect[kBatchBlock], s_end[kBatchBlock];
     __shared__ uint32_t s_wave[kBatchBlock / 64u];
     if (ctr->overflow) return;
-    // workgroup -> (bin gt, 16x16 tile q of the bin's 2^bl x 2^bl)
-    const uint32_t gt = blockIdx.x >> (2u * bl), q = blockIdx.x & ((1u << (2u * bl)) - 1u);
+    // workgroup -> view (last v with raster_prefix[v] <= blockIdx.x) -> (bin lt of the view,
+    // 16x16 tile q of the bin's 2^b x 2^b)
     uint32_t lo = 0, hi = n_views - 1u;
     while (lo < hi) {
         const uint32_t mid = (lo + hi + 1u) >> 1;
-        if (tile_prefix[mid] <= gt) lo = mid;
+        if (raster_prefix[mid] <= blockIdx.x) lo = mid;
         else hi = mid - 1u;
     }
     const ViewDesc& vd = views[lo];
-    const uint32_t lt = gt - tile_prefix[lo];
-    const uint32_t gw = vd.grid_w[bl];
+    const uint32_t bl = vd.bin_log, r = blockIdx.x - raster_prefix[lo];
+    const uint32_t lt = r >> (2u * bl), q = r & ((1u << (2u * bl)) - 1u);
+    const uint32_t gt = vd.tile_begin + lt;
+    const uint32_t gw = vd.grid_w;
     const uint32_t ox = ((lt % gw) << (kRasterLog2 + bl)) + (q & ((1u << bl) - 1u)) * kTile;
     const uint32_t oy = ((lt / gw) << (kRasterLog2 + bl)) + (q >> bl) * kTile;
     const uint32_t W = vd.cam.width_i, H = vd.cam.height_i;
@@ -1153,7 +1157,7 @@ __global__ void single_view_table_kernel(ViewDesc d, uint32_t views_bytes, Singl
     d.cam_z = cam_position[2];
     *reinterpret_cast<ViewDesc*>(table) = d;
     uint32_t* prefix = reinterpret_cast<uint32_t*>(table + views_bytes);
-    for (uint32_t i = 0; i < 2u * (1u + kBinSizes); i++) prefix[i] = prefixes.w[i];
+    for (uint32_t i = 0; i < 6u; i++) prefix[i] = prefixes.w[i];
 }
 
 hipError_t launch_single_view_table(const ViewDesc& desc, size_t views_bytes, const SingleViewPrefixes& prefixes,
@@ -1165,11 +1169,10 @@ hipError_t launch_single_view_table(const ViewDesc& desc, size_t views_bytes, co
 }
 
 hipError_t launch_batch_tiles(const SceneArgs& sc, const ViewDesc* views, uint32_t n_views, const uint32_t* tile_prefix,
-                              const TileBuffers& tb, uint32_t layers, BatchCounters* counters, uint64_t* fb,
-                              hipEvent_t raster_event, hipStream_t stream)
+                              const uint32_t* raster_prefix, const TileBuffers& tb, uint32_t layers,
+                              BatchCounters* counters, uint64_t* fb, hipEvent_t raster_event, hipStream_t stream)
 {
     if (n_views == 0 || tb.n_tiles == 0) return hipSuccess;
-    if (tb.bin_log >= kBinSizes) return hipErrorInvalidValue;
     const uint32_t n_chunks = (sc.n + kBatchBlock - 1u) / kBatchBlock;
     const uint64_t n_items64 = (uint64_t)n_chunks * n_views;
     if (n_items64 >= 0xffffffffull) return hipErrorInvalidValue;
@@ -1190,7 +1193,7 @@ hipError_t launch_batch_tiles(const SceneArgs& sc, const ViewDesc* views, uint32
                            n_chunks, tb.view_keys);
         hipLaunchKernelGGL(cov3d_kernel, dim3(n_chunks), dim3(kBatchBlock), 0, stream, sc, tb.cov3d);
         hipLaunchKernelGGL((batch_bin_kernel<false>), dim3((unsigned)grid), dim3(kBatchBlock), 0, stream, sc, tb.cov3d,
-                           views, n_views, tb.bin_log, tb.tile_count, tb.tile_cursor, tb.offsets,
+                           views, n_views, tb.tile_count, tb.tile_cursor, tb.offsets,
                            tb.instances, tb.keys, tb.pair_rect, tb.pair_key, counters);
     }
     hipLaunchKernelGGL((exclusive_scan_kernel<CountInput>), dim3(1), dim3(kScanThreads), 0, stream,
@@ -1198,7 +1201,7 @@ hipError_t launch_batch_tiles(const SceneArgs& sc, const ViewDesc* views, uint32
                        &counters->overflow, tb.capacity, &counters->total_instances);
     if (sc.n > 0) {
         hipLaunchKernelGGL((batch_bin_kernel<true>), dim3((unsigned)grid), dim3(kBatchBlock), 0, stream, sc, tb.cov3d,
-                           views, n_views, tb.bin_log, tb.tile_count, tb.tile_cursor, tb.offsets,
+                           views, n_views, tb.tile_count, tb.tile_cursor, tb.offsets,
                            tb.instances, tb.keys, tb.pair_rect, tb.pair_key, counters);
     }
     hipLaunchKernelGGL(tile_sort_kernel, dim3(tb.n_tiles), dim3(kBatchBlock), 0, stream, n_views, tile_prefix,
@@ -1207,16 +1210,17 @@ hipError_t launch_batch_tiles(const SceneArgs& sc, const ViewDesc* views, uint32
         e = hipEventRecord(raster_event, stream);
         if (e != hipSuccess) return e;
     }
-    // one workgroup per 16x16 tile: 4^bin_log per bin (the caller keeps the total below 2^31)
-    const dim3 raster_grid(tb.n_tiles << (2u * tb.bin_log));
+    // one workgroup per 16x16 tile, 4^b per bin of (16 << b) pixels (the caller keeps the total
+    // below 2^31; it equals raster_prefix[n_views])
+    const dim3 raster_grid(tb.n_raster);
     if (layers == 8u)
         hipLaunchKernelGGL(batch_tile_raster_kernel<8>, raster_grid, dim3(kBatchBlock), 0, stream, sc, tb.cov3d,
-                           views, n_views, tb.bin_log, tile_prefix, tb.offsets, tb.sorted_instances, tb.sorted_keys,
-                           counters, fb);
+                           views, n_views, raster_prefix, tb.offsets, tb.sorted_instances, tb.sorted_keys, counters,
+                           fb);
     else
         hipLaunchKernelGGL(batch_tile_raster_kernel<16>, raster_grid, dim3(kBatchBlock), 0, stream, sc, tb.cov3d,
-                           views, n_views, tb.bin_log, tile_prefix, tb.offsets, tb.sorted_instances, tb.sorted_keys,
-                           counters, fb);
+                           views, n_views, raster_prefix, tb.offsets, tb.sorted_instances, tb.sorted_keys, counters,
+                           fb);
     return hipGetLastError();
 }
 

@@ -10,13 +10,15 @@
 
 namespace sgs {
 
-// Tile schedule geometry: instances are binned per framebuffer bin of (16 << b) x (16 << b)
-// pixels, b = 0 .. kBinSizes - 1 chosen per launch; the rasteriser keeps 16x16-pixel tiles in
-// LDS, 4^b of them per bin, each reading its bin's depth-sorted list. The view table holds
-// every view's bin grid for each b.
+// Tile schedule geometry: each view's instances are binned per framebuffer bin of
+// (16 << b) x (16 << b) pixels, b = 0 .. kBinSizes - 1 chosen per view; the rasteriser keeps
+// 16x16-pixel tiles in LDS, 4^b of them per bin, each reading its bin's depth-sorted list.
 constexpr uint32_t kRasterPixels = 16;
 constexpr uint32_t kRasterLog2 = 4;
 constexpr uint32_t kBinSizes = 3;
+// default bin of a view: 16 pixels for framebuffers of at most this many pixels (few bins
+// with long lists otherwise, each read by four tiles), 32 pixels above
+constexpr uint64_t kSmallBinMaxPixels = 1ull << 19;
 // bin_size (16, 32 or 64 pixels) -> b; kBinSizes for any other value
 inline uint32_t bin_log2(uint32_t bin_size)
 {
@@ -111,9 +113,10 @@ struct alignas(16) ViewDesc {
     uint32_t frame0;            // RNG frame of sample 0; sample s uses frame0 + s
     int32_t mode;               // temporal mode of sample 0; later samples are STATIC
     uint32_t block_begin;       // first resolve block of this view
-    uint32_t tile_begin[kBinSizes];  // first bin of this view in the batch's bin numbering, per bin size
-    uint32_t grid_w[kBinSizes];      // bins per row of this view's framebuffer, per bin size
-    uint32_t grid_h[kBinSizes];      // bins per column
+    uint32_t tile_begin;        // first bin of this view in the batch's bin numbering
+    uint32_t grid_w, grid_h;    // bins per row / column of this view's framebuffer
+    uint32_t bin_log;           // bins of (16 << bin_log) pixels
+    uint32_t raster_begin;      // first rasteriser workgroup (16x16 tile) of this view
     uint32_t level;             // resolve order: 1 + the level of the view whose output is this view's history
     uint64_t fb_offset;         // first framebuffer word of sample 0 (samples are P words apart)
     uint64_t samples_offset;    // REPROJECT views: first float4 of the sample-0 buffer
@@ -150,7 +153,6 @@ struct BatchTuning {
     uint32_t schedule;          // 0: LDS tiles, 1: scatter with global atomics
     uint32_t max_tile_instances;
     uint32_t tile_samples;      // samples per LDS pass of the tile rasteriser (8 or 16)
-    uint32_t bin_log;           // tile schedule bins of (16 << bin_log) pixels
     uint32_t sequential_threshold;
     uint32_t big_threshold;     // fragments above which a Gaussian goes to the queue
     uint32_t entry_fragments;   // fragments per queue entry
@@ -204,19 +206,20 @@ struct TileBuffers {
     uint2* pair_rect;           // [n_views * n] tile rectangle of each (view, Gaussian) pair
     uint32_t* pair_key;         // [n_views * n] its depth key
     uint2* view_keys;           // [n_views] depth-key bucketing of each view (kmin, shift)
-    uint32_t n_tiles, capacity;  // bins of the batch at this bin size; instance capacity
-    uint32_t bin_log;            // bins of (16 << bin_log) pixels
+    uint32_t n_tiles, capacity;  // bins of the batch; instance capacity
+    uint32_t n_raster;           // rasteriser workgroups: 16x16 tiles of every view's bins
 };
+// tile_prefix / raster_prefix: first bin / rasteriser workgroup of each view, n_views + 1 each
 hipError_t launch_batch_tiles(const SceneArgs& sc, const ViewDesc* views, uint32_t n_views, const uint32_t* tile_prefix,
-                              const TileBuffers& tb, uint32_t layers, BatchCounters* counters, uint64_t* fb,
-                              hipEvent_t raster_event, hipStream_t stream);
+                              const uint32_t* raster_prefix, const TileBuffers& tb, uint32_t layers,
+                              BatchCounters* counters, uint64_t* fb, hipEvent_t raster_event, hipStream_t stream);
 hipError_t launch_fill_u64_if(uint64_t* p, size_t n, uint64_t value, const uint32_t* run_if_set, hipStream_t stream);
 // device view table of a single view whose world-to-view matrix and camera position are in
 // device memory: `desc` with its rows and position taken from w2c / cam_position, then the
-// prefixes (resolve blocks, then bins per bin size: {0, count} each), the table layout of
-// the batched renderer for one view, whose ViewDesc part is views_bytes long
+// prefixes (resolve blocks, bins, rasteriser workgroups: {0, count} each), the table layout
+// of the batched renderer for one view, whose ViewDesc part is views_bytes long
 struct SingleViewPrefixes {
-    uint32_t w[2u * (1u + kBinSizes)];
+    uint32_t w[6];
 };
 hipError_t launch_single_view_table(const ViewDesc& desc, size_t views_bytes, const SingleViewPrefixes& prefixes,
                                     const float* w2c, const float* cam_position, void* table, hipStream_t stream);

@@ -157,14 +157,18 @@ def test_batch_workspace_and_schedule_validation(native):
     g = _gaussians(native)
     rc = lib.sgs_render_batch(C.byref(g), views, 2, None, C.byref(bad), 0x100000, 1 << 40, None, None)
     assert rc == -1 and b"schedule" in lib.sgs_last_error()
-    bad_bin = native.sgs_batch_tuning(bin_size=8)
-    rc = lib.sgs_render_batch(C.byref(g), views, 2, None, C.byref(bad_bin), 0x100000, 1 << 40, None, None)
+    # a view's bin size is 0 (library choice), 16, 32 or 64 pixels
+    views[1].bin_size = 8
+    rc = lib.sgs_render_batch(C.byref(g), views, 2, None, None, 0x100000, 1 << 40, None, None)
     assert rc == -1 and b"bin_size" in lib.sgs_last_error()
     # the tile counters scale with the bin count: 16-pixel bins need more than 64-pixel ones
-    fine = native.sgs_batch_tuning(bin_size=16, max_tile_instances=1 << 16)
-    coarse = native.sgs_batch_tuning(bin_size=64, max_tile_instances=1 << 16)
-    assert lib.sgs_render_batch_workspace_size(views, 2, 1000, C.byref(fine)) > lib.sgs_render_batch_workspace_size(
-        views, 2, 1000, C.byref(coarse))
+    cap = native.sgs_batch_tuning(max_tile_instances=1 << 16)
+    sizes = {}
+    for b in (16, 64, 0):
+        views[0].bin_size = views[1].bin_size = b
+        sizes[b] = lib.sgs_render_batch_workspace_size(views, 2, 1000, C.byref(cap))
+    assert sizes[16] > sizes[64]
+    assert sizes[0] == sizes[16]  # frames of at most 2^19 pixels take 16-pixel bins by default
     host = (C.c_uint8 * lib.sgs_batch_view_table_bytes(2))()
     assert lib.sgs_batch_encode_views(views, 2, host) == 0
 

@@ -75,8 +75,9 @@ def make_views():
     dict(),                                                   # defaults (tile schedule)
     dict(max_tile_instances=16),                              # tile list overflow: scatter fallback
     dict(tile_samples=8),                                     # 8 samples per LDS pass
-    dict(bin_size=16),                                        # one raster tile per bin
-    dict(bin_size=64, tile_samples=16),                       # 4x4 raster tiles per bin
+    dict(bins=[16, 16, 16, 16, 16]),                          # one raster tile per bin
+    dict(bins=[32, 64, 16, 64, 32], tile_samples=16),         # bin sizes mixed in one launch
+    dict(bins=[64, 64, 64, 64, 64]),                          # 4x4 raster tiles per bin
     dict(schedule=2),                                         # scatter schedule
     dict(schedule=2, big_threshold=64, entry_fragments=64),   # most Gaussians through the work queue
     dict(schedule=2, big_threshold=64, entry_fragments=64, queue_capacity=3),  # queue overflow: in place
@@ -88,8 +89,10 @@ def test_batch_equals_plain_spp_loop(gpu, tuning):
     arrays = small_scene(3000, seed=41, log_scale_mean=-3.3, log_scale_std=0.8)
     t = to_device(arrays)
     cams, spps, frames = make_views()
-    views = [BatchView(settings_of(c), spp=n, frame_idx=f) for c, n, f in zip(cams, spps, frames)]
-    print_case(test="batch_vs_plain", **tuning)
+    tuning = dict(tuning)
+    bins = tuning.pop("bins", [0] * len(cams))
+    views = [BatchView(settings_of(c), spp=n, frame_idx=f, bin_size=b) for c, n, f, b in zip(cams, spps, frames, bins)]
+    print_case(test="batch_vs_plain", bins=bins, **tuning)
     r = BatchRenderer(t["means"], t["scales"], t["rotations"], t["opacities"], t["sh"], views, BatchTuning(**tuning))
     outs = r.render()
     torch.cuda.synchronize()
@@ -114,11 +117,11 @@ def test_batch_tile_schedule_edges(gpu, case):
         cams = [small_camera(640, 520, angle=0.2), small_camera(300, 200, angle=2.5, supersampling_factor=3)]
         spps, frames = [2, 1], [50, 60]
     t = to_device(arrays)
-    views = [BatchView(settings_of(c), spp=n, frame_idx=f) for c, n, f in zip(cams, spps, frames)]
+    bin_size = 16 if case.endswith("_16") else 32 if case == "wide_views" else 0
+    views = [BatchView(settings_of(c), spp=n, frame_idx=f, bin_size=bin_size) for c, n, f in zip(cams, spps, frames)]
     print_case(test="batch_tile_edges", case=case)
     tiles = BatchRenderer(t["means"], t["scales"], t["rotations"], t["opacities"], t["sh"], views,
-                          BatchTuning(tile_samples=8 if case.endswith("_8") else 0,
-                                      bin_size=16 if case.endswith("_16") else 0))
+                          BatchTuning(tile_samples=8 if case.endswith("_8") else 0))
     scatter = BatchRenderer(t["means"], t["scales"], t["rotations"], t["opacities"], t["sh"], views,
                             BatchTuning(schedule=2))
     tiles.render()
