@@ -517,7 +517,8 @@ hipError_t launch_shade_resolve(const SceneArgs& sc, const float* cam_position, 
 // batched resolve: every output pixel of every view folds its view's spp samples in order,
 // sample 0 with the view's temporal mode and the rest STATIC, the operations of spp
 // successive single-sample calls (the history of sample s is the output of sample s - 1).
-// A pixel's winners often repeat across samples, so the last shaded winner is reused.
+// A pixel's winners often repeat across samples, so its six most recently shaded winners are
+// kept with their colours (the colour of a winner depends only on the view and the Gaussian).
 // ------------------------------------------------------------------------------------------
 namespace sgs {
 
@@ -553,8 +554,15 @@ __global__ __launch_bounds__(kBlockResolve) void batch_resolve_kernel(SceneArgs 
     const uint32_t ss = (uint32_t)vd.cam.ss;
     const uint32_t bx = (pix % vd.width) * ss, by = (pix / vd.width) * ss;
     const float nf = 1.0f / (float)(ss * ss);
-    uint32_t last_g = 0xffffffffu;
-    float3 last_col = make_float3(0.0f, 0.0f, 0.0f);
+    // the kCached most recent distinct winners of the pixel stay shaded (most recent first)
+    constexpr uint32_t kCached = 6;
+    uint32_t cg[kCached];
+    float3 cc[kCached];
+#pragma unroll
+    for (uint32_t i = 0; i < kCached; i++) {
+        cg[i] = 0xffffffffu;
+        cc[i] = make_float3(0.0f, 0.0f, 0.0f);
+    }
     for (uint32_t s = s0; s < vd.spp; s++) {
         SpatialSums sp = { 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0u };
         for (uint32_t oy = 0; oy < ss; oy++) {
@@ -564,11 +572,27 @@ __global__ __launch_bounds__(kBlockResolve) void batch_resolve_kernel(SceneArgs 
                 float4 sm = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 if (dk != 0xffffffffu) {
                     const uint32_t g = (uint32_t)data;
-                    if (g != last_g) {
-                        last_col = shade<B>(sc, vd.cam_x, vd.cam_y, vd.cam_z, g);
-                        last_g = g;
+                    if (g != cg[0]) {
+                        uint32_t hit = kCached;
+#pragma unroll
+                        for (uint32_t i = 1; i < kCached; i++)
+                            if (cg[i] == g) hit = i;
+                        float3 c = make_float3(0.0f, 0.0f, 0.0f);
+#pragma unroll
+                        for (uint32_t i = 1; i < kCached; i++)
+                            if (hit == i) c = cc[i];
+                        if (hit == kCached) c = shade<B>(sc, vd.cam_x, vd.cam_y, vd.cam_z, g);
+                        // move to front: entries before the hit (or all, on a miss) shift back
+#pragma unroll
+                        for (uint32_t i = kCached - 1u; i > 0; i--)
+                            if (i <= hit) {
+                                cg[i] = cg[i - 1u];
+                                cc[i] = cc[i - 1u];
+                            }
+                        cg[0] = g;
+                        cc[0] = c;
                     }
-                    sm = make_float4(last_col.x, last_col.y, last_col.z, u2f(dk));
+                    sm = make_float4(cc[0].x, cc[0].y, cc[0].z, u2f(dk));
                 }
                 add_subpixel(sp, sm, sm.x, sm.y, sm.z, sm.w, sm.w > 0.0f ? 1.0f : 0.0f, 1.0f);
             }
