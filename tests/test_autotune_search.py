@@ -33,3 +33,27 @@ def test_coupled_cost_needs_second_pass(search):
 def test_defaults_are_candidates(search):
     best, ms, trials = search(lambda c: 1.0, {"x": [5, 6]}, {"x": 0})
     assert best == {"x": 0} and trials[0][0] == {"x": 0}
+
+
+def test_views_with_bins_threshold():
+    # 16-pixel bins for framebuffers (W ss x H ss) up to the threshold, 32 above; -1 keeps
+    # each view's own bin size (by default the library's choice)
+    import torch
+    from StochasticGSCudaBackend.autotune import views_with_bins
+    from StochasticGSCudaBackend.batch import BatchView
+    from StochasticGSCudaBackend.render import Settings
+
+    def view(w, h, ss=1, bin_size=0):
+        s = Settings(torch.eye(4), torch.zeros(3), w, h, 100.0, 100.0, w / 2, h / 2, 0.2, 100.0, False, ss, False,
+                     False, False)
+        return BatchView(s, spp=2, frame_idx=7, bin_size=bin_size)
+
+    views = [view(256, 256), view(640, 480), view(640, 480, ss=2), view(1920, 1080, bin_size=64)]
+    assert [v.bin_size for v in views_with_bins(views, -1)] == [0, 0, 0, 64]
+    assert [v.bin_size for v in views_with_bins(views, 1 << 19)] == [16, 16, 32, 32]
+    assert [v.bin_size for v in views_with_bins(views, 0)] == [32, 32, 32, 32]
+    out = views_with_bins(views, 1 << 40)
+    assert [v.bin_size for v in out] == [16, 16, 16, 16]
+    # the other fields are kept and the input views are not modified
+    assert all(o.spp == 2 and o.frame_idx == 7 and o.settings is v.settings for o, v in zip(out, views))
+    assert [v.bin_size for v in views] == [0, 0, 0, 64]
