@@ -96,8 +96,9 @@ typedef struct sgs_tuning {
                                          does change results when it binds. */
     uint32_t schedule;                /* stochastic transparency without hybrid scheduling: 0 or
                                          SGS_BATCH_TILES (1) bins the view into framebuffer bins
-                                         and rasterises their 16x16 tiles in LDS, the batched renderer's tile
-                                         schedule for one view and one sample; SGS_BATCH_SCATTER
+                                         and rasterises their 16x16 tiles in LDS, the batched
+                                         renderer's tile schedule for one view and one sample;
+                                         SGS_BATCH_SCATTER
                                          (2) scatters with global atomicMin as the reference
                                          does (accumulate_scatter.cu:19-77). Calls with blue
                                          noise or debug counters always scatter. Identical
@@ -203,10 +204,10 @@ typedef struct sgs_batch_tuning {
 } sgs_batch_tuning;
 
 /* SGS_BATCH_TILES: every (view, Gaussian) is binned into the framebuffer bins (the view's
- * bin_size) its bounding box overlaps and each bin's list is depth-sorted; one workgroup per 16x16 tile of
- * a bin keeps the tile's words of up to 16 samples in LDS and writes them out once. SGS_BATCH_SCATTER: one persistent launch over
- * (Gaussian chunk, view) items with 64-bit atomicMin into the framebuffer in device memory
- * and a work queue for large Gaussians. */
+ * bin_size) its bounding box overlaps and each bin's list is depth-sorted; one workgroup per
+ * 16x16 tile of a bin keeps the tile's words of up to 16 samples in LDS and writes them out
+ * once. SGS_BATCH_SCATTER: one persistent launch over (Gaussian chunk, view) items with 64-bit
+ * atomicMin into the framebuffer in device memory and a work queue for large Gaussians. */
 #define SGS_BATCH_TILES 1u
 #define SGS_BATCH_SCATTER 2u
 

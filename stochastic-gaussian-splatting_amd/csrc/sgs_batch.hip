@@ -331,9 +331,10 @@ hipError_t launch_batch_accumulate(const SceneArgs& sc, const ViewDesc* views, u
 }
 
 // ============================================================================================
-// Tile schedule: bin every (view, Gaussian) into the 16x16 tiles its AABB overlaps, then
-// rasterise each (view, tile) in one workgroup that keeps the tile's framebuffer words of up
-// to 16 samples in LDS. Each fragment's alpha is evaluated once; a sample is hashed only if
+// Tile schedule: bin every (view, Gaussian) into the view's framebuffer bins (16, 32 or 64
+// pixels) its AABB overlaps, sort each bin's list by depth bucket, then rasterise each
+// 16x16 tile of every bin in one workgroup that keeps the tile's framebuffer words of up to 16
+// samples in LDS. Each fragment's alpha is evaluated once; a sample is hashed only if
 // the fragment is closer than that sample's current LDS word, and accepted fragments land with
 // ds_min_u64; the tile is written out once with plain stores. No framebuffer clear is needed
 // because every tile writes all of its pixels. The tested pixels, alpha and thresholds are
