@@ -98,9 +98,8 @@ typedef struct sgs_tuning {
                                          SGS_BATCH_TILES (1) bins the view into framebuffer bins
                                          and rasterises their 16x16 tiles in LDS, the batched
                                          renderer's tile schedule for one view and one sample;
-                                         SGS_BATCH_SCATTER
-                                         (2) scatters with global atomicMin as the reference
-                                         does (accumulate_scatter.cu:19-77). Calls with blue
+                                         SGS_BATCH_SCATTER (2) scatters with global atomicMin as
+                                         the reference does (accumulate_scatter.cu:19-77). Calls with blue
                                          noise or debug counters always scatter. Identical
                                          framebuffers either way. */
     uint32_t bin_size;                /* tile schedule: edge in pixels of the framebuffer bins
