@@ -696,6 +696,8 @@ int accumulate_tiles_single(const sgs_gaussians* g, const sgs_settings* s, const
     tb.n_tiles = L.n_tiles;
     tb.capacity = L.capacity;
     tb.n_raster = prefixes.w[5];
+    tb.has_single = true;
+    tb.has_multi = false;
     const SceneArgs sc = scene_args(g);
     const uint32_t* dev_prefix = reinterpret_cast<const uint32_t*>(tile_ws + L.table + L.views_bytes);
     e = launch_batch_tiles(sc, desc, 1, dev_prefix + 2, dev_prefix + 4, tb, 8u, counters, fb, nullptr, stream);
@@ -803,6 +805,8 @@ int sgs_render_batch(const sgs_gaussians* gaussians, const sgs_view* views, uint
         tb.n_tiles = (uint32_t)L.n_tiles;
         tb.capacity = L.capacity;
         tb.n_raster = (uint32_t)L.n_raster;
+        tb.has_single = tb.has_multi = false;
+        for (uint32_t v = 0; v < n_views; v++) (views[v].spp == 1u ? tb.has_single : tb.has_multi) = true;
         e = launch_batch_tiles(sc, desc, n_views, tile_prefix, raster_prefix, tb, t.tile_samples, counters, fb,
                                stage_events ? static_cast<hipEvent_t>(stage_events[2]) : nullptr, stream);
         if (e != hipSuccess) return hip_fail(e, "batched tiles");

@@ -866,7 +866,8 @@ __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t x, int m)
 constexpr uint32_t kSub = 4, kSubtiles = (kTile / kSub) * (kTile / kSub);
 // One queued fragment (instance j << 8 | tile pixel l): alpha once, then every sample word of
 // the pixel the fragment still lies in front of gets its hash test and, on acceptance, the
-// LDS atomicMin. All kLayers words are read (rows at and beyond G are masked off).
+// LDS atomicMin. All kLayers words are read (rows at and beyond G are masked off), in groups of
+// up to four.
 template <uint32_t kLayers>
 __device__ __forceinline__ void tile_fragment(uint32_t entry, unsigned long long (*layer)[kTile * kTile],
                                               const uint64_t* s_frag, const float* s_msx, const float* s_msy,
@@ -885,15 +886,16 @@ __device__ __forceinline__ void tile_fragment(uint32_t entry, unsigned long long
     // only the depth halves of the words are read: a sample is live if its word's depth is not
     // in front of the fragment (equal depths are resolved by the atomicMin), and words only
     // decrease, so a stale read never drops a winner
+    constexpr uint32_t kGroup = kLayers < 4u ? kLayers : 4u;
 #pragma unroll
-    for (uint32_t s0 = 0; s0 < kLayers; s0 += 4u) {
-        uint32_t w[4];
+    for (uint32_t s0 = 0; s0 < kLayers; s0 += kGroup) {
+        uint32_t w[kGroup];
 #pragma unroll
-        for (uint32_t i = 0; i < 4u; i++)
+        for (uint32_t i = 0; i < kGroup; i++)
             w[i] = __hip_atomic_load(reinterpret_cast<uint32_t*>(&layer[s0 + i][l]) + 1, __ATOMIC_RELAXED,
                                      __HIP_MEMORY_SCOPE_WORKGROUP);
 #pragma unroll
-        for (uint32_t i = 0; i < 4u; i++) live |= fd <= w[i] ? 1u << (s0 + i) : 0u;
+        for (uint32_t i = 0; i < kGroup; i++) live |= fd <= w[i] ? 1u << (s0 + i) : 0u;
     }
     live &= (1u << G) - 1u;
     if (!live) return;
@@ -911,7 +913,8 @@ __device__ __forceinline__ void tile_fragment(uint32_t entry, unsigned long long
     }
 }
 
-// kLayers: samples held in LDS per pass over the tile's instances
+// kLayers: samples held in LDS per pass over the tile's instances (1 for views of one sample,
+// whose LDS then holds one framebuffer word per pixel)
 template <uint32_t kLayers>
 __global__ __launch_bounds__(kBatchBlock, kLayers <= 8u ? 5 : 3) void batch_tile_raster_kernel(SceneArgs sc,
                                                                         const float4* __restrict__ cov3d,
@@ -944,6 +947,8 @@ __global__ __launch_bounds__(kBatchBlock, kLayers <= 8u ? 5 : 3) void batch_tile
         else hi = mid - 1u;
     }
     const ViewDesc& vd = views[lo];
+    // one-sample views are rasterised by the one-layer kernel, the others by the layered one
+    if ((kLayers == 1u) != (vd.spp == 1u)) return;
     const uint32_t bl = vd.bin_log, r = blockIdx.x - raster_prefix[lo];
     const uint32_t lt = r >> (2u * bl), q = r & ((1u << (2u * bl)) - 1u);
     const uint32_t gt = vd.tile_begin + lt;
@@ -1214,11 +1219,19 @@ hipError_t launch_batch_tiles(const SceneArgs& sc, const ViewDesc* views, uint32
     // one workgroup per 16x16 tile, 4^b per bin of (16 << b) pixels (the caller keeps the total
     // below 2^31; it equals raster_prefix[n_views])
     const dim3 raster_grid(tb.n_raster);
-    if (layers == 8u)
+    // Views of one sample take the one-layer kernel (less LDS per workgroup), the others the
+    // kernel with `layers` samples per pass. Both launches span every view's tiles and return at
+    // once on the other's (measured faster than two launches over class-ordered tiles, whose
+    // sequential tails cost more than the early returns).
+    if (tb.has_single)
+        hipLaunchKernelGGL(batch_tile_raster_kernel<1>, raster_grid, dim3(kBatchBlock), 0, stream, sc, tb.cov3d,
+                           views, n_views, raster_prefix, tb.offsets, tb.sorted_instances, tb.sorted_keys, counters,
+                           fb);
+    if (tb.has_multi && layers == 8u)
         hipLaunchKernelGGL(batch_tile_raster_kernel<8>, raster_grid, dim3(kBatchBlock), 0, stream, sc, tb.cov3d,
                            views, n_views, raster_prefix, tb.offsets, tb.sorted_instances, tb.sorted_keys, counters,
                            fb);
-    else
+    else if (tb.has_multi)
         hipLaunchKernelGGL(batch_tile_raster_kernel<16>, raster_grid, dim3(kBatchBlock), 0, stream, sc, tb.cov3d,
                            views, n_views, raster_prefix, tb.offsets, tb.sorted_instances, tb.sorted_keys, counters,
                            fb);

@@ -208,6 +208,7 @@ struct TileBuffers {
     uint2* view_keys;           // [n_views] depth-key bucketing of each view (kmin, shift)
     uint32_t n_tiles, capacity;  // bins of the batch; instance capacity
     uint32_t n_raster;           // rasteriser workgroups: 16x16 tiles of every view's bins
+    bool has_single, has_multi;  // views of one sample / of several samples in the batch
 };
 // tile_prefix / raster_prefix: first bin / rasteriser workgroup of each view, n_views + 1 each
 hipError_t launch_batch_tiles(const SceneArgs& sc, const ViewDesc* views, uint32_t n_views, const uint32_t* tile_prefix,
