@@ -234,12 +234,13 @@ def roofline_render_samples(scene_np, views, scene, step_ms, work, atomics_per_s
 # HBM bytes per launch of the tile rasteriser on the default workload (6M Gaussians, 64 views,
 # 8 samples per LDS pass), from rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0 (calibrated on a
 # 1 GiB device copy); the commands and raw counters are in profiles/tile_raster_c4_6M_64views.md.
-PROFILED_RASTER_TRAFFIC = {"bytes": 10.92e9, "profile": "profiles/tile_raster_c4_6M_64views.md"}
+PROFILED_RASTER_TRAFFIC = {"bytes": 10.98e9, "profile": "profiles/tile_raster_c4_6M_64views.md"}
 
 
 def roofline_tiles(views, raster_ms: list, n_instances: int, visible_pairs: int, default_workload: bool):
-    """Algorithmic bytes of one tile-rasteriser launch: 44 B of Gaussian parameters per
-    visible (view, Gaussian) pair (means 12 B + the cached covariance and opacity 32 B, read
+    """Algorithmic bytes of the tile-rasteriser stage (its one-layer launch for the views of one
+    sample and its 8-layer launch for the others, timed together): 44 B of Gaussian parameters
+    per visible (view, Gaussian) pair (means 12 B + the cached covariance and opacity 32 B, read
     at least once), 8 B per (view, tile, Gaussian) list entry (index + depth bucket) and 8 B
     per framebuffer word written (every sample of every pixel, once)."""
     words = sum(cam["width"] * cam["height"] * cam["spp"] for cam in views)
@@ -247,7 +248,7 @@ def roofline_tiles(views, raster_ms: list, n_instances: int, visible_pairs: int,
     t = sum(raster_ms) / len(raster_ms) * 1e-3
     achieved = b / t / 1e9
     return {
-        "kernel": "sgs::batch_tile_raster_kernel",
+        "kernel": "sgs::batch_tile_raster_kernel<1> + <8> (one stage)",
         "bound": "hbm",
         "achieved": round(achieved, 2),
         "peak": HBM_PEAK_GBS,
