@@ -90,8 +90,8 @@ namespace {
 
 // ---- single-view tile schedule (the batched renderer's tile pipeline for one view) ----
 struct SingleTileLayout {
-    size_t counters, ready, tile_count, tile_cursor, cleared_end, offsets, instances, keys, sorted_instances,
-        sorted_keys, centroids, view_keys, cov3d, pair_rect, pair_key, table, queue, total;
+    size_t counters, ready, tile_count, tile_cursor, cleared_end, offsets, entries, sorted_instances, sorted_keys,
+        centroids, view_keys, cov3d, pair_rect, pair_key, table, queue, total;
     size_t views_bytes;
     uint32_t n_tiles, capacity, queue_capacity;
 };
@@ -127,8 +127,7 @@ SingleTileLayout single_tile_layout(const FbCamera& cam, uint32_t n_gaussians, u
     L.cleared_end = a.offset;
     a.offset = (a.offset + 255u) & ~size_t(255u);
     L.offsets = a.take(4u * ((size_t)L.n_tiles + 1u));
-    L.instances = a.take(4u * (size_t)L.capacity);
-    L.keys = a.take(4u * (size_t)L.capacity);
+    L.entries = a.take(8u * (size_t)L.capacity);  // (depth key, Gaussian index) per instance
     L.sorted_instances = a.take(4u * (size_t)L.capacity);
     L.sorted_keys = a.take(4u * (size_t)L.capacity);
     L.centroids = a.take(16u * (((size_t)n_gaussians + 255u) / 256u));
@@ -465,7 +464,7 @@ uint64_t view_raster_groups(const sgs_view& v)
 // Workspace of a batch. The counters, the queue's ready flags and the tile counts and
 // cursors are contiguous so one fill clears them all.
 struct BatchLayout {
-    size_t counters, ready, tile_count, tile_cursor, cleared_end, offsets, instances, keys, sorted_instances, sorted_keys,
+    size_t counters, ready, tile_count, tile_cursor, cleared_end, offsets, entries, sorted_instances, sorted_keys,
         centroids, cov3d, pair_rect, pair_key, view_keys, queue, table, fb, samples, total;
     uint64_t fb_words, n_samples, n_tiles, n_raster;
     uint32_t n_blocks, capacity;
@@ -502,8 +501,7 @@ BatchLayout batch_layout(const sgs_view* views, uint32_t n, uint32_t n_gaussians
     L.cleared_end = a.offset;
     a.offset = (a.offset + 255u) & ~size_t(255u);
     L.offsets = a.take(4u * (tiles_n + 1u));
-    L.instances = a.take(4u * (size_t)L.capacity);
-    L.keys = a.take(4u * (size_t)L.capacity);
+    L.entries = a.take(8u * (size_t)L.capacity);  // (depth key, Gaussian index) per instance
     L.sorted_instances = a.take(4u * (size_t)L.capacity);
     L.sorted_keys = a.take(4u * (size_t)L.capacity);
     const size_t n_chunks = tiles ? ((size_t)n_gaussians + 255u) / 256u : 0u;
@@ -684,8 +682,7 @@ int accumulate_tiles_single(const sgs_gaussians* g, const sgs_settings* s, const
     tb.tile_count = reinterpret_cast<uint32_t*>(tile_ws + L.tile_count);
     tb.tile_cursor = reinterpret_cast<uint32_t*>(tile_ws + L.tile_cursor);
     tb.offsets = reinterpret_cast<uint32_t*>(tile_ws + L.offsets);
-    tb.instances = reinterpret_cast<uint32_t*>(tile_ws + L.instances);
-    tb.keys = reinterpret_cast<uint32_t*>(tile_ws + L.keys);
+    tb.entries = reinterpret_cast<uint2*>(tile_ws + L.entries);
     tb.sorted_instances = reinterpret_cast<uint32_t*>(tile_ws + L.sorted_instances);
     tb.sorted_keys = reinterpret_cast<uint32_t*>(tile_ws + L.sorted_keys);
     tb.centroids = reinterpret_cast<float4*>(tile_ws + L.centroids);
@@ -793,8 +790,7 @@ int sgs_render_batch(const sgs_gaussians* gaussians, const sgs_view* views, uint
         tb.tile_count = reinterpret_cast<uint32_t*>(ws + L.tile_count);
         tb.tile_cursor = reinterpret_cast<uint32_t*>(ws + L.tile_cursor);
         tb.offsets = reinterpret_cast<uint32_t*>(ws + L.offsets);
-        tb.instances = reinterpret_cast<uint32_t*>(ws + L.instances);
-        tb.keys = reinterpret_cast<uint32_t*>(ws + L.keys);
+        tb.entries = reinterpret_cast<uint2*>(ws + L.entries);
         tb.sorted_instances = reinterpret_cast<uint32_t*>(ws + L.sorted_instances);
         tb.sorted_keys = reinterpret_cast<uint32_t*>(ws + L.sorted_keys);
         tb.centroids = reinterpret_cast<float4*>(ws + L.centroids);

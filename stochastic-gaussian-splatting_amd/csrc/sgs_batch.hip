@@ -380,13 +380,11 @@ constexpr uint32_t kSmallTiles = 32;
 
 template <bool kFill>
 __device__ __forceinline__ void bin_direct(uint32_t gt, uint32_t g, uint32_t key, uint32_t* tile_count,
-                                           uint32_t* tile_cursor, const uint32_t* offsets, uint32_t* instances,
-                                           uint32_t* keys)
+                                           uint32_t* tile_cursor, const uint32_t* offsets, uint2* entries)
 {
     if constexpr (kFill) {
         const uint32_t slot = offsets[gt] + atomicAdd(&tile_cursor[gt], 1u);
-        instances[slot] = g;
-        keys[slot] = key;
+        entries[slot] = make_uint2(key, g);
     } else {
         atomicAdd(&tile_count[gt], 1u);
     }
@@ -442,8 +440,7 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
                                                                 uint32_t* __restrict__ tile_count,
                                                                 uint32_t* __restrict__ tile_cursor,
                                                                 const uint32_t* __restrict__ offsets,
-                                                                uint32_t* __restrict__ instances,
-                                                                uint32_t* __restrict__ keys,
+                                                                uint2* __restrict__ entries,
                                                                 uint2* __restrict__ pair_rect,
                                                                 uint32_t* __restrict__ pair_key,
                                                                 BatchCounters* __restrict__ ctr)
@@ -551,8 +548,7 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
                 const uint32_t bg = bcast(g, l), bkey = bcast(key, l);
                 for (uint32_t k = lane; k < bn; k += 64u) {
                     const uint32_t ty = by0 + k / bw, tx = bx0 + k % bw;
-                    bin_direct<kFill>(bin0 + ty * gw + tx, bg, bkey, tile_count, tile_cursor, offsets,
-                                      instances, keys);
+                    bin_direct<kFill>(bin0 + ty * gw + tx, bg, bkey, tile_count, tile_cursor, offsets, entries);
                 }
             }
             __syncthreads();
@@ -597,8 +593,7 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
                                 for (uint32_t tx = tr.tx0; tx <= tr.tx1; tx++) {
                                     const uint32_t i = ((ty - wy0) * wt + tx - wx0) * kHistCopies + hc;
                                     const uint32_t slot = hbase[i] + atomicAdd(&hist[i], 1u);
-                                    instances[slot] = g;
-                                    keys[slot] = key;
+                                    entries[slot] = make_uint2(key, g);
                                 }
                     }
                 } else if (small) {
@@ -606,7 +601,7 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
                     for (uint32_t ty = tr.ty0; ty <= tr.ty1; ty++)
                         for (uint32_t tx = tr.tx0; tx <= tr.tx1; tx++)
                             bin_direct<kFill>(bin0 + ty * gw + tx, g, key, tile_count, tile_cursor,
-                                              offsets, instances, keys);
+                                              offsets, entries);
                 }
             }
             __syncthreads();
@@ -728,8 +723,7 @@ __device__ __forceinline__ uint32_t bucket_floor(uint32_t b, uint2 vk)
 __global__ __launch_bounds__(kBatchBlock) void tile_sort_kernel(uint32_t n_views, const uint32_t* __restrict__ tile_prefix,
                                                                 const uint2* __restrict__ view_keys,
                                                                 const uint32_t* __restrict__ offsets,
-                                                                const uint32_t* __restrict__ keys_in,
-                                                                const uint32_t* __restrict__ inst_in,
+                                                                const uint2* __restrict__ entries_in,
                                                                 uint32_t* __restrict__ keys_out,
                                                                 uint32_t* __restrict__ inst_out,
                                                                 const BatchCounters* __restrict__ ctr)
@@ -758,8 +752,9 @@ __global__ __launch_bounds__(kBatchBlock) void tile_sort_kernel(uint32_t n_views
 #pragma unroll
         for (uint32_t u = 0; u < kSortRegKeys; u++) {
             const uint32_t idx = tid + u * kBatchBlock;
-            rb[u] = idx < n ? keys_in[begin + idx] : 0u;
-            rg[u] = idx < n ? inst_in[begin + idx] : 0u;
+            const uint2 e = idx < n ? entries_in[begin + idx] : make_uint2(0u, 0u);
+            rb[u] = e.x;
+            rg[u] = e.y;
         }
 #pragma unroll
         for (uint32_t u = 0; u < kSortRegKeys; u++) {
@@ -774,11 +769,11 @@ __global__ __launch_bounds__(kBatchBlock) void tile_sort_kernel(uint32_t n_views
         for (; i + (kSortInFlight - 1u) * kBatchBlock < n; i += kSortInFlight * kBatchBlock) {
             uint32_t k[kSortInFlight];
 #pragma unroll
-            for (uint32_t u = 0; u < kSortInFlight; u++) k[u] = keys_in[begin + i + u * kBatchBlock];
+            for (uint32_t u = 0; u < kSortInFlight; u++) k[u] = entries_in[begin + i + u * kBatchBlock].x;
 #pragma unroll
             for (uint32_t u = 0; u < kSortInFlight; u++) atomicAdd(&hist[sort_bucket(k[u], vk)], 1u);
         }
-        for (; i < n; i += kBatchBlock) atomicAdd(&hist[sort_bucket(keys_in[begin + i], vk)], 1u);
+        for (; i < n; i += kBatchBlock) atomicAdd(&hist[sort_bucket(entries_in[begin + i].x, vk)], 1u);
     }
     __syncthreads();
     // exclusive scan of the buckets, 4 per thread
@@ -828,15 +823,18 @@ __global__ __launch_bounds__(kBatchBlock) void tile_sort_kernel(uint32_t n_views
         uint32_t k[kSortInFlight], g[kSortInFlight];
 #pragma unroll
         for (uint32_t u = 0; u < kSortInFlight; u++) {
-            k[u] = keys_in[begin + i + u * kBatchBlock];
-            g[u] = inst_in[begin + i + u * kBatchBlock];
+            const uint2 e = entries_in[begin + i + u * kBatchBlock];
+            k[u] = e.x;
+            g[u] = e.y;
         }
 #pragma unroll
         for (uint32_t u = 0; u < kSortInFlight; u++)
             inst_out[begin + atomicAdd(&hist[sort_bucket(k[u], vk)], 1u)] = g[u];
     }
-    for (; i < n; i += kBatchBlock)
-        inst_out[begin + atomicAdd(&hist[sort_bucket(keys_in[begin + i], vk)], 1u)] = inst_in[begin + i];
+    for (; i < n; i += kBatchBlock) {
+        const uint2 e = entries_in[begin + i];
+        inst_out[begin + atomicAdd(&hist[sort_bucket(e.x, vk)], 1u)] = e.y;
+    }
 }
 
 // inclusive scan of one value per thread over the 256-thread workgroup
@@ -1200,7 +1198,7 @@ hipError_t launch_batch_tiles(const SceneArgs& sc, const ViewDesc* views, uint32
         hipLaunchKernelGGL(cov3d_kernel, dim3(n_chunks), dim3(kBatchBlock), 0, stream, sc, tb.cov3d);
         hipLaunchKernelGGL((batch_bin_kernel<false>), dim3((unsigned)grid), dim3(kBatchBlock), 0, stream, sc, tb.cov3d,
                            views, n_views, tb.tile_count, tb.tile_cursor, tb.offsets,
-                           tb.instances, tb.keys, tb.pair_rect, tb.pair_key, counters);
+                           tb.entries, tb.pair_rect, tb.pair_key, counters);
     }
     hipLaunchKernelGGL((exclusive_scan_kernel<CountInput>), dim3(1), dim3(kScanThreads), 0, stream,
                        CountInput{ tb.tile_count }, tb.offsets, nullptr, tb.n_tiles, tb.n_tiles, &counters->n_instances,
@@ -1208,10 +1206,10 @@ hipError_t launch_batch_tiles(const SceneArgs& sc, const ViewDesc* views, uint32
     if (sc.n > 0) {
         hipLaunchKernelGGL((batch_bin_kernel<true>), dim3((unsigned)grid), dim3(kBatchBlock), 0, stream, sc, tb.cov3d,
                            views, n_views, tb.tile_count, tb.tile_cursor, tb.offsets,
-                           tb.instances, tb.keys, tb.pair_rect, tb.pair_key, counters);
+                           tb.entries, tb.pair_rect, tb.pair_key, counters);
     }
     hipLaunchKernelGGL(tile_sort_kernel, dim3(tb.n_tiles), dim3(kBatchBlock), 0, stream, n_views, tile_prefix,
-                       tb.view_keys, tb.offsets, tb.keys, tb.instances, tb.sorted_keys, tb.sorted_instances, counters);
+                       tb.view_keys, tb.offsets, tb.entries, tb.sorted_keys, tb.sorted_instances, counters);
     if (raster_event) {
         e = hipEventRecord(raster_event, stream);
         if (e != hipSuccess) return e;

@@ -197,8 +197,7 @@ struct TileBuffers {
     uint32_t* tile_count;       // [n_tiles] instances per tile (zeroed by the caller)
     uint32_t* tile_cursor;      // [n_tiles] fill cursors (zeroed by the caller)
     uint32_t* offsets;          // [n_tiles + 1] exclusive scan of tile_count
-    uint32_t* instances;        // [capacity] Gaussian index of each (view, tile, Gaussian) instance
-    uint32_t* keys;             // [capacity] its depth key (float bits of the view depth)
+    uint2* entries;             // [capacity] (depth key, Gaussian index) of each (view, bin, Gaussian) instance
     uint32_t* sorted_instances; // [capacity] each tile's instances, nearest depth bucket first
     uint32_t* sorted_keys;      // [capacity] depth bucket floor at each tile batch start (begin + 256 m)
     float4* centroids;          // [n_chunks] mean position of each chunk of 256 Gaussians
