@@ -91,7 +91,7 @@ namespace {
 // ---- single-view tile schedule (the batched renderer's tile pipeline for one view) ----
 struct SingleTileLayout {
     size_t counters, ready, tile_count, tile_cursor, cleared_end, offsets, entries, sorted_instances, sorted_keys,
-        centroids, view_keys, cov3d, pair_rect, pair_key, table, queue, total;
+        centroids, view_keys, cov3d, pair_rect, table, queue, total;
     size_t views_bytes;
     uint32_t n_tiles, capacity, queue_capacity;
 };
@@ -134,7 +134,6 @@ SingleTileLayout single_tile_layout(const FbCamera& cam, uint32_t n_gaussians, u
     L.view_keys = a.take(8u);
     L.cov3d = a.take(32u * (size_t)n_gaussians);
     L.pair_rect = a.take(8u * (size_t)n_gaussians);
-    L.pair_key = a.take(4u * (size_t)n_gaussians);
     L.table = a.take(L.views_bytes + sizeof(SingleViewPrefixes));
     L.queue = a.take(sizeof(BigEntry) * (size_t)L.queue_capacity);
     L.total = a.offset;
@@ -465,7 +464,7 @@ uint64_t view_raster_groups(const sgs_view& v)
 // cursors are contiguous so one fill clears them all.
 struct BatchLayout {
     size_t counters, ready, tile_count, tile_cursor, cleared_end, offsets, entries, sorted_instances, sorted_keys,
-        centroids, cov3d, pair_rect, pair_key, view_keys, queue, table, fb, samples, total;
+        centroids, cov3d, pair_rect, view_keys, queue, table, fb, samples, total;
     uint64_t fb_words, n_samples, n_tiles, n_raster;
     uint32_t n_blocks, capacity;
 };
@@ -509,7 +508,6 @@ BatchLayout batch_layout(const sgs_view* views, uint32_t n, uint32_t n_gaussians
     L.view_keys = a.take(8u * (size_t)n);
     L.cov3d = a.take(tiles ? 32u * (size_t)n_gaussians : 0u);
     L.pair_rect = a.take(tiles ? 8u * (size_t)n_gaussians * n : 0u);
-    L.pair_key = a.take(tiles ? 4u * (size_t)n_gaussians * n : 0u);
     L.queue = a.take(sizeof(BigEntry) * (size_t)t.queue_capacity);
     L.table = a.take(table_bytes(n));
     L.fb = a.take(8u * L.fb_words);
@@ -689,7 +687,6 @@ int accumulate_tiles_single(const sgs_gaussians* g, const sgs_settings* s, const
     tb.view_keys = reinterpret_cast<uint2*>(tile_ws + L.view_keys);
     tb.cov3d = reinterpret_cast<float4*>(tile_ws + L.cov3d);
     tb.pair_rect = reinterpret_cast<uint2*>(tile_ws + L.pair_rect);
-    tb.pair_key = reinterpret_cast<uint32_t*>(tile_ws + L.pair_key);
     tb.n_tiles = L.n_tiles;
     tb.capacity = L.capacity;
     tb.n_raster = prefixes.w[5];
@@ -797,7 +794,6 @@ int sgs_render_batch(const sgs_gaussians* gaussians, const sgs_view* views, uint
         tb.view_keys = reinterpret_cast<uint2*>(ws + L.view_keys);
         tb.cov3d = reinterpret_cast<float4*>(ws + L.cov3d);
         tb.pair_rect = reinterpret_cast<uint2*>(ws + L.pair_rect);
-        tb.pair_key = reinterpret_cast<uint32_t*>(ws + L.pair_key);
         tb.n_tiles = (uint32_t)L.n_tiles;
         tb.capacity = L.capacity;
         tb.n_raster = (uint32_t)L.n_raster;

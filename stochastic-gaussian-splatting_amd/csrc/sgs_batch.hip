@@ -442,7 +442,6 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
                                                                 const uint32_t* __restrict__ offsets,
                                                                 uint2* __restrict__ entries,
                                                                 uint2* __restrict__ pair_rect,
-                                                                uint32_t* __restrict__ pair_key,
                                                                 BatchCounters* __restrict__ ctr)
 {
     __shared__ uint32_t s_wx0, s_wx1, s_wy0, s_wy1;
@@ -463,16 +462,18 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
     // chunk on one address; the per-chunk work is nearly uniform). A lane loads its
     // Gaussian's parameters once and bins it for every view in turn, so the parameter
     // stream is read from memory once per pass instead of once per view. The counting pass
-    // projects and records each (view, Gaussian) pair's tile rectangle and depth key; the
-    // fill pass reads them back instead of projecting again.
+    // projects and records each (view, Gaussian) pair's bin rectangle; the fill pass reads it
+    // back instead of projecting again, with the depth key recomputed from the mean.
     for (uint32_t chunk = blockIdx.x; chunk < n_chunks; chunk += gridDim.x) {
         const uint32_t g = chunk * kBatchBlock + tid;
         float mx = 0.0f, my = 0.0f, mz = 0.0f;
         Cov3d cv = { 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, -1.0f, 0.0f };
-        if (!kFill && g < sc.n) {
+        if (g < sc.n) {
             mx = sc.means[3u * g + 0u];
             my = sc.means[3u * g + 1u];
             mz = sc.means[3u * g + 2u];
+        }
+        if (!kFill && g < sc.n) {
             const float4 a = cov3d[2u * g], b = cov3d[2u * g + 1u];
             cv = { a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w };
         }
@@ -480,11 +481,7 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
         // the fill pass reads each view's pair record one view ahead, so the load of view v + 1
         // is in flight while view v is binned
         uint2 next_rect = make_uint2(kNoTiles, kNoTiles);
-        uint32_t next_key = 0u;
-        if (kFill && g < sc.n) {
-            next_rect = pair_rect[g];
-            next_key = pair_key[g];
-        }
+        if (kFill && g < sc.n) next_rect = pair_rect[g];
         for (uint32_t v = 0; v < n_views; v++) {
             if (tid == 0) {
                 s_wx0 = 0xffffffffu;
@@ -501,20 +498,18 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
             const size_t pair = (size_t)v * sc.n + g;
             if constexpr (kFill) {
                 const uint2 r = next_rect;
-                key = next_key;
-                if (g < sc.n && v + 1u < n_views) {
-                    next_rect = pair_rect[pair + sc.n];
-                    next_key = pair_key[pair + sc.n];
-                }
+                if (g < sc.n && v + 1u < n_views) next_rect = pair_rect[pair + sc.n];
                 has = r.x != kNoTiles;
                 tr = { r.x & 0xffffu, r.y & 0xffffu, r.x >> 16, r.y >> 16 };
+                // the depth key again from the mean in registers: the operations of
+                // gaussian_tiles_reg, so the same bits
+                const float4 r3 = vd.r3;
+                key = f2u(r3.x * mx + r3.y * my + r3.z * mz + r3.w);
             } else {
                 has = gaussian_tiles_reg(mx, my, mz, cv, valid, vd, kRasterLog2 + vd.bin_log, tr, key);
-                if (g < sc.n) {
+                if (g < sc.n)
                     pair_rect[pair] = has ? make_uint2(tr.tx0 | (tr.ty0 << 16), tr.tx1 | (tr.ty1 << 16))
                                           : make_uint2(kNoTiles, kNoTiles);
-                    pair_key[pair] = key;
-                }
             }
             const uint32_t tw = has ? tr.tx1 - tr.tx0 + 1u : 0u;
             const uint32_t n_t = has ? tw * (tr.ty1 - tr.ty0 + 1u) : 0u;
@@ -1198,7 +1193,7 @@ hipError_t launch_batch_tiles(const SceneArgs& sc, const ViewDesc* views, uint32
         hipLaunchKernelGGL(cov3d_kernel, dim3(n_chunks), dim3(kBatchBlock), 0, stream, sc, tb.cov3d);
         hipLaunchKernelGGL((batch_bin_kernel<false>), dim3((unsigned)grid), dim3(kBatchBlock), 0, stream, sc, tb.cov3d,
                            views, n_views, tb.tile_count, tb.tile_cursor, tb.offsets,
-                           tb.entries, tb.pair_rect, tb.pair_key, counters);
+                           tb.entries, tb.pair_rect, counters);
     }
     hipLaunchKernelGGL((exclusive_scan_kernel<CountInput>), dim3(1), dim3(kScanThreads), 0, stream,
                        CountInput{ tb.tile_count }, tb.offsets, nullptr, tb.n_tiles, tb.n_tiles, &counters->n_instances,
@@ -1206,7 +1201,7 @@ hipError_t launch_batch_tiles(const SceneArgs& sc, const ViewDesc* views, uint32
     if (sc.n > 0) {
         hipLaunchKernelGGL((batch_bin_kernel<true>), dim3((unsigned)grid), dim3(kBatchBlock), 0, stream, sc, tb.cov3d,
                            views, n_views, tb.tile_count, tb.tile_cursor, tb.offsets,
-                           tb.entries, tb.pair_rect, tb.pair_key, counters);
+                           tb.entries, tb.pair_rect, counters);
     }
     hipLaunchKernelGGL(tile_sort_kernel, dim3(tb.n_tiles), dim3(kBatchBlock), 0, stream, n_views, tile_prefix,
                        tb.view_keys, tb.offsets, tb.entries, tb.sorted_keys, tb.sorted_instances, counters);
