@@ -424,7 +424,11 @@ uint32_t default_tile_instances(const sgs_view* views, uint32_t n_views, uint32_
     return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(want, 1ull << 16), 1ull << 31);
 }
 
-constexpr uint32_t kResolveBlock = 256;
+// batched resolve workgroups of a view: one per 16x16 tile of its image pixels
+uint64_t view_resolve_blocks(const sgs_view& v)
+{
+    return ((uint64_t)(v.width + kResolveTile - 1u) / kResolveTile) * ((v.height + kResolveTile - 1u) / kResolveTile);
+}
 
 // device view table: ViewDesc[n], then the resolve block prefix[n+1], the bin prefix[n+1] and
 // the rasteriser workgroup prefix[n+1]
@@ -454,10 +458,11 @@ uint32_t view_grid_h(const sgs_view& v)
     const uint32_t bin = kRasterPixels << view_bin_log(v);
     return (uint32_t)((v.height * view_ss(v) + bin - 1u) / bin);
 }
-// rasteriser workgroups of a view: 16x16 tiles of its bins, 4^b per bin
+// rasteriser workgroups of a view: 16x16 tiles of its bins, 4^b per bin, padded to a multiple
+// of 4 so every view starts a group of four (the extra workgroups return at once)
 uint64_t view_raster_groups(const sgs_view& v)
 {
-    return ((uint64_t)view_grid_w(v) * view_grid_h(v)) << (2u * view_bin_log(v));
+    return ((((uint64_t)view_grid_w(v) * view_grid_h(v)) << (2u * view_bin_log(v))) + 3u) & ~3ull;
 }
 
 // Workspace of a batch. The counters, the queue's ready flags and the tile counts and
@@ -481,7 +486,7 @@ BatchLayout batch_layout(const sgs_view* views, uint32_t n, uint32_t n_gaussians
         const uint64_t P = view_fb_pixels(views[v]);
         L.fb_words += P * views[v].spp;
         if (views[v].temporal_mode == SGS_TEMPORAL_REPROJECT) L.n_samples += P;
-        blocks += ((uint64_t)views[v].width * views[v].height + kResolveBlock - 1u) / kResolveBlock;
+        blocks += view_resolve_blocks(views[v]);
         L.n_tiles += (uint64_t)view_grid_w(views[v]) * view_grid_h(views[v]);
         L.n_raster += view_raster_groups(views[v]);
     }
@@ -554,7 +559,7 @@ int check_views(const sgs_view* views, uint32_t n)
             (v.history.image == v.out.image || v.history.depth == v.out.depth || v.history.alpha == v.out.alpha ||
              v.history.weight_mass == v.out.weight_mass))
             return fail(SGS_ERROR_INVALID_ARGUMENT, at + "outputs must not alias the history");
-        blocks += ((uint64_t)v.width * v.height + kResolveBlock - 1u) / kResolveBlock;
+        blocks += view_resolve_blocks(v);
     }
     if (blocks >= (1ull << 31)) return fail(SGS_ERROR_INVALID_ARGUMENT, "too many output pixels in one batch");
     // a view may read an earlier view's outputs as its history (all four buffers); no view
@@ -636,7 +641,7 @@ void encode_views(const sgs_view* views, uint32_t n, void* host)
         const uint64_t P = view_fb_pixels(v);
         fb_off += P * v.spp;
         if (v.temporal_mode == SGS_TEMPORAL_REPROJECT) smp_off += P;
-        block += (uint32_t)(((uint64_t)v.width * v.height + kResolveBlock - 1u) / kResolveBlock);
+        block += (uint32_t)view_resolve_blocks(v);
     }
     prefix[n] = block;
     tile_prefix[n] = tile;

@@ -931,25 +931,32 @@ __global__ __launch_bounds__(kBatchBlock, kLayers <= 8u ? 5 : 3) void batch_tile
     __shared__ uint32_t s_rect[kBatchBlock], s_end[kBatchBlock];
     __shared__ uint32_t s_wave[kBatchBlock / 64u];
     if (ctr->overflow) return;
-    // workgroup -> view (last v with raster_prefix[v] <= blockIdx.x) -> (bin lt of the view,
-    // 16x16 tile q of the bin's 2^b x 2^b)
+    // Workgroups are handed to the eight XCDs round-robin (workgroup i to XCD i % 8), each XCD
+    // with its own L2. Within every 32 workgroups, the four that share an XCD take four
+    // consecutive tiles (a 32-pixel bin's four tiles, as views start at multiples of 4), so a
+    // bin's list and its Gaussians are fetched into one L2: blk is the tile in view order.
+    const uint32_t j = blockIdx.x & 31u;
+    const uint32_t blk = (blockIdx.x & ~31u) | ((j & 7u) << 2) | (j >> 3);
+    if (blk >= raster_prefix[n_views]) return;  // the grid is a multiple of 32
+    // tile -> view (last v with raster_prefix[v] <= blk) -> (bin lt of the view, 16x16 tile q
+    // of the bin's 2^b x 2^b)
     uint32_t lo = 0, hi = n_views - 1u;
     while (lo < hi) {
         const uint32_t mid = (lo + hi + 1u) >> 1;
-        if (raster_prefix[mid] <= blockIdx.x) lo = mid;
+        if (raster_prefix[mid] <= blk) lo = mid;
         else hi = mid - 1u;
     }
     const ViewDesc& vd = views[lo];
     // one-sample views are rasterised by the one-layer kernel, the others by the layered one
     if ((kLayers == 1u) != (vd.spp == 1u)) return;
-    const uint32_t bl = vd.bin_log, r = blockIdx.x - raster_prefix[lo];
+    const uint32_t bl = vd.bin_log, r = blk - raster_prefix[lo];
     const uint32_t lt = r >> (2u * bl), q = r & ((1u << (2u * bl)) - 1u);
     const uint32_t gt = vd.tile_begin + lt;
     const uint32_t gw = vd.grid_w;
     const uint32_t ox = ((lt % gw) << (kRasterLog2 + bl)) + (q & ((1u << bl) - 1u)) * kTile;
     const uint32_t oy = ((lt / gw) << (kRasterLog2 + bl)) + (q >> bl) * kTile;
     const uint32_t W = vd.cam.width_i, H = vd.cam.height_i;
-    if (ox >= W || oy >= H) return;  // a tile of an edge bin beyond the framebuffer
+    if (ox >= W || oy >= H) return;  // a tile of an edge bin beyond the framebuffer, or padding
     const size_t P = (size_t)W * H;
     const uint32_t begin = offsets[gt], end = offsets[gt + 1u];
     const uint32_t tid = threadIdx.x, lane = lane_id();
@@ -1210,8 +1217,8 @@ hipError_t launch_batch_tiles(const SceneArgs& sc, const ViewDesc* views, uint32
         if (e != hipSuccess) return e;
     }
     // one workgroup per 16x16 tile, 4^b per bin of (16 << b) pixels (the caller keeps the total
-    // below 2^31; it equals raster_prefix[n_views])
-    const dim3 raster_grid(tb.n_raster);
+    // below 2^31; it equals raster_prefix[n_views]), rounded up to a multiple of 32
+    const dim3 raster_grid((tb.n_raster + 31u) & ~31u);
     // Views of one sample take the one-layer kernel (less LDS per workgroup), the others the
     // kernel with `layers` samples per pass. Both launches span every view's tiles and return at
     // once on the other's (measured faster than two launches over class-ordered tiles, whose

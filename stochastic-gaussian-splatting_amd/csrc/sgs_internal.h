@@ -19,6 +19,9 @@ constexpr uint32_t kBinSizes = 3;
 // default bin of a view: 16 pixels for framebuffers of at most this many pixels (few bins
 // with long lists otherwise, each read by four tiles), 32 pixels above
 constexpr uint64_t kSmallBinMaxPixels = 1ull << 19;
+// Batched resolve: one workgroup per 16x16 tile of image pixels, so a winner covering a
+// pixel's neighbours above and below is shaded from the same CU's caches.
+constexpr uint32_t kResolveTile = 16;
 // bin_size (16, 32 or 64 pixels) -> b; kBinSizes for any other value
 inline uint32_t bin_log2(uint32_t bin_size)
 {

@@ -540,8 +540,12 @@ __global__ __launch_bounds__(kBlockResolve) void batch_resolve_kernel(SceneArgs 
     const ViewDesc& vd = views[lo];
     // views whose history is another view's output of the batch resolve in a later launch
     if (vd.level != level) return;
-    const uint32_t pix = (blockIdx.x - block_prefix[lo]) * kBlockResolve + threadIdx.x;
-    if (pix >= vd.width * vd.height) return;
+    // 16x16 tile t of the view's image pixels, row-major over ceil(W/16) tiles per row
+    const uint32_t t = blockIdx.x - block_prefix[lo], tw = (vd.width + kResolveTile - 1u) / kResolveTile;
+    const uint32_t px = (t % tw) * kResolveTile + threadIdx.x % kResolveTile;
+    const uint32_t py = (t / tw) * kResolveTile + threadIdx.x / kResolveTile;
+    if (px >= vd.width || py >= vd.height) return;
+    const uint32_t pix = py * vd.width + px;
     const size_t P = (size_t)vd.cam.width_i * vd.cam.height_i;
     const uint64_t* fbv = fb + vd.fb_offset;
     TemporalOut o;

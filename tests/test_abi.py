@@ -7,6 +7,7 @@ import ctypes as C
 import re
 from pathlib import Path
 
+import numpy as np
 import pytest
 
 REPO = Path(__file__).resolve().parents[1]
@@ -171,6 +172,17 @@ def test_batch_workspace_and_schedule_validation(native):
     assert sizes[0] == sizes[16]  # frames of at most 2^19 pixels take 16-pixel bins by default
     host = (C.c_uint8 * lib.sgs_batch_view_table_bytes(2))()
     assert lib.sgs_batch_encode_views(views, 2, host) == 0
+    # the table ends with three prefixes over the views: batched resolve workgroups (one per
+    # 16x16 tile of image pixels), bins, and rasteriser workgroups (16x16 tiles of the bins,
+    # padded to a multiple of 4 per view)
+    views3 = _batch_views(native, 3)  # 100x60, 137x65, 174x70
+    views3[0].bin_size, views3[1].bin_size, views3[2].bin_size = 16, 16, 32
+    host3 = (C.c_uint8 * lib.sgs_batch_view_table_bytes(3))()
+    assert lib.sgs_batch_encode_views(views3, 3, host3) == 0
+    words = np.frombuffer(bytes(host3), np.uint32)[-12:].reshape(3, 4)
+    assert words[0].tolist() == [0, 7 * 4, 7 * 4 + 9 * 5, 7 * 4 + 9 * 5 + 11 * 5]
+    assert words[1].tolist() == [0, 7 * 4, 7 * 4 + 9 * 5, 7 * 4 + 9 * 5 + 6 * 3]
+    assert words[2].tolist() == [0, 28, 28 + 48, 28 + 48 + 4 * 18]
 
 
 def test_batch_output_aliasing_rules(native):
