@@ -234,7 +234,7 @@ def roofline_render_samples(scene_np, views, scene, step_ms, work, atomics_per_s
 # HBM bytes per launch of the tile rasteriser on the default workload (6M Gaussians, 64 views,
 # 8 samples per LDS pass), from rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0 (calibrated on a
 # 1 GiB device copy); the commands and raw counters are in profiles/tile_raster_c4_6M_64views.md.
-PROFILED_RASTER_TRAFFIC = {"bytes": 10.99e9, "profile": "profiles/tile_raster_c4_6M_64views.md"}
+PROFILED_RASTER_TRAFFIC = {"bytes": 6.58e9, "profile": "profiles/tile_raster_c4_6M_64views.md"}
 
 
 def roofline_tiles(views, raster_ms: list, n_instances: int, visible_pairs: int, default_workload: bool):

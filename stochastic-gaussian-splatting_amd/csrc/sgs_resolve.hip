@@ -517,10 +517,26 @@ hipError_t launch_shade_resolve(const SceneArgs& sc, const float* cam_position, 
 // batched resolve: every output pixel of every view folds its view's spp samples in order,
 // sample 0 with the view's temporal mode and the rest STATIC, the operations of spp
 // successive single-sample calls (the history of sample s is the output of sample s - 1).
-// A pixel's winners often repeat across samples, so its six most recently shaded winners are
-// kept with their colours (the colour of a winner depends only on the view and the Gaussian).
+// The colour of a winner depends only on the view and the Gaussian, and the pixels of a tile
+// share most of their winners across pixels and samples, so each workgroup (one 16x16 tile of
+// image pixels) keeps its distinct winners in an LDS hash table. Every framebuffer word is read
+// once: the pixels take kChunk words each into registers and insert their winners, the
+// workgroup's lanes shade the winners new in the table, then each pixel folds its kChunk words
+// with the colours looked up in the table. A winner the table cannot take (kShadeProbes
+// occupied slots) is shaded by its pixel, so the colours are those of shade() in every case.
 // ------------------------------------------------------------------------------------------
 namespace sgs {
+
+constexpr uint32_t kShadeLog2 = 11;
+constexpr uint32_t kShadeSlots = 1u << kShadeLog2;
+constexpr uint32_t kShadeProbes = 64;
+constexpr uint32_t kNoWinner = 0xffffffffu;  // empty slot (Gaussian indices are below 2^32 - 1)
+constexpr uint32_t kChunk = 4;  // framebuffer words per pixel in registers between the phases
+
+__device__ __forceinline__ uint32_t shade_slot(uint32_t g)
+{
+    return (g * 0x9E3779B1u) >> (32u - kShadeLog2);  // multiplicative (Fibonacci) hashing
+}
 
 template <uint32_t B>
 __global__ __launch_bounds__(kBlockResolve) void batch_resolve_kernel(SceneArgs sc, const ViewDesc* __restrict__ views,
@@ -530,6 +546,10 @@ __global__ __launch_bounds__(kBlockResolve) void batch_resolve_kernel(SceneArgs 
                                                                       const float4* __restrict__ samples,
                                                                       uint32_t level)
 {
+    __shared__ uint32_t s_key[kShadeSlots];
+    __shared__ float s_red[kShadeSlots], s_green[kShadeSlots], s_blue[kShadeSlots];
+    __shared__ uint16_t s_new[kShadeSlots];  // slots in the order their winners were inserted
+    __shared__ uint32_t s_n_new;
     // view of this block: last v with block_prefix[v] <= blockIdx.x
     uint32_t lo = 0, hi = n_views - 1u;
     while (lo < hi) {
@@ -544,86 +564,129 @@ __global__ __launch_bounds__(kBlockResolve) void batch_resolve_kernel(SceneArgs 
     const uint32_t t = blockIdx.x - block_prefix[lo], tw = (vd.width + kResolveTile - 1u) / kResolveTile;
     const uint32_t px = (t % tw) * kResolveTile + threadIdx.x % kResolveTile;
     const uint32_t py = (t / tw) * kResolveTile + threadIdx.x / kResolveTile;
-    if (px >= vd.width || py >= vd.height) return;
-    const uint32_t pix = py * vd.width + px;
+    const bool active = px < vd.width && py < vd.height;
+    const uint32_t pix = active ? py * vd.width + px : 0u;
     const size_t P = (size_t)vd.cam.width_i * vd.cam.height_i;
     const uint64_t* fbv = fb + vd.fb_offset;
-    TemporalOut o;
-    uint32_t s0 = 0;
-    if (vd.mode == SGS_TEMPORAL_REPROJECT) {
+    const uint32_t s0 = vd.mode == SGS_TEMPORAL_REPROJECT ? 1u : 0u;
+    const uint32_t ss = (uint32_t)vd.cam.ss;
+    const uint32_t bx = px * ss, by = py * ss;
+    for (uint32_t i = threadIdx.x; i < kShadeSlots; i += kBlockResolve) s_key[i] = kNoWinner;
+    if (threadIdx.x == 0) s_n_new = 0u;
+    TemporalOut o = { 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f };
+    if (s0 == 1u && active)
         o = resolve_from_samples<false>(vd.cam, vd.width, vd.height, vd.focal_x, vd.focal_y, vd.center_x, vd.center_y,
                                         samples + vd.samples_offset, nullptr, vd.hist, vd.v2p, vd.mode, pix);
-        s0 = 1;
-    }
-    const uint32_t ss = (uint32_t)vd.cam.ss;
-    const uint32_t bx = (pix % vd.width) * ss, by = (pix / vd.width) * ss;
+    __syncthreads();
+    // the pixel's framebuffer words in (sample, subpixel row, subpixel column) order, kChunk at a
+    // time (the count is the same for every pixel of the view, so the barriers are uniform)
+    const uint32_t n_words = (vd.spp - s0) * ss * ss;
     const float nf = 1.0f / (float)(ss * ss);
-    // the kCached most recent distinct winners of the pixel stay shaded (most recent first)
-    constexpr uint32_t kCached = 6;
-    uint32_t cg[kCached];
-    float3 cc[kCached];
+    uint32_t ls = s0, ly = 0, lx = 0;  // load cursor
+    uint32_t fs = s0, fy = 0, fx = 0;  // fold cursor
+    SpatialSums sp = { 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0u };
+    uint32_t last = kNoWinner, cg = kNoWinner, n_done = 0;
+    float3 cc = make_float3(0.0f, 0.0f, 0.0f);
+    for (uint32_t c0 = 0; c0 < n_words; c0 += kChunk) {
+        uint64_t w[kChunk];
 #pragma unroll
-    for (uint32_t i = 0; i < kCached; i++) {
-        cg[i] = 0xffffffffu;
-        cc[i] = make_float3(0.0f, 0.0f, 0.0f);
-    }
-    for (uint32_t s = s0; s < vd.spp; s++) {
-        SpatialSums sp = { 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0u };
-        for (uint32_t oy = 0; oy < ss; oy++) {
-            for (uint32_t ox = 0; ox < ss; ox++) {
-                const uint64_t data = fbv[s * P + (size_t)(by + oy) * vd.cam.width_i + bx + ox];
-                const uint32_t dk = (uint32_t)(data >> 32);
-                float4 sm = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                if (dk != 0xffffffffu) {
-                    const uint32_t g = (uint32_t)data;
-                    if (g != cg[0]) {
-                        uint32_t hit = kCached;
-#pragma unroll
-                        for (uint32_t i = 1; i < kCached; i++)
-                            if (cg[i] == g) hit = i;
-                        float3 c = make_float3(0.0f, 0.0f, 0.0f);
-#pragma unroll
-                        for (uint32_t i = 1; i < kCached; i++)
-                            if (hit == i) c = cc[i];
-                        if (hit == kCached) c = shade<B>(sc, vd.cam_x, vd.cam_y, vd.cam_z, g);
-                        // move to front: entries before the hit (or all, on a miss) shift back
-#pragma unroll
-                        for (uint32_t i = kCached - 1u; i > 0; i--)
-                            if (i <= hit) {
-                                cg[i] = cg[i - 1u];
-                                cc[i] = cc[i - 1u];
-                            }
-                        cg[0] = g;
-                        cc[0] = c;
+        for (uint32_t u = 0; u < kChunk; u++) {
+            w[u] = ~0ull;
+            if (c0 + u < n_words) {
+                if (active) w[u] = fbv[(size_t)ls * P + (size_t)(by + ly) * vd.cam.width_i + bx + lx];
+                if (++lx == ss) {
+                    lx = 0;
+                    if (++ly == ss) {
+                        ly = 0;
+                        ls++;
                     }
-                    sm = make_float4(cc[0].x, cc[0].y, cc[0].z, u2f(dk));
                 }
-                add_subpixel(sp, sm, sm.x, sm.y, sm.z, sm.w, sm.w > 0.0f ? 1.0f : 0.0f, 1.0f);
             }
         }
-        sp.c0 = sp.c0 * nf;
-        sp.c1 = sp.c1 * nf;
-        sp.c2 = sp.c2 * nf;
-        sp.d = sp.d * nf;
-        sp.a = sp.a * nf;
-        sp.w = sp.w * nf;
-        if (s == 0) {
-            o = { sp.c0, sp.c1, sp.c2, sp.d, sp.a, sp.w };
-            if (vd.mode == SGS_TEMPORAL_STATIC) o = temporal_static(sp, vd.hist, pix);
-        } else {
-            // STATIC against the previous sample's output (resolve.cu:250-258)
-            const float hm = o.w;
-            const float w = hm + sp.w;
-            const float hf = hm / w;
-            o.c0 = lerpf(sp.c0, o.c0, hf);
-            o.c1 = lerpf(sp.c1, o.c1, hf);
-            o.c2 = lerpf(sp.c2, o.c2, hf);
-            o.d = lerpf(sp.d, o.d, hf);
-            o.a = lerpf(sp.a, o.a, hf);
-            o.w = w;
+        // 1. the chunk's winners into the table (open addressing with linear probing; slots are
+        // never freed, so a winner that found no slot never gets one later)
+#pragma unroll
+        for (uint32_t u = 0; u < kChunk; u++) {
+            const uint32_t g = (uint32_t)w[u];
+            if ((uint32_t)(w[u] >> 32) == 0xffffffffu || g == last) continue;
+            last = g;
+            uint32_t h = shade_slot(g);
+            for (uint32_t k = 0; k < kShadeProbes; k++) {
+                uint32_t old = s_key[h];
+                if (old == kNoWinner) old = atomicCAS(&s_key[h], kNoWinner, g);
+                if (old == kNoWinner) {
+                    s_new[atomicAdd(&s_n_new, 1u)] = (uint16_t)h;
+                    break;
+                }
+                if (old == g) break;
+                h = (h + 1u) & (kShadeSlots - 1u);
+            }
+        }
+        __syncthreads();
+        // 2. the winners new in this chunk shaded once, the workgroup's lanes over the list
+        const uint32_t n_new = s_n_new;
+        for (uint32_t i = n_done + threadIdx.x; i < n_new; i += kBlockResolve) {
+            const uint32_t h = s_new[i];
+            const float3 c = shade<B>(sc, vd.cam_x, vd.cam_y, vd.cam_z, s_key[h]);
+            s_red[h] = c.x;
+            s_green[h] = c.y;
+            s_blue[h] = c.z;
+        }
+        n_done = n_new;
+        __syncthreads();
+        if (!active) continue;
+        // 3. the chunk's words folded in order; a sample's box filter completes at its last word
+#pragma unroll
+        for (uint32_t u = 0; u < kChunk; u++) {
+            if (c0 + u >= n_words) break;
+            const uint32_t dk = (uint32_t)(w[u] >> 32);
+            float4 sm = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (dk != 0xffffffffu) {
+                const uint32_t g = (uint32_t)w[u];
+                if (g != cg) {
+                    uint32_t h = shade_slot(g), k = 0;
+                    for (; k < kShadeProbes; k++) {
+                        const uint32_t key = s_key[h];
+                        if (key == g || key == kNoWinner) break;
+                        h = (h + 1u) & (kShadeSlots - 1u);
+                    }
+                    if (k < kShadeProbes && s_key[h] == g) cc = make_float3(s_red[h], s_green[h], s_blue[h]);
+                    else cc = shade<B>(sc, vd.cam_x, vd.cam_y, vd.cam_z, g);
+                    cg = g;
+                }
+                sm = make_float4(cc.x, cc.y, cc.z, u2f(dk));
+            }
+            add_subpixel(sp, sm, sm.x, sm.y, sm.z, sm.w, sm.w > 0.0f ? 1.0f : 0.0f, 1.0f);
+            if (++fx < ss) continue;
+            fx = 0;
+            if (++fy < ss) continue;
+            fy = 0;
+            sp.c0 = sp.c0 * nf;
+            sp.c1 = sp.c1 * nf;
+            sp.c2 = sp.c2 * nf;
+            sp.d = sp.d * nf;
+            sp.a = sp.a * nf;
+            sp.w = sp.w * nf;
+            if (fs == 0) {
+                o = { sp.c0, sp.c1, sp.c2, sp.d, sp.a, sp.w };
+                if (vd.mode == SGS_TEMPORAL_STATIC) o = temporal_static(sp, vd.hist, pix);
+            } else {
+                // STATIC against the previous sample's output (resolve.cu:250-258)
+                const float hm = o.w;
+                const float wsum = hm + sp.w;
+                const float hf = hm / wsum;
+                o.c0 = lerpf(sp.c0, o.c0, hf);
+                o.c1 = lerpf(sp.c1, o.c1, hf);
+                o.c2 = lerpf(sp.c2, o.c2, hf);
+                o.d = lerpf(sp.d, o.d, hf);
+                o.a = lerpf(sp.a, o.a, hf);
+                o.w = wsum;
+            }
+            fs++;
+            sp = { 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0u };
         }
     }
-    store_output(vd.out, pix, o);
+    if (active) store_output(vd.out, pix, o);
 }
 
 hipError_t launch_batch_resolve(const SceneArgs& sc, const ViewDesc* views, uint32_t n_views,

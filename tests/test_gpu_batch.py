@@ -102,16 +102,28 @@ def test_batch_equals_plain_spp_loop(gpu, tuning):
         assert_close(outs[v], ref)
 
 
-@pytest.mark.parametrize("case", ["many_samples", "many_samples_8", "wide_views", "wide_views_16"])
-def test_batch_tile_schedule_edges(gpu, case):
-    """More than 16 samples (several LDS passes per tile), and views wider than a
-    workgroup's 1024-tile binning window with large Gaussians (direct binning), each
-    against the scatter schedule and the plain SPP loop."""
+@pytest.mark.parametrize("case", ["many_samples", "many_samples_8", "wide_views", "wide_views_16",
+                                  "shade_table_overflow"])
+def test_batch_tile_schedule_edges(gpu, case, oracle_mod):
+    """More than 16 samples (several LDS passes per tile), views wider than a workgroup's
+    1024-tile binning window with large Gaussians (direct binning), and a resolve tile with
+    more distinct winners than the batched resolve's 2048-slot shading table (the winners it
+    cannot take are shaded by their pixel), each against the scatter schedule and the plain
+    SPP loop."""
     import torch
     from StochasticGSCudaBackend.batch import BatchRenderer, BatchTuning, BatchView
     if case.startswith("many_samples"):
         arrays = small_scene(2000, seed=45, log_scale_mean=-3.3, log_scale_std=0.8)
         cams, spps, frames = [small_camera(72, 40, angle=0.4), small_camera(33, 47, angle=1.9)], [20, 17], [9, 4000]
+    elif case == "shade_table_overflow":
+        # one 16x16 image tile, 4x4 subpixels, 40 samples of 100k tiny Gaussians
+        arrays = small_scene(100000, seed=47, log_scale_mean=-5.0, log_scale_std=0.5)
+        cams, spps, frames = [small_camera(16, 16, angle=0.3, supersampling_factor=4)], [40], [500]
+        sc = oracle_scene(arrays)
+        fb = np.stack([oracle_mod.accumulate(sc, cams[0], 500 + k, n_threads=oracle_mod.n_threads_available())[0]
+                       for k in range(40)])
+        won = fb[(fb >> np.uint64(32)) != np.uint64(0xFFFFFFFF)] & np.uint64(0xFFFFFFFF)
+        assert len(np.unique(won)) > 2 * 2048  # the oracle's winners: 4198 distinct
     else:
         arrays = small_scene(1500, seed=46, log_scale_mean=-2.6, log_scale_std=0.6)
         cams = [small_camera(640, 520, angle=0.2), small_camera(300, 200, angle=2.5, supersampling_factor=3)]
