@@ -580,15 +580,19 @@ def bench_plain(args, scene, views, settings, k_buffers, S):
 
 
 def bench_batched(args, scene, views, S, plain_out0, dev):
-    """Every view and sample in one persistent launch; stage timing with HIP events on the
-    library's stream; graph replay; bit comparison with the plain outputs of the same frames."""
+    """Every view and sample in one batched render (or, when the autotuner picks it, in view
+    groups rendered by concurrent batches on their own streams); stage timing with HIP events
+    on the library's stream of a single batch; graph replay; bit comparison with the plain
+    outputs of the same frames."""
     import numpy as np
     import torch
-    from StochasticGSCudaBackend.batch import SCHEDULE_SCATTER, SCHEDULE_TILES, BatchRenderer, BatchTuning, BatchView
+    from StochasticGSCudaBackend.batch import (SCHEDULE_SCATTER, SCHEDULE_TILES, BatchRenderer, BatchTuning, BatchView,
+                                               StreamedBatchRenderer)
     bviews = [BatchView(host_settings(cam), spp=cam["spp"], frame_idx=0) for cam in views]
     tuning = BatchTuning(schedule=SCHEDULE_TILES if args.schedule == "tiles" else SCHEDULE_SCATTER,
                          tile_samples=args.tile_samples, view_major=int(args.view_major))
     tuned = None
+    streams, part_tunings = 1, None
     if not args.no_autotune:
         from StochasticGSCudaBackend.autotune import tune_batch
         ts = time.perf_counter()
@@ -596,26 +600,34 @@ def bench_batched(args, scene, views, S, plain_out0, dev):
                          repeats=2)
         tuning = res.best._replace(view_major=int(args.view_major))
         bviews = res.views
+        streams, part_tunings = res.streams, res.part_tunings
         args.schedule = "tiles" if tuning.schedule in (0, SCHEDULE_TILES) else "scatter"
         args.tile_samples = tuning.tile_samples or 8
-        tuned = {"best": dict(tuning._asdict()), "view_bin_sizes": sorted({v.bin_size for v in bviews}),
+        tuned = {"best": dict(tuning._asdict()), "streams": streams,
+                 "part_max_tile_instances": [int(t.max_tile_instances) for t in part_tunings] if part_tunings else None,
+                 "view_bin_sizes": sorted({v.bin_size for v in bviews}),
                  "seconds": round(time.perf_counter() - ts, 1),
                  "trials": [[dict(c), round(ms, 3)] for c, ms in res.trials]}
         log(f"autotuned batch: {tuned['best']} from {tuned['trials']}")
     br = BatchRenderer(scene["means"], scene["scales"], scene["rotations"], scene["opacities"], scene["sh"], bviews,
                        tuning)
     log(f"batched workspace {br.workspace.numel() / 2**30:.2f} GiB")
-    max_diff = None
-    if plain_out0 is not None:
-        br.set_frames(frames_for_step(views, 0))
-        br.upload_table()
-        br.render()
+
+    def diff_vs_plain(r):
+        if plain_out0 is None:
+            return None
+        r.set_frames(frames_for_step(views, 0))
+        r.upload_table()
+        r.render()
         torch.cuda.synchronize()
-        max_diff = 0.0
-        for a, b in zip(br.outputs, plain_out0):
+        d = 0.0
+        for a, b in zip(r.outputs, plain_out0):
             for x, y in zip(a, b):
-                max_diff = max(max_diff, float((x - y).abs().max()))
-        log(f"batched vs plain (same frames): max abs difference {max_diff}")
+                d = max(d, float((x - y).abs().max()))
+        return d
+
+    max_diff = diff_vs_plain(br)
+    log(f"batched vs plain (same frames): max abs difference {max_diff}")
     stream = torch.cuda.current_stream(dev)
     for w in range(args.warmup):
         br.set_frames(frames_for_step(views, 2_000_000 + 10_000 * w))
@@ -644,17 +656,43 @@ def bench_batched(args, scene, views, S, plain_out0, dev):
              "resolve_ms": round(float(np.mean([e5[3].elapsed_time(e5[4]) for e5 in ev])), 3)}
     n_inst, n_visible, overflowed = br.tile_stats()
     log(f"stages {stage}; tile instances {n_inst} (overflow {overflowed})")
-    graph = br.capture()
+    single = {"value": round(S * args.steps / batch_s / 1e6, 2), "ms_per_step": round(batch_s * 1e3 / args.steps, 2)}
+    head = br
+    if streams > 1:
+        # the headline steps: the view groups as concurrent batches on their own streams
+        del br
+        torch.cuda.empty_cache()
+        head = StreamedBatchRenderer(scene["means"], scene["scales"], scene["rotations"], scene["opacities"],
+                                     scene["sh"], bviews, part_tunings or tuning, streams)
+        log(f"{streams} view groups {[len(g) for g in head.groups]}, workspace {head.workspace_bytes / 2**30:.2f} GiB")
+        max_diff = diff_vs_plain(head)
+        log(f"streamed batches vs plain (same frames): max abs difference {max_diff}")
+        for w in range(args.warmup):
+            head.set_frames(frames_for_step(views, 2_000_000 + 10_000 * w))
+            head.upload_table()
+            head.render()
+        torch.cuda.synchronize()
+        t_all = time.perf_counter()
+        for k in range(args.steps):
+            ts = time.perf_counter()
+            head.set_frames(frames_for_step(views, 10_000 * k))
+            head.upload_table()
+            head.render()
+            torch.cuda.synchronize()
+            log(f"streamed batched step {k + 1}/{args.steps}: {(time.perf_counter() - ts) * 1e3:.1f} ms")
+        batch_s = time.perf_counter() - t_all
+    graph = head.capture()
     torch.cuda.synchronize()
     g_ms = []
     for k in range(args.steps):
         ts = time.perf_counter()
-        br.set_frames(frames_for_step(views, 3_000_000 + 10_000 * k))
-        br.upload_table()
+        head.set_frames(frames_for_step(views, 3_000_000 + 10_000 * k))
+        head.upload_table()
         graph.replay()
         torch.cuda.synchronize()
         g_ms.append((time.perf_counter() - ts) * 1e3)
     return {"value": round(S * args.steps / batch_s / 1e6, 2), "ms_per_step": round(batch_s * 1e3 / args.steps, 2),
+            "streams": streams, "single_batch": single,
             "graph_ms_per_step": round(float(np.mean(g_ms)), 2), "stages": stage,
             "tile_instances": n_inst, "visible_pairs": n_visible, "tile_overflow": overflowed, "autotune": tuned,
             "max_abs_diff_vs_plain": max_diff}, main_ms

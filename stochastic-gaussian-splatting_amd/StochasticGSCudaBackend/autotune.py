@@ -11,10 +11,12 @@ median. The max_gps_points cap does change results and is never tuned.
   binning pixel and point thresholds, max_binned_gaussians and the sequential threshold
   (Tuning), timing the accumulation stage (sgs_accumulate).
 * ``tune_batch`` tunes a batched render: schedule and samples per LDS pass of the tile
-  schedule (BatchTuning) and the views' bin sizes (16-pixel bins for framebuffers up to a
+  schedule (BatchTuning), the views' bin sizes (16-pixel bins for framebuffers up to a
   pixel-count threshold, 32 above; ``TuneResult.views`` holds the views with the chosen bin
-  sizes), timing whole renders; the instance capacity is set from the measured instance
-  count with a margin, which shrinks the workspace.
+  sizes) and the number of view groups rendered on concurrent streams
+  (StreamedBatchRenderer; ``TuneResult.streams``, with one BatchTuning per group in
+  ``TuneResult.part_tunings``), timing whole renders; each batch's instance capacity is set
+  from its measured instance count with a margin, which shrinks the workspace.
 """
 from __future__ import annotations
 
@@ -34,6 +36,8 @@ class TuneResult:
     ms: float
     trials: list = field(default_factory=list)  # (config, ms) in evaluation order
     views: list | None = None  # tune_batch: the views with the chosen bin sizes
+    streams: int = 1  # tune_batch: view groups on concurrent streams (1: one BatchRenderer)
+    part_tunings: list | None = None  # tune_batch with streams > 1: one BatchTuning per group
 
 
 def coordinate_search(evaluate: Callable[[dict], float], space: dict[str, Sequence], start: dict,
@@ -129,8 +133,10 @@ def tune_render(means, scales, rotations, opacities, sh_coefficients, settings: 
 
 
 # small_bin_max_pixels: views whose framebuffer has at most this many pixels get 16-pixel
-# bins, the others 32 (-1: each view's own bin_size, by default the library's choice)
-BATCH_SPACE = {"schedule": [1, 2], "tile_samples": [8, 16], "small_bin_max_pixels": [-1, 0, 1 << 18, 1 << 21]}
+# bins, the others 32 (-1: each view's own bin_size, by default the library's choice);
+# streams: view groups rendered by separate batches on concurrent streams
+BATCH_SPACE = {"schedule": [1, 2], "tile_samples": [8, 16], "small_bin_max_pixels": [-1, 0, 1 << 18, 1 << 21],
+               "streams": [1, 2]}
 
 
 def views_with_bins(views, small_bin_max_pixels: int) -> list:
@@ -150,30 +156,40 @@ def views_with_bins(views, small_bin_max_pixels: int) -> list:
 def tune_batch(means, scales, rotations, opacities, sh_coefficients, views, *, repeats: int = 2,
                space: dict | None = None, capacity_margin: float = 1.25) -> TuneResult:
     """Fastest BatchTuning for rendering ``views`` (BatchView list) in one call; the tile
-    instance capacity of the result is the measured instance count x ``capacity_margin``."""
-    from .batch import BatchRenderer, BatchTuning
+    instance capacity of the result (of each group's, with streams > 1) is the measured
+    instance count x ``capacity_margin``."""
+    from .batch import BatchRenderer, BatchTuning, StreamedBatchRenderer
     space = dict(BATCH_SPACE if space is None else space)
 
-    def split(cfg: dict) -> tuple:
+    def renderer(cfg: dict, tuning=None):
         cfg = dict(cfg)
         vs = views_with_bins(views, cfg.pop("small_bin_max_pixels", -1))
-        return BatchTuning(**cfg), vs
+        streams = int(cfg.pop("streams", 1))
+        tuning = BatchTuning(**cfg) if tuning is None else tuning
+        if streams > 1:
+            return StreamedBatchRenderer(means, scales, rotations, opacities, sh_coefficients, vs, tuning, streams)
+        return BatchRenderer(means, scales, rotations, opacities, sh_coefficients, vs, tuning)
 
     def evaluate(cfg: dict) -> float:
-        tuning, vs = split(cfg)
-        r = BatchRenderer(means, scales, rotations, opacities, sh_coefficients, vs, tuning)
+        r = renderer(cfg)
         ms = _time_ms(r.render, repeats)
         del r
         return ms
 
     best, ms, trials = coordinate_search(evaluate, space, {k: v[0] for k, v in space.items()}, passes=1)
     best_views = views_with_bins(views, best.get("small_bin_max_pixels", -1))
-    best = {k: v for k, v in best.items() if k != "small_bin_max_pixels"}
-    if best.get("schedule", 1) == 1:
-        probe = BatchRenderer(means, scales, rotations, opacities, sh_coefficients, best_views, BatchTuning(**best))
+    streams = int(best.get("streams", 1))
+    tuning = BatchTuning(**{k: v for k, v in best.items() if k not in ("small_bin_max_pixels", "streams")})
+    part_tunings = None
+    if tuning.schedule in (0, 1):
+        probe = renderer(best, tuning)
         probe.render()
-        n_inst, _, overflowed = probe.tile_stats()
+        stats = probe.part_tile_stats() if streams > 1 else [probe.tile_stats()]
         del probe
-        if not overflowed and n_inst > 0:
-            best["max_tile_instances"] = min(int(n_inst * capacity_margin) + 1, (1 << 31))
-    return TuneResult(BatchTuning(**best), ms, [(dict(c), t) for c, t in trials], best_views)
+        caps = [tuning._replace(max_tile_instances=min(int(n * capacity_margin) + 1, (1 << 31)))
+                if not o and n > 0 else tuning for n, _, o in stats]
+        if streams > 1:
+            part_tunings = caps
+        else:
+            tuning = caps[0]
+    return TuneResult(tuning, ms, [(dict(c), t) for c, t in trials], best_views, streams, part_tunings)

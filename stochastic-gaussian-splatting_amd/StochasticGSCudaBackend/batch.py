@@ -184,6 +184,99 @@ class BatchRenderer:
         return graph
 
 
+def split_views(views: Sequence[BatchView], groups: int) -> list[list[int]]:
+    """min(groups, len(views)) non-empty contiguous groups of view indices, each boundary placed
+    where the framebuffer pixel-samples before it are closest to its share of the total."""
+    prefix = [0]
+    for v in views:
+        ss = max(1, int(v.settings.supersampling_factor))
+        prefix.append(prefix[-1] + int(v.settings.width) * ss * int(v.settings.height) * ss * int(v.spp))
+    n = len(views)
+    g = max(1, min(int(groups), n))
+    bounds = [0]
+    for k in range(1, g):
+        target = prefix[n] * k / g
+        bounds.append(min(range(bounds[-1] + 1, n - (g - k) + 1), key=lambda b: abs(prefix[b] - target)))
+    bounds.append(n)
+    return [list(range(bounds[k], bounds[k + 1])) for k in range(g)]
+
+
+class StreamedBatchRenderer:
+    """The views as ``streams`` contiguous groups (split_views), each rendered by its own
+    BatchRenderer (own workspace) on its own HIP stream of default priority, forked from and
+    joined to the caller's current stream: one group's binning passes (memory bound) run beside
+    another group's rasteriser (VALU bound). A view's result depends only on the view, so the
+    outputs equal one BatchRenderer over all views bit for bit, and frames taken from the
+    process-wide counter are drawn in view order as there. ``tuning`` is one BatchTuning for
+    every group or a list with one per group (e.g. each group's instance capacity)."""
+
+    def __init__(self, means, scales, rotations, opacities, sh_coefficients, views: Sequence[BatchView],
+                 tuning: BatchTuning | Sequence[BatchTuning] | None = None, streams: int = 2):
+        import torch
+        if streams < 1:
+            raise ValueError("streams must be at least 1")
+        self.device = means.device
+        self.views = list(views)
+        self.groups = split_views(self.views, streams)
+        tunings = [tuning] * len(self.groups) if tuning is None or isinstance(tuning, BatchTuning) else list(tuning)
+        if len(tunings) != len(self.groups):
+            raise ValueError(f"{len(tunings)} tunings for {len(self.groups)} view groups")
+        self.parts = [BatchRenderer(means, scales, rotations, opacities, sh_coefficients,
+                                    [self.views[i] for i in g], t) for g, t in zip(self.groups, tunings)]
+        self.streams = [torch.cuda.Stream(device=self.device) for _ in self.parts]
+        self.outputs: list[History] = [None] * len(self.views)  # type: ignore[list-item]
+        for g, r in zip(self.groups, self.parts):
+            for j, i in enumerate(g):
+                self.outputs[i] = r.outputs[j]
+
+    @property
+    def workspace_bytes(self) -> int:
+        return sum(r.workspace.numel() for r in self.parts)
+
+    def set_frames(self, frames: Sequence[int | None]) -> None:
+        """As BatchRenderer.set_frames, per view of the whole list."""
+        for g, r in zip(self.groups, self.parts):
+            r.set_frames([frames[i] for i in g])
+
+    def upload_table(self) -> None:
+        for r in self.parts:
+            r.upload_table()
+
+    def render(self) -> list[History]:
+        import torch
+        cur = torch.cuda.current_stream(self.device)
+        for s in self.streams:
+            s.wait_stream(cur)
+        for r, s in zip(self.parts, self.streams):
+            with torch.cuda.stream(s):
+                r.render()
+        for s in self.streams:
+            cur.wait_stream(s)
+        return self.outputs
+
+    def tile_stats(self) -> tuple[int, int, bool]:
+        """Summed over the groups (BatchRenderer.tile_stats)."""
+        stats = [r.tile_stats() for r in self.parts]
+        return sum(x[0] for x in stats), sum(x[1] for x in stats), any(x[2] for x in stats)
+
+    def part_tile_stats(self) -> list[tuple[int, int, bool]]:
+        return [r.tile_stats() for r in self.parts]
+
+    def capture(self) -> "torch.cuda.CUDAGraph":
+        """Captures render() (every group's stream, forked from and joined to the capturing
+        stream) into one graph."""
+        import torch
+        graph = torch.cuda.CUDAGraph()
+        side = torch.cuda.Stream(device=self.device)
+        side.wait_stream(torch.cuda.current_stream(self.device))
+        with torch.cuda.stream(side):
+            self.render()  # warm-up on the capture stream
+        torch.cuda.current_stream(self.device).wait_stream(side)
+        with torch.cuda.graph(graph):
+            self.render()
+        return graph
+
+
 def render_batch(means, scales, rotations, opacities, sh_coefficients, views: Sequence[BatchView],
                  tuning: BatchTuning | None = None) -> list[History]:
     """Renders every view (all of its samples) in one persistent launch; fresh outputs."""

@@ -57,3 +57,31 @@ def test_views_with_bins_threshold():
     # the other fields are kept and the input views are not modified
     assert all(o.spp == 2 and o.frame_idx == 7 and o.settings is v.settings for o, v in zip(out, views))
     assert [v.bin_size for v in views] == [0, 0, 0, 64]
+
+
+def test_split_views_contiguous_balanced_groups():
+    # min(groups, views) non-empty contiguous groups covering every view once, each boundary
+    # where the pixel-samples before it come closest to its share (checked by brute force)
+    import itertools
+
+    import torch
+    from StochasticGSCudaBackend.batch import BatchView, split_views
+    from StochasticGSCudaBackend.render import Settings
+
+    def view(w, h, spp, ss=1):
+        s = Settings(torch.eye(4), torch.zeros(3), w, h, 100.0, 100.0, w / 2, h / 2, 0.2, 100.0, False, ss, False,
+                     False, False)
+        return BatchView(s, spp=spp)
+
+    views = [view(256, 256, 1), view(640, 480, 4), view(1920, 1080, 16), view(100, 80, 2, ss=2), view(1280, 720, 1)]
+    cost = [256 * 256, 640 * 480 * 4, 1920 * 1080 * 16, 200 * 160 * 2, 1280 * 720]
+    for k in range(1, 8):
+        groups = split_views(views, k)
+        assert len(groups) == min(k, len(views)) and all(groups)
+        assert [i for g in groups for i in g] == list(range(len(views)))
+        if len(groups) == 2:
+            best = min(range(1, len(views)), key=lambda b: abs(sum(cost[:b]) - sum(cost) / 2))
+            assert groups[0] == list(range(best))
+    assert split_views(views[:2], 2) == [[0], [1]]  # the larger view second still gets its own group
+    assert split_views(views[:1], 3) == [[0]]
+    assert list(itertools.chain(*split_views(views, 1))) == list(range(5))

@@ -252,6 +252,56 @@ def test_batch_graph_replay(gpu):
             assert_close(a, b)
 
 
+@pytest.mark.parametrize("streams", [2, 3])
+def test_streamed_batches_equal_one_batch(gpu, streams):
+    """Views split into groups rendered by concurrent batches on their own streams give one
+    batch's bits, eagerly and replayed from a graph (with eager work between replays), and
+    frames taken from the process counter are drawn in view order as by one batch."""
+    import torch
+    from StochasticGSCudaBackend.batch import BatchRenderer, BatchView, StreamedBatchRenderer, split_views
+    arrays = small_scene(3000, seed=48, log_scale_mean=-3.3, log_scale_std=0.8)
+    t = to_device(arrays)
+    cams, spps, frames = make_views()
+    views = [BatchView(settings_of(c), spp=n, frame_idx=f) for c, n, f in zip(cams, spps, frames)]
+    groups = split_views(views, streams)
+    assert [i for g in groups for i in g] == list(range(len(views))) and len(groups) == streams
+    print_case(test="streamed_batches", streams=streams, groups=[len(g) for g in groups])
+    one = BatchRenderer(t["means"], t["scales"], t["rotations"], t["opacities"], t["sh"], views)
+    split = StreamedBatchRenderer(t["means"], t["scales"], t["rotations"], t["opacities"], t["sh"], views,
+                                  streams=streams)
+    one.render()
+    split.render()
+    torch.cuda.synchronize()
+    for a, b in zip(split.outputs, one.outputs):
+        assert_close(a, b)
+    graph = split.capture()
+    for step in range(3):
+        new = [f + 1000 * (step + 1) for f in frames]
+        split.set_frames(new)
+        split.upload_table()
+        graph.replay()
+        one.set_frames(new)
+        one.upload_table()
+        one.render()
+        torch.cuda.synchronize()
+        _ = (t["means"] * 2.0).sum().item()  # unrelated eager work between replays
+        for a, b in zip(split.outputs, one.outputs):
+            assert_close(a, b)
+    # frames from the counter: consecutive spp-long runs in view order, as for one batch
+    auto = [BatchView(settings_of(c), spp=n) for c, n in zip(cams, spps)]
+    drawn_one = [int(v.frame_idx) for v in BatchRenderer(t["means"], t["scales"], t["rotations"], t["opacities"],
+                                                         t["sh"], auto).c_views[:len(auto)]]
+    r = StreamedBatchRenderer(t["means"], t["scales"], t["rotations"], t["opacities"], t["sh"], auto,
+                              streams=streams)
+    drawn = [0] * len(auto)
+    for g, part in zip(r.groups, r.parts):
+        for j, i in enumerate(g):
+            drawn[i] = int(part.c_views[j].frame_idx)
+    steps = [drawn_one[i + 1] - drawn_one[i] for i in range(len(auto) - 1)]
+    assert steps == spps[:-1]
+    assert [drawn[i + 1] - drawn[i] for i in range(len(auto) - 1)] == steps
+
+
 def test_batch_rejects_unsupported_views(gpu):
     from StochasticGSCudaBackend.batch import BatchView, render_batch
     arrays = small_scene(10)
