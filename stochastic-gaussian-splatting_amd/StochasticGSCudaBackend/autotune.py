@@ -136,7 +136,7 @@ def tune_render(means, scales, rotations, opacities, sh_coefficients, settings: 
 # bins, the others 32 (-1: each view's own bin_size, by default the library's choice);
 # streams: view groups rendered by separate batches on concurrent streams
 BATCH_SPACE = {"schedule": [1, 2], "tile_samples": [8, 16], "small_bin_max_pixels": [-1, 0, 1 << 18, 1 << 21],
-               "streams": [1, 2]}
+               "streams": [1, 2, 4]}
 
 
 def views_with_bins(views, small_bin_max_pixels: int) -> list:

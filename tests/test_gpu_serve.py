@@ -61,7 +61,9 @@ def test_mix_equals_alone(gpu, max_views, inflight, chain, burst):
                     chain=chain, arena_bytes=256 << 20, keep_outputs=True)
     done = server.run(reqs)
     assert len(done) == len(reqs) and all(r.output is not None for r in done)
-    if max_views > 1:
+    if max_views > 1 and burst:
+        # with every request queued before the first dispatch, batches of several form for
+        # certain; with spread arrivals that depends on render speed against the wall clock
         assert max(server.batch_sizes) > 1, "no batching happened"
     by_key = {(r.stream, r.frame): r for r in done}
     for s in range(len(sizes)):
