@@ -5,7 +5,9 @@ View v's result equals ``spp`` successive ``render()`` calls with frames
 ``frame_idx .. frame_idx + spp - 1``, the first with ``temporal_mode`` and the rest STATIC
 (the SPP loop of the reference's Renderer.py:85-87), with identical framebuffer winners.
 ``BatchRenderer`` keeps the workspace, outputs and the encoded view table resident so a
-render can be repeated or captured into a graph and replayed.
+render can be repeated or captured into a graph and replayed. ``StreamedBatchRenderer``
+renders contiguous groups of the views as concurrent batches on their own streams, with the
+same outputs.
 """
 from __future__ import annotations
 
