@@ -246,6 +246,28 @@ def render(scene: OracleScene, cam: dict, frame_idx: int, history=None, temporal
     return img, dep, alp, wm, fb.reshape(FH, FW), st.as_dict()
 
 
+def shade_resolve(scene: OracleScene, cam: dict, framebuffer: np.ndarray, history=None, temporal_mode: int = 0,
+                  view_to_previous_view=None):
+    """Shading and resolve of a given packed framebuffer (sample_cu + resolve_cu without
+    spatial reuse): (image, depth, alpha, weight_mass)."""
+    W, H = int(cam["width"]), int(cam["height"])
+    FW, FH = fb_dims(cam)
+    fb = np.ascontiguousarray(np.asarray(framebuffer, np.uint64).reshape(-1))
+    assert fb.size == FW * FH
+    c = make_camera(cam)
+    c.spatial_reuse = 0
+    samples = np.zeros(FW * FH * 4, np.float32)
+    lib().orc_sample(C.byref(scene.c), C.byref(c), fb, samples, None, None, None)
+    outs = [np.zeros((H, W, 3), np.float32)] + [np.zeros((H, W, 1), np.float32) for _ in range(3)]
+    hist = [None] * 4 if history is None else [np.ascontiguousarray(h, np.float32) for h in history]
+    v2p = np.ascontiguousarray(np.eye(4, dtype=np.float32) if view_to_previous_view is None
+                               else np.asarray(view_to_previous_view, np.float32))
+    rc = lib().orc_resolve(C.byref(c), samples, None, *[_ptr(h) for h in hist], _ptr(v2p), int(temporal_mode), *outs)
+    if rc != 0:
+        raise ValueError(f"invalid temporal mode {temporal_mode}")
+    return tuple(outs)
+
+
 def blend_fp64(scene: OracleScene, cam: dict):
     W, H = int(cam["width"]), int(cam["height"])
     img = np.zeros((H, W, 3), np.float64)

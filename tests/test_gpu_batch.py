@@ -208,10 +208,15 @@ def test_batch_history_chain(gpu):
         assert_close(outs[k], hist)
 
 
-def test_batch_matches_oracle(gpu, oracle_mod):
+@pytest.mark.parametrize("sh_degree", [0, 1, 2, 3])
+def test_batch_matches_oracle(gpu, oracle_mod, sh_degree):
+    """The batched resolve shades with every SH degree the reference evaluates
+    (total_sh_bases 1, 4, 9, 16: gs_utils.h:212-243)."""
     import torch
     from StochasticGSCudaBackend.batch import BatchView, render_batch
-    arrays = small_scene(1500, seed=43, log_scale_mean=-3.4)
+    arrays = small_scene(1500, seed=43, log_scale_mean=-3.4, sh_degree=sh_degree)
+    assert arrays["sh"].shape[1] == (sh_degree + 1) ** 2
+    print_case(test="batch_oracle", sh_degree=sh_degree)
     sc, t = oracle_scene(arrays), to_device(arrays)
     cams = [small_camera(48, 40, angle=0.1), small_camera(40, 30, angle=1.0, supersampling_factor=2)]
     views = [BatchView(settings_of(c), spp=3, frame_idx=300 + 10 * i) for i, c in enumerate(cams)]

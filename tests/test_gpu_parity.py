@@ -184,6 +184,45 @@ def test_render_temporal_modes_match_oracle(gpu, oracle_mod, ss, gps):
     _close(got, ref[:4])
 
 
+@pytest.mark.parametrize("sh_degree", [0, 1, 2])
+def test_render_sh_degrees_match_oracle(gpu, oracle_mod, sh_degree):
+    """Shading with SH degrees 0-2 (total_sh_bases 1, 4, 9; gs_utils.h:212-243,
+    render.cu:150): the fused shade+resolve (RESET, STATIC), the separate sample + resolve
+    (REPROJECT) and the spatial-reuse sample kernel, against the oracle."""
+    import torch
+    arrays = small_scene(1500, seed=29, log_scale_mean=-3.5, sh_degree=sh_degree)
+    assert arrays["sh"].shape[1] == (sh_degree + 1) ** 2
+    sc, t = oracle_scene(arrays), to_device(arrays)
+    cam = small_camera(80, 60, supersampling_factor=2 if sh_degree == 1 else 1)
+    hist = None
+    for k, frame in enumerate((3, 4)):
+        print_case(test="sh_degree", sh_degree=sh_degree, mode=min(k, 1), frame=frame)
+        ref = oracle_mod.render(sc, cam, frame, history=hist, temporal_mode=min(k, 1))
+        got, fb = _gpu_render(gpu, t, cam, frame, hist=hist, mode=min(k, 1))
+        assert np.array_equal(fb, ref[4])
+        _close(got, ref[:4])
+        hist = ref[:4]
+    cam2 = small_camera(80, 60, angle=0.03, supersampling_factor=2 if sh_degree == 1 else 1)
+    v2p = (np.asarray(cam["w2c"], np.float64) @ np.linalg.inv(np.asarray(cam2["w2c"], np.float64))).astype(np.float32)
+    print_case(test="sh_degree_reproject", sh_degree=sh_degree)
+    ref = oracle_mod.render(sc, cam2, 5, history=hist, temporal_mode=2, view_to_previous_view=v2p)
+    got, fb = _gpu_render(gpu, t, cam2, 5, hist=hist, mode=2, v2p=v2p)
+    assert np.array_equal(fb, ref[4])
+    _close(got, ref[:4])
+    cam3 = small_camera(64, 48, spatial_reuse=True)
+    kb_o = oracle_mod.empty_k_buffer(cam3)
+    kb_g = tuple(torch.from_numpy(k.copy()).cuda() for k in kb_o)
+    hist = None
+    for k, frame in enumerate((6, 7)):
+        print_case(test="sh_degree_reuse", sh_degree=sh_degree, frame=frame)
+        ref = oracle_mod.render(sc, cam3, frame, history=hist, temporal_mode=min(k, 1), k_buffer=kb_o)
+        got, fb = _gpu_render(gpu, t, cam3, frame, hist=hist, mode=min(k, 1), kb=kb_g)
+        assert np.array_equal(fb, ref[4])
+        _close(got, ref[:4])
+        assert np.array_equal(kb_g[0].cpu().numpy(), kb_o[0])
+        hist = ref[:4]
+
+
 def test_spatial_reuse_matches_oracle(gpu, oracle_mod):
     import torch
     arrays = small_scene(1500, seed=26, log_scale_mean=-3.4)
