@@ -136,6 +136,26 @@ int sgs_render(const sgs_gaussians* gaussians, const sgs_settings* settings,
                const sgs_history* out, uint32_t frame_idx, const sgs_tuning* tuning,
                void* workspace, size_t workspace_bytes, void* stream);
 
+/* No counterpart in the reference. sgs_render for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the
+ * same arguments and outputs as sgs_render without the workspace. It calls no hipMalloc,
+ * hipFree or allocating library routine; its framebuffer (and the per-subpixel samples and
+ * reuse weights REPROJECT and spatial reuse read) live in a fixed device array of
+ * SGS_NOALLOC_SCRATCH_BYTES that the library holds on every GPU it runs on until the process
+ * ends. A view needing more (8 bytes per framebuffer pixel, +16 with REPROJECT or spatial
+ * reuse, +16 more with spatial reuse: 1080p with spatial reuse fits, and 4K in RESET or STATIC) returns
+ * SGS_ERROR_WORKSPACE_TOO_SMALL before anything is enqueued. It accumulates with the scatter
+ * schedule (accumulate_scatter.cu:19-77, no binning buffers), whose framebuffer equals the
+ * tile and hybrid schedules' bit for bit. Every call on one GPU shares that GPU's array:
+ * calls that overlap in time on one GPU (other streams or threads) race; calls ordered on one
+ * stream do not. */
+#define SGS_NOALLOC_SCRATCH_BYTES (128u * 1024u * 1024u)
+int sgs_render_noalloc(const sgs_gaussians* gaussians, const sgs_settings* settings,
+                       const uint8_t* blue_noise, uint64_t blue_noise_numel,
+                       int32_t* k_buffer_primitives, float* k_buffer_occlusion,
+                       const sgs_history* history, const float* view_to_previous_view, int32_t temporal_mode,
+                       const sgs_history* out, uint32_t frame_idx, const sgs_tuning* tuning, void* stream);
+
 /* No reference counterpart (the reference only binds render): the accumulation stage
  * of sgs_render alone, into a caller framebuffer of (H*ss)*(W*ss) uint64 words that it
  * clears first. counters, if not NULL, is a device uint64[4] that receives

@@ -81,6 +81,11 @@ _lib.sgs_render.restype = C.c_int
 _lib.sgs_render.argtypes = [_P(sgs_gaussians), _P(sgs_settings), C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p,
                             _P(sgs_history), C.c_void_p, C.c_int32, _P(sgs_history), C.c_uint32, _P(sgs_tuning),
                             C.c_void_p, C.c_size_t, C.c_void_p]
+_lib.sgs_render_noalloc.restype = C.c_int
+_lib.sgs_render_noalloc.argtypes = [_P(sgs_gaussians), _P(sgs_settings), C.c_void_p, C.c_uint64, C.c_void_p,
+                                    C.c_void_p, _P(sgs_history), C.c_void_p, C.c_int32, _P(sgs_history), C.c_uint32,
+                                    _P(sgs_tuning), C.c_void_p]
+NOALLOC_SCRATCH_BYTES = 128 * 1024 * 1024  # SGS_NOALLOC_SCRATCH_BYTES of include/sgs.h
 _lib.sgs_accumulate_workspace_size.restype = C.c_size_t
 _lib.sgs_accumulate_workspace_size.argtypes = [_P(sgs_settings), C.c_uint32, _P(sgs_tuning)]
 _lib.sgs_accumulate.restype = C.c_int

@@ -2,6 +2,7 @@
 // workspace carve-up and stage dispatch. Replaces the host dispatcher and pybind11
 // binding of render.cu:16-165.
 #include <cmath>
+#include <mutex>
 #include <cstring>
 #include <algorithm>
 #include <string>
@@ -268,6 +269,109 @@ int check_blue_noise(const uint8_t* bn, uint64_t numel)
     return SGS_SUCCESS;
 }
 
+int check_render_args(const sgs_gaussians* gaussians, const sgs_settings* s, const uint8_t* blue_noise,
+                      uint64_t blue_noise_numel, const int32_t* k_buffer_primitives, const float* k_buffer_occlusion,
+                      const sgs_history* history, const float* view_to_previous_view, int32_t temporal_mode,
+                      const sgs_history* out, const sgs_tuning* tuning)
+{
+    int rc;
+    if ((rc = check_gaussians(gaussians)) || (rc = check_settings(s)) || (rc = check_tuning(tuning)) ||
+        (rc = check_blue_noise(blue_noise, blue_noise_numel)))
+        return rc;
+    if (temporal_mode < SGS_TEMPORAL_RESET || temporal_mode > SGS_TEMPORAL_REPROJECT)
+        return fail(SGS_ERROR_INVALID_ARGUMENT, "invalid temporal mode " + std::to_string(temporal_mode));
+    if (!out || !out->image || !out->depth || !out->alpha || !out->weight_mass)
+        return fail(SGS_ERROR_INVALID_ARGUMENT, "output pointers are NULL");
+    if (temporal_mode != SGS_TEMPORAL_RESET &&
+        (!history || !history->image || !history->depth || !history->alpha || !history->weight_mass))
+        return fail(SGS_ERROR_INVALID_ARGUMENT, "STATIC and REPROJECT need a history");
+    if (temporal_mode == SGS_TEMPORAL_REPROJECT && !view_to_previous_view)
+        return fail(SGS_ERROR_INVALID_ARGUMENT, "REPROJECT needs view_to_previous_view");
+    if (s->spatial_reuse && (!k_buffer_primitives || !k_buffer_occlusion))
+        return fail(SGS_ERROR_INVALID_ARGUMENT, "spatial reuse needs the k-buffer");
+    if (history && temporal_mode != SGS_TEMPORAL_RESET &&
+        (history->image == out->image || history->depth == out->depth || history->alpha == out->alpha ||
+         history->weight_mass == out->weight_mass))
+        return fail(SGS_ERROR_INVALID_ARGUMENT, "outputs must not alias the history");
+    return SGS_SUCCESS;
+}
+
+// shading of each (sub)pixel's winner and the resolve into the outputs (resolve.cu:355-444):
+// fused when neither spatial reuse nor REPROJECT needs the per-subpixel samples
+int shade_and_resolve(const sgs_gaussians* g, const sgs_settings* s, const FbCamera& cam, const uint64_t* fb,
+                      float4* samples, float4* reuse_weights, int32_t* k_buffer_primitives, float* k_buffer_occlusion,
+                      const sgs_history* history, const float* view_to_previous_view, int32_t temporal_mode,
+                      const sgs_history* out, hipStream_t stream)
+{
+    const SceneArgs sc = scene_args(g);
+    HistoryPtrs hp = { nullptr, nullptr, nullptr, nullptr };
+    if (history) hp = { history->image, history->depth, history->alpha, history->weight_mass };
+    const OutputPtrs op = { out->image, out->depth, out->alpha, out->weight_mass };
+    hipError_t e;
+    if (!s->spatial_reuse && temporal_mode != SGS_TEMPORAL_REPROJECT) {
+        e = launch_shade_resolve(sc, s->cam_position, cam, s->width, s->height, fb, hp, op, temporal_mode, stream);
+        if (e != hipSuccess) return hip_fail(e, "shade_resolve");
+        return SGS_SUCCESS;
+    }
+    float4* rw = s->spatial_reuse ? reuse_weights : nullptr;
+    e = launch_sample(sc, s->w2c, s->cam_position, cam, fb, samples, rw, reinterpret_cast<uint2*>(k_buffer_primitives),
+                      reinterpret_cast<float4*>(k_buffer_occlusion), s->spatial_reuse != 0, stream);
+    if (e != hipSuccess) return hip_fail(e, "sample");
+    e = launch_resolve(cam, s->width, s->height, s->focal_x, s->focal_y, s->center_x, s->center_y, samples, rw, hp,
+                       view_to_previous_view, op, s->spatial_reuse != 0, temporal_mode, stream);
+    if (e != hipSuccess) return hip_fail(e, "resolve");
+    return SGS_SUCCESS;
+}
+
+// ---- fixed device scratch of sgs_render_noalloc ----------------------------------------
+// One array per GPU, allocated with the code object when the library is loaded on that GPU
+// and held until the process ends. Host code never passes the array's name to a kernel (on
+// the host it is a stand-in, not a device address): each device's address is taken once,
+// with that device current, through hipGetSymbolAddress and kept in g_scratch_ptr. Calls
+// on one GPU share its array, so calls that overlap in time on one GPU race.
+static __device__ __align__(256) unsigned char scratch[SGS_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+char* g_scratch_ptr[kMaxDevices] = {};
+
+int device_scratch(char** out)
+{
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    if (dev < 0 || dev >= kMaxDevices) return fail(SGS_ERROR_INVALID_ARGUMENT, "device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+        if (!p || reinterpret_cast<uintptr_t>(p) % 256u != 0)
+            return fail(SGS_ERROR_HIP, "scratch array has no 256-byte aligned device address");
+        g_scratch_ptr[dev] = static_cast<char*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return SGS_SUCCESS;
+}
+
+struct ScratchLayout {
+    size_t fb, samples, reuse, total;
+};
+
+// framebuffer words, and the per-subpixel samples / reuse weights only where the resolve reads them
+ScratchLayout scratch_layout(const sgs_settings* s, const FbCamera& cam, int32_t temporal_mode)
+{
+    const size_t P = (size_t)cam.width_i * cam.height_i;
+    Arena a;
+    ScratchLayout L;
+    L.fb = a.take(8u * P);
+    const bool need_samples = s->spatial_reuse || temporal_mode == SGS_TEMPORAL_REPROJECT;
+    L.samples = need_samples ? a.take(16u * P) : 0;
+    L.reuse = s->spatial_reuse ? a.take(16u * P) : 0;
+    L.total = a.offset;
+    return L;
+}
+
 }  // namespace
 
 extern "C" {
@@ -293,24 +397,9 @@ int sgs_render(const sgs_gaussians* gaussians, const sgs_settings* s, const uint
                size_t workspace_bytes, void* stream_ptr)
 {
     int rc;
-    if ((rc = check_gaussians(gaussians)) || (rc = check_settings(s)) || (rc = check_tuning(tuning)) ||
-        (rc = check_blue_noise(blue_noise, blue_noise_numel)))
+    if ((rc = check_render_args(gaussians, s, blue_noise, blue_noise_numel, k_buffer_primitives, k_buffer_occlusion,
+                                history, view_to_previous_view, temporal_mode, out, tuning)))
         return rc;
-    if (temporal_mode < SGS_TEMPORAL_RESET || temporal_mode > SGS_TEMPORAL_REPROJECT)
-        return fail(SGS_ERROR_INVALID_ARGUMENT, "invalid temporal mode " + std::to_string(temporal_mode));
-    if (!out || !out->image || !out->depth || !out->alpha || !out->weight_mass)
-        return fail(SGS_ERROR_INVALID_ARGUMENT, "output pointers are NULL");
-    if (temporal_mode != SGS_TEMPORAL_RESET &&
-        (!history || !history->image || !history->depth || !history->alpha || !history->weight_mass))
-        return fail(SGS_ERROR_INVALID_ARGUMENT, "STATIC and REPROJECT need a history");
-    if (temporal_mode == SGS_TEMPORAL_REPROJECT && !view_to_previous_view)
-        return fail(SGS_ERROR_INVALID_ARGUMENT, "REPROJECT needs view_to_previous_view");
-    if (s->spatial_reuse && (!k_buffer_primitives || !k_buffer_occlusion))
-        return fail(SGS_ERROR_INVALID_ARGUMENT, "spatial reuse needs the k-buffer");
-    if (history && temporal_mode != SGS_TEMPORAL_RESET &&
-        (history->image == out->image || history->depth == out->depth || history->alpha == out->alpha ||
-         history->weight_mass == out->weight_mass))
-        return fail(SGS_ERROR_INVALID_ARGUMENT, "outputs must not alias the history");
 
     const Tuning t = resolve_tuning(tuning, gaussians->n_gaussians);
     const FbCamera cam = make_fb_camera(s->width, s->height, s->focal_x, s->focal_y, s->center_x, s->center_y,
@@ -328,26 +417,46 @@ int sgs_render(const sgs_gaussians* gaussians, const sgs_settings* s, const uint
     if ((rc = accumulate_into(gaussians, s, cam, blue_noise, blue_noise_numel, fb, frame_idx, t, nullptr,
                               L.hybrid ? ws + L.hybrid : nullptr, L.tiles ? ws + L.tiles : nullptr, stream)))
         return rc;
+    return shade_and_resolve(gaussians, s, cam, fb, reinterpret_cast<float4*>(ws + L.samples),
+                             s->spatial_reuse ? reinterpret_cast<float4*>(ws + L.reuse) : nullptr, k_buffer_primitives,
+                             k_buffer_occlusion, history, view_to_previous_view, temporal_mode, out, stream);
+}
 
-    const SceneArgs sc = scene_args(gaussians);
-    HistoryPtrs hp = { nullptr, nullptr, nullptr, nullptr };
-    if (history) hp = { history->image, history->depth, history->alpha, history->weight_mass };
-    const OutputPtrs op = { out->image, out->depth, out->alpha, out->weight_mass };
-    hipError_t e;
-    if (!s->spatial_reuse && temporal_mode != SGS_TEMPORAL_REPROJECT) {
-        e = launch_shade_resolve(sc, s->cam_position, cam, s->width, s->height, fb, hp, op, temporal_mode, stream);
-        if (e != hipSuccess) return hip_fail(e, "shade_resolve");
-    } else {
-        float4* samples = reinterpret_cast<float4*>(ws + L.samples);
-        float4* rw = s->spatial_reuse ? reinterpret_cast<float4*>(ws + L.reuse) : nullptr;
-        e = launch_sample(sc, s->w2c, s->cam_position, cam, fb, samples, rw, reinterpret_cast<uint2*>(k_buffer_primitives),
-                          reinterpret_cast<float4*>(k_buffer_occlusion), s->spatial_reuse != 0, stream);
-        if (e != hipSuccess) return hip_fail(e, "sample");
-        e = launch_resolve(cam, s->width, s->height, s->focal_x, s->focal_y, s->center_x, s->center_y, samples, rw, hp,
-                           view_to_previous_view, op, s->spatial_reuse != 0, temporal_mode, stream);
-        if (e != hipSuccess) return hip_fail(e, "resolve");
-    }
-    return SGS_SUCCESS;
+int sgs_render_noalloc(const sgs_gaussians* gaussians, const sgs_settings* s, const uint8_t* blue_noise,
+                       uint64_t blue_noise_numel, int32_t* k_buffer_primitives, float* k_buffer_occlusion,
+                       const sgs_history* history, const float* view_to_previous_view, int32_t temporal_mode,
+                       const sgs_history* out, uint32_t frame_idx, const sgs_tuning* tuning, void* stream_ptr)
+{
+    int rc;
+    if ((rc = check_render_args(gaussians, s, blue_noise, blue_noise_numel, k_buffer_primitives, k_buffer_occlusion,
+                                history, view_to_previous_view, temporal_mode, out, tuning)))
+        return rc;
+    const FbCamera cam = make_fb_camera(s->width, s->height, s->focal_x, s->focal_y, s->center_x, s->center_y,
+                                        s->near_plane, s->far_plane, s->proper_antialiasing, s->supersampling_factor);
+    const ScratchLayout L = scratch_layout(s, cam, temporal_mode);
+    if (L.total > SGS_NOALLOC_SCRATCH_BYTES)
+        return fail(SGS_ERROR_WORKSPACE_TOO_SMALL,
+                    "sgs_render_noalloc: this view needs " + std::to_string(L.total) + " bytes of scratch, more than "
+                    "the library's fixed " + std::to_string((size_t)SGS_NOALLOC_SCRATCH_BYTES) +
+                    " (8 bytes per framebuffer pixel, +16 with REPROJECT or spatial reuse, +16 more with spatial "
+                    "reuse); use sgs_render with a workspace");
+    char* ws = nullptr;
+    if ((rc = device_scratch(&ws))) return rc;
+
+    // the scatter schedule of the reference (no binning buffers): the same framebuffer as the
+    // tile and hybrid schedules, bit for bit
+    sgs_settings scatter = *s;
+    scatter.hybrid_scheduling = 0;
+    Tuning t = resolve_tuning(tuning, gaussians->n_gaussians);
+    t.tiles = false;
+    hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
+    uint64_t* fb = reinterpret_cast<uint64_t*>(ws + L.fb);
+    if ((rc = accumulate_into(gaussians, &scatter, cam, blue_noise, blue_noise_numel, fb, frame_idx, t, nullptr,
+                              nullptr, nullptr, stream)))
+        return rc;
+    return shade_and_resolve(gaussians, s, cam, fb, L.samples ? reinterpret_cast<float4*>(ws + L.samples) : nullptr,
+                             L.reuse ? reinterpret_cast<float4*>(ws + L.reuse) : nullptr, k_buffer_primitives,
+                             k_buffer_occlusion, history, view_to_previous_view, temporal_mode, out, stream);
 }
 
 size_t sgs_accumulate_workspace_size(const sgs_settings* settings, uint32_t n_gaussians, const sgs_tuning* tuning)

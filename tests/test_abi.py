@@ -82,6 +82,27 @@ def test_host_side_validation(native):
     assert _render(native, g, s, ws_bytes=16) == native.SGS_ERROR_WORKSPACE_TOO_SMALL
 
 
+def test_noalloc_rejects_views_beyond_scratch(native):
+    """sgs_render_noalloc refuses, on the host, a view whose framebuffer (and samples) exceed
+    its fixed scratch; nothing is enqueued."""
+    lib = native.library()
+    g = _gaussians(native)
+    o = native.sgs_history(0x10000, 0x20000, 0x30000, 0x40000)
+    for w, h, ss, reuse, mode in ((4096, 4100, 1, 0, 0), (2560, 1440, 1, 1, 0), (1920, 1080, 2, 0, 2)):
+        s = _settings(native, width=w, height=h, supersampling_factor=ss, spatial_reuse=reuse)
+        kp = 0x50000 if reuse else None
+        v2p = 0x60000 if mode == 2 else None
+        hist = native.sgs_history(0x70000, 0x80000, 0x90000, 0xA0000) if mode else None
+        need = w * h * ss * ss * (8 + (16 if reuse or mode == 2 else 0) + (16 if reuse else 0))
+        assert need > native.NOALLOC_SCRATCH_BYTES
+        rc = lib.sgs_render_noalloc(C.byref(g), C.byref(s), None, 0, kp, kp, C.byref(hist) if hist else None, v2p,
+                                    mode, C.byref(o), 0, None, None)
+        assert rc == native.SGS_ERROR_WORKSPACE_TOO_SMALL and b"scratch" in lib.sgs_last_error()
+    # argument errors are the ones sgs_render reports
+    assert lib.sgs_render_noalloc(C.byref(g), C.byref(_settings(native)), None, 0, None, None, None, None, 3,
+                                  C.byref(o), 0, None, None) == native.SGS_ERROR_INVALID_ARGUMENT
+
+
 def test_workspace_sizes(native):
     lib = native.library()
     s = _settings(native)
