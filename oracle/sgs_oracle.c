@@ -22,9 +22,14 @@
  *   orc_accumulate (GPS)             point counts against lambda, winners inside the image,
  *                                    point density against the Gaussian [test_oracle_accumulate]
  *   orc_resolve                      box filter = subpixel mean, STATIC = running mean (closed form),
- *                                    REPROJECT with identity motion on a constant field [test_oracle_resolve]
- *   orc_sample spatial reuse         parity unpinned beyond invariants (weights >= 0, k-buffer
- *                                    insertion order, no reuse on an empty k-buffer) [test_oracle_resolve]
+ *                                    REPROJECT with identity motion on a constant field [test_oracle_resolve];
+ *                                    REPROJECT under motion: whole-pixel translations read one known texel,
+ *                                    checkerboard moments fix gamma(velocity), linear-ramp histories are
+ *                                    fetched at the FP64 reprojected position [test_oracle_reuse_motion]
+ *   orc_sample spatial reuse         weights = FP64 alpha ratios of the winner, k-buffer occlusion product,
+ *                                    insertion and eviction, neighbour components in the resolve
+ *                                    [test_oracle_reuse_motion]
+ *   everything                       ASan + UBSan build over every entry point [test_oracle_sanitize]
  */
 #include "sgs_oracle.h"
 

@@ -31,7 +31,7 @@ def load_camera_path(path, camera: PerspectiveCamera | None = None, render_width
     size. A missing file gives a warning and no views, like the reference."""
     path_file = Path(path)
     if not path_file.is_file():
-        warnings.warn(f'skipping camera path loading: "{path_file}" does not exist')
+        warnings.warn(f"no camera path file at {path_file}; returning no views")
         return []
     with open(path_file, "r") as f:
         camera_path = json.load(f)

@@ -21,22 +21,34 @@ from ._native import Tuning
 
 
 class Settings(NamedTuple):
-    """Camera and feature flags of one render call (render.py:8-42)."""
-    w2c: torch.Tensor  # affine transformation from model/world space to view space
-    cam_position: torch.Tensor  # camera position in world space
-    width: int  # width of the image plane in pixels
-    height: int  # height of the image plane in pixels
-    focal_x: float  # focal length in x direction in pixels
-    focal_y: float  # focal length in y direction in pixels
-    center_x: float  # x coordinate of the image center in pixels (positive -> right)
-    center_y: float  # y coordinate of the image center in pixels (positive -> down)
-    near_plane: float  # near clipping plane distance
-    far_plane: float  # far clipping plane distance
-    proper_antialiasing: bool  # whether to use proper antialiasing
-    supersampling_factor: int  # factor for supersampling (2 -> render 2x2 subpixels)
-    use_gps: bool  # whether to use Gaussian point splatting
-    hybrid_scheduling: bool  # whether to process large Gaussians in a separate load-balanced pass
-    spatial_reuse: bool  # whether to resample from the four neighbors of each (sub)pixel
+    """Camera and feature flags of one render call (render.py:8-42).
+
+    Conventions: ``w2c`` is the (4, 4) row-major matrix taking world points into the camera
+    frame (+x right, +y down, +z forward, OpenCV style); ``cam_position`` is the camera centre
+    in world coordinates (the viewing direction for SH shading starts there); sizes and
+    intrinsics are in pixels of the output image, the principal point measured from the top
+    left corner; geometry outside [near_plane, far_plane] of view depth is culled.
+    ``proper_antialiasing`` uses the smaller screen-space dilation with opacity compensation
+    instead of the plain dilation; ``supersampling_factor`` s renders s x s framebuffer cells
+    per output pixel and box-filters them; ``use_gps`` splats Poisson-distributed points
+    instead of per-fragment stochastic transparency; ``hybrid_scheduling`` moves Gaussians with
+    large footprints to a tile-binned pass; ``spatial_reuse`` adds the four edge neighbours'
+    samples, weighted by their winners' footprints, to every framebuffer cell."""
+    w2c: torch.Tensor
+    cam_position: torch.Tensor
+    width: int
+    height: int
+    focal_x: float
+    focal_y: float
+    center_x: float
+    center_y: float
+    near_plane: float
+    far_plane: float
+    proper_antialiasing: bool
+    supersampling_factor: int
+    use_gps: bool
+    hybrid_scheduling: bool
+    spatial_reuse: bool
 
     def as_tuple(self) -> tuple:
         return (
@@ -59,16 +71,23 @@ class Settings(NamedTuple):
 
 
 class TemporalMode(IntEnum):
-    RESET = 0  # discard the history
-    STATIC = 1  # same camera as the history: weighted cumulative average
-    REPROJECT = 2  # moving camera: reprojected history with soft rejection
+    """How a sample is merged with the previous result (resolve.h:11-15): RESET ignores it;
+    STATIC (camera unchanged) keeps a running average weighted by sample mass; REPROJECT
+    (camera moved) fetches it at each pixel's reprojected position and down-weights it where
+    it disagrees with the new sample's neighbourhood."""
+    RESET = 0
+    STATIC = 1
+    REPROJECT = 2
 
 
 class History(NamedTuple):
-    image: torch.Tensor  # (H, W, 3) color premultiplied by alpha
-    depth: torch.Tensor  # (H, W, 1) depth premultiplied by alpha
-    alpha: torch.Tensor  # (H, W, 1)
-    weight_mass: torch.Tensor  # (H, W, 1) accumulated resampling weight
+    """The resolved image and what accumulating more samples into it needs: colour (H, W, 3)
+    and depth (H, W, 1), both multiplied by coverage; coverage (H, W, 1); and the summed
+    sample weight per pixel (H, W, 1)."""
+    image: torch.Tensor
+    depth: torch.Tensor
+    alpha: torch.Tensor
+    weight_mass: torch.Tensor
 
 
 @cache
@@ -83,7 +102,9 @@ def blue_noise() -> torch.Tensor:
 
 
 def create_k_buffer(settings: Settings) -> tuple[torch.Tensor, torch.Tensor]:
-    """Empty k-buffers: primitive index and depth, occlusion factors towards the four neighbors."""
+    """Fresh spatial-reuse state for ``settings``: per framebuffer cell, k_buffer_size slots of
+    (Gaussian index, depth bits) as int32 pairs, all -1 (no entry), and of four float32
+    occlusion ratios, one per edge neighbour; both have zero slots without spatial reuse."""
     n_entries = (_C.k_buffer_size if settings.spatial_reuse else 0,
                  settings.width * settings.height * settings.supersampling_factor ** 2)
     return (

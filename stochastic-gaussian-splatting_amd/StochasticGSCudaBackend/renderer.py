@@ -173,7 +173,8 @@ class StochasticGSRenderer:
         self.history_c2w = view.c2w_numpy
         self.history_w2c = view.w2c
 
-        # history is premultiplied by alpha and must not be modified in place
+        # the history tensors are the next call's input (colour and depth carry a factor of
+        # coverage): composite into new tensors, never in place
         image, depth, alpha, _ = self.history
         alpha = alpha.clamp(0.0, 1.0)
         bg = torch.as_tensor(view.camera.background_color, dtype=image.dtype, device=image.device)

@@ -196,6 +196,46 @@ def test_renderer_state_machine(gpu):
         assert_close(r.history, h)
 
 
+def test_renderer_composites_over_background(gpu):
+    """Renderer.py:91-103 on a coloured background: rgb = clamp(image + (1 - clamp(alpha)) bg),
+    depth = depth / max(alpha, 1e-8), alpha clamped, the history left as rendered; the
+    benchmark form returns the composited rgb only."""
+    import torch
+    from StochasticGSCudaBackend.model import StochasticGSModel
+    from StochasticGSCudaBackend.renderer import PerspectiveCamera, StochasticGSRenderer, View
+    from StochasticGSCudaBackend.synthetic import look_at, orbit_eye
+    arrays = small_scene(1500, seed=9, log_scale_mean=-3.4)
+    t = to_device(arrays)
+    model = StochasticGSModel.from_arrays(t["means"], t["scales"], t["rotations"], t["opacities"], t["sh"])
+    bg = (0.2, 0.5, 0.9)
+    f = 24.0 / np.tan(np.radians(30.0))
+    cam = PerspectiveCamera(64, 48, f, f, background_color=bg)
+    view = View.from_w2c(cam, look_at(orbit_eye(0.3, 3.0)), frame_idx=0)
+    r = StochasticGSRenderer(model, SAMPLES_PER_PIXEL=3)
+    r.frame_idx = 40
+    print_case(test="composite")
+    out = r.render_image(view)
+    torch.cuda.synchronize()
+    ca = {"w2c": view.w2c.numpy(), "cam_position": view.position.numpy(), "width": 64, "height": 48,
+          "focal_x": f, "focal_y": f, "center_x": 32.0, "center_y": 24.0, "near_plane": 0.2, "far_plane": 10000.0}
+    h = [x.cpu().numpy().astype(np.float64) for x in plain_spp(gpu, t, ca, 40, 3)]
+    assert_close(r.history, plain_spp(gpu, t, ca, 40, 3))       # the history is not modified in place
+    a = np.clip(h[2], 0.0, 1.0)
+    rgb = np.clip(h[0] + (1.0 - a) * np.asarray(bg), 0.0, 1.0)
+    depth = h[1] / np.maximum(a, 1e-8)
+    assert 0.05 < a.mean() < 0.95 and (a == 0).any()           # covered and uncovered pixels
+    np.testing.assert_allclose(out["rgb"].cpu().numpy(), rgb, atol=1e-6)
+    np.testing.assert_allclose(out["alpha"].cpu().numpy(), a, atol=0)
+    np.testing.assert_allclose(out["depth"].cpu().numpy(), depth, rtol=1e-6, atol=1e-6)
+    bgpix = h[2][..., 0] == 0
+    np.testing.assert_allclose(out["rgb"].cpu().numpy()[bgpix], np.broadcast_to(bg, (int(bgpix.sum()), 3)), atol=1e-7)
+    assert np.all(out["depth"].cpu().numpy()[bgpix] == 0.0)
+    chw = r.render_image(view, to_chw=True)
+    assert chw["rgb"].shape == (3, 48, 64) and chw["depth"].shape == (1, 48, 64)
+    bench = r.render_image(view, benchmark=True)
+    assert set(bench) == {"rgb"}
+
+
 def test_settings_change_resets(gpu):
     from StochasticGSCudaBackend import TemporalMode
     from StochasticGSCudaBackend.model import StochasticGSModel
