@@ -14,6 +14,9 @@ CSRC = Path(__file__).resolve().parent
 PKG = CSRC.parent
 REPO = PKG.parent
 OUT = PKG / "StochasticGSCudaBackend" / "libsgs.so"
+# measurement kernels (copy ceiling, random atomicMin rate) for bench.py, a separate library
+MICROBENCH_OUT = PKG / "StochasticGSCudaBackend" / "libsgs_microbench.so"
+MICROBENCH_SOURCES = ["sgs_microbench.hip"]
 SOURCES = ["sgs_api.hip", "sgs_accumulate.hip", "sgs_batch.hip", "sgs_hybrid.hip", "sgs_resolve.hip", "sgs_scene.hip"]
 ARCH = os.environ.get("SGS_OFFLOAD_ARCH", "gfx950")
 
@@ -35,13 +38,18 @@ def flags() -> list[str]:
     ]
 
 
-def build(verbose: bool = False) -> Path:
-    cmd = [hipcc(), *flags(), *[str(CSRC / s) for s in SOURCES], "-o", str(OUT) + ".tmp"]
+def _compile(sources: list[str], out: Path, verbose: bool) -> Path:
+    cmd = [hipcc(), *flags(), *[str(CSRC / s) for s in sources], "-o", str(out) + ".tmp"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
-    os.replace(str(OUT) + ".tmp", OUT)
-    return OUT
+    os.replace(str(out) + ".tmp", out)
+    return out
+
+
+def build(verbose: bool = False) -> Path:
+    _compile(MICROBENCH_SOURCES, MICROBENCH_OUT, verbose)
+    return _compile(SOURCES, OUT, verbose)
 
 
 if __name__ == "__main__":

@@ -14,9 +14,35 @@ REPO = Path(__file__).resolve().parents[1]
 HEADER = REPO / "include" / "sgs.h"
 
 
-def declared_functions() -> list[str]:
-    text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
+def declared_functions(header: Path = HEADER) -> list[str]:
+    text = re.sub(r"/\*.*?\*/", "", header.read_text(), flags=re.S)
     return sorted(set(re.findall(r"\b(sgs_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_every_header_is_exported_by_its_library():
+    """include/sgs.h by libsgs.so, include/sgs_microbench.h by libsgs_microbench.so; no other
+    header exists."""
+    import ctypes
+    pkg = REPO / "stochastic-gaussian-splatting_amd" / "StochasticGSCudaBackend"
+    libs = {"sgs.h": pkg / "libsgs.so", "sgs_microbench.h": pkg / "libsgs_microbench.so"}
+    headers = sorted(p.name for p in (REPO / "include").glob("*.h"))
+    assert headers == sorted(libs)
+    import torch  # noqa: F401  (loads the HIP runtime the libraries bind to)
+    for h, lib_path in libs.items():
+        lib = ctypes.CDLL(str(lib_path))
+        names = declared_functions(REPO / "include" / h)
+        assert names
+        for name in names:
+            assert hasattr(lib, name), f"{lib_path.name} lacks {name} ({h})"
+
+
+def test_microbench_rejects_bad_arguments():
+    from StochasticGSCudaBackend import microbench
+    lib = microbench.library()
+    assert lib.sgs_mb_copy(0x1000, 0x2000, 100, None) == -1          # not a multiple of 16
+    assert lib.sgs_mb_copy(0x1000, 0x1010, 64, None) == -1           # overlap
+    assert b"overlap" in lib.sgs_mb_last_error()
+    assert lib.sgs_mb_atomic_min_u64(0x1000, 0, 10, 0, None) == -1   # empty buffer
 
 
 @pytest.fixture(scope="module")
