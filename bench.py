@@ -463,10 +463,11 @@ def kernel_table(profile, views, n_instances, visible_pairs, scene_n):
         "sgs::cov3d_kernel": 64.0 * scene_n,
         # 44 B of parameters per Gaussian, 8 B pair record written per (view, Gaussian)
         "sgs::batch_bin_kernel<false>": 44.0 * scene_n + 8.0 * pairs_all,
-        # the pair record read back, 12 B of means per Gaussian, one 8 B entry per instance
-        "sgs::batch_bin_kernel<true>": 8.0 * pairs_all + 12.0 * scene_n + 8.0 * n_instances,
-        # every entry read once, a 4 B sorted index written per instance
-        "sgs::tile_sort_kernel": 12.0 * n_instances,
+        # the pair record read back, 12 B of means per Gaussian, a 2 B depth bucket and a 4 B
+        # Gaussian index written per instance
+        "sgs::batch_bin_kernel<true>": 8.0 * pairs_all + 12.0 * scene_n + 6.0 * n_instances,
+        # every bucket and index read once, a 4 B sorted index written per instance
+        "sgs::tile_sort_kernel": 10.0 * n_instances,
     }
     raster_algo = 44.0 * visible_pairs + 8.0 * n_instances + 8.0 * words
     rows = []

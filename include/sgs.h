@@ -214,7 +214,7 @@ typedef struct sgs_batch_tuning {
     uint32_t blocks_per_cu;           /* persistent workgroups per CU (0: occupancy limit) */
     uint32_t schedule;                /* SGS_BATCH_TILES (default) or SGS_BATCH_SCATTER */
     uint32_t max_tile_instances;      /* capacity of the tile schedule's (view, tile, Gaussian)
-                                         list, 16 bytes of workspace each (default per Gaussian
+                                         list, 14 bytes of workspace each (default per Gaussian
                                          and view 4 + 8 P / 2^21 for a framebuffer of P pixels,
                                          within [2^16, 2^31]); a batch that needs more runs the
                                          scatter schedule */

@@ -171,7 +171,7 @@ class Server:
             free, _ = torch.cuda.mem_get_info(self.device)
             arena_bytes = int(0.5 * free) // inflight
         self.arena_bytes = int(arena_bytes) & ~255
-        # The tile schedule's instance list takes up to half of the arena (16 bytes per
+        # The tile schedule's instance list takes up to half of the arena (14 bytes per
         # instance) rather than the library's per-view estimate: an overflowing batch falls
         # back to the much slower scatter schedule, which would show as a latency spike.
         tuning = BatchTuning() if tuning is None else tuning

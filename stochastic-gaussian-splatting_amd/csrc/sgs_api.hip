@@ -91,7 +91,8 @@ namespace {
 
 // ---- single-view tile schedule (the batched renderer's tile pipeline for one view) ----
 struct SingleTileLayout {
-    size_t counters, ready, tile_count, tile_cursor, cleared_end, offsets, entries, sorted_instances, sorted_keys,
+    size_t counters, ready, tile_count, tile_cursor, cleared_end, offsets, entry_buckets, entry_gaussians,
+        sorted_instances, sorted_keys,
         centroids, view_keys, cov3d, pair_rect, table, queue, total;
     size_t views_bytes;
     uint32_t n_tiles, capacity, queue_capacity;
@@ -128,7 +129,8 @@ SingleTileLayout single_tile_layout(const FbCamera& cam, uint32_t n_gaussians, u
     L.cleared_end = a.offset;
     a.offset = (a.offset + 255u) & ~size_t(255u);
     L.offsets = a.take(4u * ((size_t)L.n_tiles + 1u));
-    L.entries = a.take(8u * (size_t)L.capacity);  // (depth key, Gaussian index) per instance
+    L.entry_buckets = a.take(2u * (size_t)L.capacity);    // depth bucket per instance
+    L.entry_gaussians = a.take(4u * (size_t)L.capacity);  // Gaussian index per instance
     L.sorted_instances = a.take(4u * (size_t)L.capacity);
     L.sorted_keys = a.take(4u * (size_t)L.capacity);
     L.centroids = a.take(16u * (((size_t)n_gaussians + 255u) / 256u));
@@ -522,7 +524,7 @@ BatchTuning resolve_batch_tuning(const sgs_batch_tuning* t)
 
 uint64_t view_fb_pixels(const sgs_view& v);
 
-// default capacity of the tile schedule's instance list (16 bytes of workspace each): per
+// default capacity of the tile schedule's instance list (14 bytes of workspace each): per
 // view, 4 + 8 P / 2^21 per Gaussian for a framebuffer of P pixels (a Gaussian covers more
 // 16x16 tiles the finer the framebuffer), as for a single view
 uint32_t default_tile_instances(const sgs_view* views, uint32_t n_views, uint32_t n_gaussians)
@@ -577,7 +579,8 @@ uint64_t view_raster_groups(const sgs_view& v)
 // Workspace of a batch. The counters, the queue's ready flags and the tile counts and
 // cursors are contiguous so one fill clears them all.
 struct BatchLayout {
-    size_t counters, ready, tile_count, tile_cursor, cleared_end, offsets, entries, sorted_instances, sorted_keys,
+    size_t counters, ready, tile_count, tile_cursor, cleared_end, offsets, entry_buckets, entry_gaussians,
+        sorted_instances, sorted_keys,
         centroids, cov3d, pair_rect, view_keys, queue, table, fb, samples, total;
     uint64_t fb_words, n_samples, n_tiles, n_raster;
     uint32_t n_blocks, capacity;
@@ -614,7 +617,8 @@ BatchLayout batch_layout(const sgs_view* views, uint32_t n, uint32_t n_gaussians
     L.cleared_end = a.offset;
     a.offset = (a.offset + 255u) & ~size_t(255u);
     L.offsets = a.take(4u * (tiles_n + 1u));
-    L.entries = a.take(8u * (size_t)L.capacity);  // (depth key, Gaussian index) per instance
+    L.entry_buckets = a.take(2u * (size_t)L.capacity);    // depth bucket per instance
+    L.entry_gaussians = a.take(4u * (size_t)L.capacity);  // Gaussian index per instance
     L.sorted_instances = a.take(4u * (size_t)L.capacity);
     L.sorted_keys = a.take(4u * (size_t)L.capacity);
     const size_t n_chunks = tiles ? ((size_t)n_gaussians + 255u) / 256u : 0u;
@@ -794,7 +798,8 @@ int accumulate_tiles_single(const sgs_gaussians* g, const sgs_settings* s, const
     tb.tile_count = reinterpret_cast<uint32_t*>(tile_ws + L.tile_count);
     tb.tile_cursor = reinterpret_cast<uint32_t*>(tile_ws + L.tile_cursor);
     tb.offsets = reinterpret_cast<uint32_t*>(tile_ws + L.offsets);
-    tb.entries = reinterpret_cast<uint2*>(tile_ws + L.entries);
+    tb.entries = { reinterpret_cast<uint16_t*>(tile_ws + L.entry_buckets),
+                   reinterpret_cast<uint32_t*>(tile_ws + L.entry_gaussians) };
     tb.sorted_instances = reinterpret_cast<uint32_t*>(tile_ws + L.sorted_instances);
     tb.sorted_keys = reinterpret_cast<uint32_t*>(tile_ws + L.sorted_keys);
     tb.centroids = reinterpret_cast<float4*>(tile_ws + L.centroids);
@@ -901,7 +906,8 @@ int sgs_render_batch(const sgs_gaussians* gaussians, const sgs_view* views, uint
         tb.tile_count = reinterpret_cast<uint32_t*>(ws + L.tile_count);
         tb.tile_cursor = reinterpret_cast<uint32_t*>(ws + L.tile_cursor);
         tb.offsets = reinterpret_cast<uint32_t*>(ws + L.offsets);
-        tb.entries = reinterpret_cast<uint2*>(ws + L.entries);
+        tb.entries = { reinterpret_cast<uint16_t*>(ws + L.entry_buckets),
+                       reinterpret_cast<uint32_t*>(ws + L.entry_gaussians) };
         tb.sorted_instances = reinterpret_cast<uint32_t*>(ws + L.sorted_instances);
         tb.sorted_keys = reinterpret_cast<uint32_t*>(ws + L.sorted_keys);
         tb.centroids = reinterpret_cast<float4*>(ws + L.centroids);

@@ -373,18 +373,37 @@ __device__ __forceinline__ bool gaussian_tiles_reg(float mx, float my, float mz,
     return true;
 }
 
+// Depth buckets of the per-bin sort (tile_sort_kernel): the fill pass stores each instance's
+// bucket, the sort counts and places by it.
+constexpr uint32_t kSortBins = 1024;
+constexpr uint32_t kSortInFlight = 16;  // entries per lane in flight in the two-pass path of larger tiles
+constexpr uint32_t kSortRegKeys = 32;  // entries per lane kept in registers for tiles of <= 8192
+
+__device__ __forceinline__ uint32_t sort_bucket(uint32_t key, uint2 vk)
+{
+    if (key < vk.x) return 0u;
+    return 1u + min((key - vk.x) >> vk.y, kSortBins - 2u);
+}
+
+__device__ __forceinline__ uint32_t bucket_floor(uint32_t b, uint2 vk)
+{
+    if (b == 0u) return 0u;
+    const uint64_t f = (uint64_t)vk.x + ((uint64_t)(b - 1u) << vk.y);
+    return f > 0xffffffffull ? 0xffffffffu : (uint32_t)f;
+}
+
 // Gaussians touching at most kSmallTiles tiles are counted through the workgroup's LDS
 // histogram (one global atomic per touched tile and workgroup); larger ones are spread over
 // their wave's lanes with one global atomic per tile.
 constexpr uint32_t kSmallTiles = 32;
 
 template <bool kFill>
-__device__ __forceinline__ void bin_direct(uint32_t gt, uint32_t g, uint32_t key, uint32_t* tile_count,
-                                           uint32_t* tile_cursor, const uint32_t* offsets, uint2* entries)
+__device__ __forceinline__ void bin_direct(uint32_t gt, uint32_t g, uint32_t bucket, uint32_t* tile_count,
+                                           uint32_t* tile_cursor, const uint32_t* offsets, const EntryArrays& entries)
 {
     if constexpr (kFill) {
         const uint32_t slot = offsets[gt] + atomicAdd(&tile_cursor[gt], 1u);
-        entries[slot] = make_uint2(key, g);
+        entries.store(slot, bucket, g);
     } else {
         atomicAdd(&tile_count[gt], 1u);
     }
@@ -440,7 +459,8 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
                                                                 uint32_t* __restrict__ tile_count,
                                                                 uint32_t* __restrict__ tile_cursor,
                                                                 const uint32_t* __restrict__ offsets,
-                                                                uint2* __restrict__ entries,
+                                                                EntryArrays entries,
+                                                                const uint2* __restrict__ view_keys,
                                                                 uint2* __restrict__ pair_rect,
                                                                 BatchCounters* __restrict__ ctr)
 {
@@ -501,10 +521,10 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
                 if (g < sc.n && v + 1u < n_views) next_rect = pair_rect[pair + sc.n];
                 has = r.x != kNoTiles;
                 tr = { r.x & 0xffffu, r.y & 0xffffu, r.x >> 16, r.y >> 16 };
-                // the depth key again from the mean in registers: the operations of
-                // gaussian_tiles_reg, so the same bits
+                // the depth key again from the mean in registers (the operations of
+                // gaussian_tiles_reg, so the same bits), stored as its sort bucket
                 const float4 r3 = vd.r3;
-                key = f2u(r3.x * mx + r3.y * my + r3.z * mz + r3.w);
+                key = sort_bucket(f2u(r3.x * mx + r3.y * my + r3.z * mz + r3.w), view_keys[v]);
             } else {
                 has = gaussian_tiles_reg(mx, my, mz, cv, valid, vd, kRasterLog2 + vd.bin_log, tr, key);
                 if (g < sc.n)
@@ -588,7 +608,7 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
                                 for (uint32_t tx = tr.tx0; tx <= tr.tx1; tx++) {
                                     const uint32_t i = ((ty - wy0) * wt + tx - wx0) * kHistCopies + hc;
                                     const uint32_t slot = hbase[i] + atomicAdd(&hist[i], 1u);
-                                    entries[slot] = make_uint2(key, g);
+                                    entries.store(slot, key, g);
                                 }
                     }
                 } else if (small) {
@@ -698,27 +718,11 @@ __global__ __launch_bounds__(kRangeThreads) void view_key_range_kernel(const Vie
 // sorted position 256 m (0, or kmin + (b - 1) << shift): a lower bound of the true keys from
 // that position on, read where the rasteriser's batches of 256 start; other positions of
 // keys_out are not written.
-constexpr uint32_t kSortBins = 1024;
-constexpr uint32_t kSortInFlight = 16;  // entries per lane in flight in the two-pass path of larger tiles
-constexpr uint32_t kSortRegKeys = 32;  // entries per lane kept in registers for tiles of <= 8192
-
-__device__ __forceinline__ uint32_t sort_bucket(uint32_t key, uint2 vk)
-{
-    if (key < vk.x) return 0u;
-    return 1u + min((key - vk.x) >> vk.y, kSortBins - 2u);
-}
-
-__device__ __forceinline__ uint32_t bucket_floor(uint32_t b, uint2 vk)
-{
-    if (b == 0u) return 0u;
-    const uint64_t f = (uint64_t)vk.x + ((uint64_t)(b - 1u) << vk.y);
-    return f > 0xffffffffull ? 0xffffffffu : (uint32_t)f;
-}
 
 __global__ __launch_bounds__(kBatchBlock) void tile_sort_kernel(uint32_t n_views, const uint32_t* __restrict__ tile_prefix,
                                                                 const uint2* __restrict__ view_keys,
                                                                 const uint32_t* __restrict__ offsets,
-                                                                const uint2* __restrict__ entries_in,
+                                                                const EntryArrays entries_in,
                                                                 uint32_t* __restrict__ keys_out,
                                                                 uint32_t* __restrict__ inst_out,
                                                                 const BatchCounters* __restrict__ ctr)
@@ -741,34 +745,31 @@ __global__ __launch_bounds__(kBatchBlock) void tile_sort_kernel(uint32_t n_views
     __syncthreads();
     // Tiles of up to kSortRegKeys * 256 instances (most of them) keep each lane's buckets and
     // instances in registers between the histogram and the scatter: every entry is read once.
+    // (two 16-bit buckets per register, so more workgroups fit a CU)
     const bool in_regs = n <= kSortRegKeys * kBatchBlock;
-    uint32_t rb[kSortRegKeys], rg[kSortRegKeys];
+    uint32_t rb[kSortRegKeys / 2u], rg[kSortRegKeys];
     if (in_regs) {
 #pragma unroll
         for (uint32_t u = 0; u < kSortRegKeys; u++) {
             const uint32_t idx = tid + u * kBatchBlock;
-            const uint2 e = idx < n ? entries_in[begin + idx] : make_uint2(0u, 0u);
-            rb[u] = e.x;
-            rg[u] = e.y;
+            const uint32_t b = idx < n ? entries_in.bucket[begin + idx] : 0u;
+            rb[u / 2u] = (u & 1u) ? rb[u / 2u] | (b << 16) : b;
+            rg[u] = idx < n ? entries_in.gaussian[begin + idx] : 0u;
         }
 #pragma unroll
-        for (uint32_t u = 0; u < kSortRegKeys; u++) {
-            if (tid + u * kBatchBlock < n) {
-                rb[u] = sort_bucket(rb[u], vk);
-                atomicAdd(&hist[rb[u]], 1u);
-            }
-        }
+        for (uint32_t u = 0; u < kSortRegKeys; u++)
+            if (tid + u * kBatchBlock < n) atomicAdd(&hist[(rb[u / 2u] >> (16u * (u & 1u))) & 0xffffu], 1u);
     } else {
-        // kSortInFlight keys in flight per lane
+        // kSortInFlight buckets in flight per lane; the first pass reads the 2-byte buckets only
         uint32_t i = tid;
         for (; i + (kSortInFlight - 1u) * kBatchBlock < n; i += kSortInFlight * kBatchBlock) {
             uint32_t k[kSortInFlight];
 #pragma unroll
-            for (uint32_t u = 0; u < kSortInFlight; u++) k[u] = entries_in[begin + i + u * kBatchBlock].x;
+            for (uint32_t u = 0; u < kSortInFlight; u++) k[u] = entries_in.bucket[begin + i + u * kBatchBlock];
 #pragma unroll
-            for (uint32_t u = 0; u < kSortInFlight; u++) atomicAdd(&hist[sort_bucket(k[u], vk)], 1u);
+            for (uint32_t u = 0; u < kSortInFlight; u++) atomicAdd(&hist[k[u]], 1u);
         }
-        for (; i < n; i += kBatchBlock) atomicAdd(&hist[sort_bucket(entries_in[begin + i].x, vk)], 1u);
+        for (; i < n; i += kBatchBlock) atomicAdd(&hist[entries_in.bucket[begin + i]], 1u);
     }
     __syncthreads();
     // exclusive scan of the buckets, 4 per thread
@@ -810,7 +811,8 @@ __global__ __launch_bounds__(kBatchBlock) void tile_sort_kernel(uint32_t n_views
     if (in_regs) {
 #pragma unroll
         for (uint32_t u = 0; u < kSortRegKeys; u++)
-            if (tid + u * kBatchBlock < n) inst_out[begin + atomicAdd(&hist[rb[u]], 1u)] = rg[u];
+            if (tid + u * kBatchBlock < n)
+                inst_out[begin + atomicAdd(&hist[(rb[u / 2u] >> (16u * (u & 1u))) & 0xffffu], 1u)] = rg[u];
         return;
     }
     uint32_t i = tid;
@@ -818,18 +820,14 @@ __global__ __launch_bounds__(kBatchBlock) void tile_sort_kernel(uint32_t n_views
         uint32_t k[kSortInFlight], g[kSortInFlight];
 #pragma unroll
         for (uint32_t u = 0; u < kSortInFlight; u++) {
-            const uint2 e = entries_in[begin + i + u * kBatchBlock];
-            k[u] = e.x;
-            g[u] = e.y;
+            k[u] = entries_in.bucket[begin + i + u * kBatchBlock];
+            g[u] = entries_in.gaussian[begin + i + u * kBatchBlock];
         }
 #pragma unroll
-        for (uint32_t u = 0; u < kSortInFlight; u++)
-            inst_out[begin + atomicAdd(&hist[sort_bucket(k[u], vk)], 1u)] = g[u];
+        for (uint32_t u = 0; u < kSortInFlight; u++) inst_out[begin + atomicAdd(&hist[k[u]], 1u)] = g[u];
     }
-    for (; i < n; i += kBatchBlock) {
-        const uint2 e = entries_in[begin + i];
-        inst_out[begin + atomicAdd(&hist[sort_bucket(e.x, vk)], 1u)] = e.y;
-    }
+    for (; i < n; i += kBatchBlock) inst_out[begin + atomicAdd(&hist[entries_in.bucket[begin + i]], 1u)] =
+        entries_in.gaussian[begin + i];
 }
 
 // inclusive scan of one value per thread over the 256-thread workgroup
@@ -1200,7 +1198,7 @@ hipError_t launch_batch_tiles(const SceneArgs& sc, const ViewDesc* views, uint32
         hipLaunchKernelGGL(cov3d_kernel, dim3(n_chunks), dim3(kBatchBlock), 0, stream, sc, tb.cov3d);
         hipLaunchKernelGGL((batch_bin_kernel<false>), dim3((unsigned)grid), dim3(kBatchBlock), 0, stream, sc, tb.cov3d,
                            views, n_views, tb.tile_count, tb.tile_cursor, tb.offsets,
-                           tb.entries, tb.pair_rect, counters);
+                           tb.entries, tb.view_keys, tb.pair_rect, counters);
     }
     hipLaunchKernelGGL((exclusive_scan_kernel<CountInput>), dim3(1), dim3(kScanThreads), 0, stream,
                        CountInput{ tb.tile_count }, tb.offsets, nullptr, tb.n_tiles, tb.n_tiles, &counters->n_instances,
@@ -1208,7 +1206,7 @@ hipError_t launch_batch_tiles(const SceneArgs& sc, const ViewDesc* views, uint32
     if (sc.n > 0) {
         hipLaunchKernelGGL((batch_bin_kernel<true>), dim3((unsigned)grid), dim3(kBatchBlock), 0, stream, sc, tb.cov3d,
                            views, n_views, tb.tile_count, tb.tile_cursor, tb.offsets,
-                           tb.entries, tb.pair_rect, counters);
+                           tb.entries, tb.view_keys, tb.pair_rect, counters);
     }
     hipLaunchKernelGGL(tile_sort_kernel, dim3(tb.n_tiles), dim3(kBatchBlock), 0, stream, n_views, tile_prefix,
                        tb.view_keys, tb.offsets, tb.entries, tb.sorted_keys, tb.sorted_instances, counters);

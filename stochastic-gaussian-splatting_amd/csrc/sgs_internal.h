@@ -196,11 +196,24 @@ hipError_t launch_batch_accumulate(const SceneArgs& sc, const ViewDesc* views, u
 // tile schedule: binning (count, scan, fill) and per-tile LDS rasterisation of every view's
 // samples; the scatter schedule above runs instead (gated on counters->overflow) when the
 // instance list overflows
+// Instances of the tile schedule, (view, bin, Gaussian) each: the depth bucket of the per-bin
+// sort (2 bytes, < 1024) and the Gaussian index (4 bytes) in separate arrays, so the sort's
+// counting pass reads only the buckets.
+struct EntryArrays {
+    uint16_t* bucket;
+    uint32_t* gaussian;
+    __device__ __forceinline__ void store(uint32_t slot, uint32_t b, uint32_t g) const
+    {
+        bucket[slot] = (uint16_t)b;
+        gaussian[slot] = g;
+    }
+};
+
 struct TileBuffers {
     uint32_t* tile_count;       // [n_tiles] instances per tile (zeroed by the caller)
     uint32_t* tile_cursor;      // [n_tiles] fill cursors (zeroed by the caller)
     uint32_t* offsets;          // [n_tiles + 1] exclusive scan of tile_count
-    uint2* entries;             // [capacity] (depth key, Gaussian index) of each (view, bin, Gaussian) instance
+    EntryArrays entries;        // [capacity] depth bucket and Gaussian index of each (view, bin, Gaussian) instance
     uint32_t* sorted_instances; // [capacity] each tile's instances, nearest depth bucket first
     uint32_t* sorted_keys;      // [capacity] depth bucket floor at each tile batch start (begin + 256 m)
     float4* centroids;          // [n_chunks] mean position of each chunk of 256 Gaussians
