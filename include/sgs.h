@@ -51,6 +51,10 @@ typedef struct sgs_gaussians {
     const float* sh_coefficients; /* (N,B,3) */
     uint32_t n_gaussians;
     uint32_t total_sh_bases;      /* B in {1,4,9,16} (render.cu:150) */
+    const float* cov3d;           /* optional (N,8) cache of the view-independent projection terms
+                                     from sgs_precompute_cov3d, 16-byte aligned; NULL: computed
+                                     from scales, rotations and opacities in every call. It must
+                                     be recomputed when those change. */
 } sgs_gaussians;
 
 /* Settings (render.py:8-42) in its as_tuple() order; w2c (4,4 row-major) and
@@ -105,7 +109,25 @@ typedef struct sgs_tuning {
     uint32_t bin_size;                /* tile schedule: edge in pixels of the framebuffer bins
                                          whose depth-sorted instance lists the 16x16 raster tiles
                                          read: 16, 32 or 64 (default 16 for one view) */
+    uint32_t max_workspace_mib;       /* cap, in MiB, on the workspace of the tile schedule: the
+                                         tile schedule keeps, per (bin, Gaussian) instance, 14
+                                         bytes (a depth-bucket sort of each bin's list, which the
+                                         sorting-free reference does not have) plus 48 bytes per
+                                         Gaussian; a view whose tile workspace would exceed the
+                                         cap takes the scatter schedule instead, which needs only
+                                         the framebuffer and per-subpixel samples, as the
+                                         reference's (README.md:6). 0: no cap. Identical results
+                                         either way. */
 } sgs_tuning;
+
+/* The load-time precompute the reference suggests (README.md:139-142): per Gaussian the
+ * sigmoid opacity and the 3D covariance R diag(exp(2 s)) R^T of the normalised quaternion
+ * (gs_utils.h:78-110), with log(255 opacity), as (N,8) float32 into the device array cov3d
+ * (16-byte aligned; a Gaussian culled in every view, by opacity below 1/255 or a degenerate
+ * quaternion, gets opacity -1). Pass it as sgs_gaussians.cov3d: every projection then reads
+ * these 32 bytes instead of recomputing them from 32 bytes of raw parameters, with the same
+ * floating-point operations, so renders are bit-identical with and without it. */
+int sgs_precompute_cov3d(const sgs_gaussians* gaussians, float* cov3d, void* stream);
 
 /* Replaces _C.k_buffer_size (render.cu:164). */
 int sgs_k_buffer_size(void);

@@ -32,7 +32,7 @@ SGS_ERROR_HIP = -3
 class sgs_gaussians(C.Structure):
     _fields_ = [("means", C.c_void_p), ("scales", C.c_void_p), ("rotations", C.c_void_p),
                 ("opacities", C.c_void_p), ("sh_coefficients", C.c_void_p),
-                ("n_gaussians", C.c_uint32), ("total_sh_bases", C.c_uint32)]
+                ("n_gaussians", C.c_uint32), ("total_sh_bases", C.c_uint32), ("cov3d", C.c_void_p)]
 
 
 class sgs_settings(C.Structure):
@@ -51,7 +51,7 @@ class sgs_tuning(C.Structure):
     _fields_ = [("sequential_threshold", C.c_uint32), ("binning_pixel_threshold", C.c_uint32),
                 ("binning_point_threshold", C.c_uint32), ("max_binned_gaussians", C.c_uint32),
                 ("tile_size", C.c_uint32), ("max_tile_instances", C.c_uint32), ("max_gps_points", C.c_uint32),
-                ("schedule", C.c_uint32), ("bin_size", C.c_uint32)]
+                ("schedule", C.c_uint32), ("bin_size", C.c_uint32), ("max_workspace_mib", C.c_uint32)]
 
 
 class sgs_view(C.Structure):
@@ -86,6 +86,8 @@ _lib.sgs_render_noalloc.argtypes = [_P(sgs_gaussians), _P(sgs_settings), C.c_voi
                                     C.c_void_p, _P(sgs_history), C.c_void_p, C.c_int32, _P(sgs_history), C.c_uint32,
                                     _P(sgs_tuning), C.c_void_p]
 NOALLOC_SCRATCH_BYTES = 128 * 1024 * 1024  # SGS_NOALLOC_SCRATCH_BYTES of include/sgs.h
+_lib.sgs_precompute_cov3d.restype = C.c_int
+_lib.sgs_precompute_cov3d.argtypes = [_P(sgs_gaussians), C.c_void_p, C.c_void_p]
 _lib.sgs_accumulate_workspace_size.restype = C.c_size_t
 _lib.sgs_accumulate_workspace_size.argtypes = [_P(sgs_settings), C.c_uint32, _P(sgs_tuning)]
 _lib.sgs_accumulate.restype = C.c_int
@@ -137,7 +139,9 @@ class Tuning(NamedTuple):
     """Scheduling knobs (sgs_tuning); 0 keeps the library default. Only max_gps_points changes
     results. schedule: 0 or 1 = the tile schedule for stochastic transparency without hybrid
     scheduling (blue noise and debug counters always scatter), 2 = the reference's scatter.
-    bin_size: the tile schedule's bin edge in pixels (16, 32 or 64; 0 = 16)."""
+    bin_size: the tile schedule's bin edge in pixels (16, 32 or 64; 0 = 16).
+    max_workspace_mib: above this tile-schedule workspace a view takes the scatter schedule
+    (0: render() applies default_workspace_cap_mib() of the device)."""
     sequential_threshold: int = 0
     binning_pixel_threshold: int = 0
     binning_point_threshold: int = 0
@@ -147,9 +151,27 @@ class Tuning(NamedTuple):
     max_gps_points: int = 0
     schedule: int = 0
     bin_size: int = 0
+    max_workspace_mib: int = 0
 
     def c(self) -> sgs_tuning:
         return sgs_tuning(*[int(v) for v in self])
+
+
+_caps: dict = {}
+
+
+def default_workspace_cap_mib(device: torch.device) -> int:
+    """A quarter of the device's memory: the most a single-view render() takes for the tile
+    schedule's per-call workspace before it falls back to the scatter schedule."""
+    idx = device.index if device.index is not None else torch.cuda.current_device()
+    if idx not in _caps:
+        _caps[idx] = max(1, torch.cuda.get_device_properties(idx).total_memory // 4 >> 20)
+    return _caps[idx]
+
+
+def _capped(tuning: "Tuning | None", device: torch.device) -> "Tuning":
+    t = Tuning() if tuning is None else tuning
+    return t if t.max_workspace_mib else t._replace(max_workspace_mib=default_workspace_cap_mib(device))
 
 
 def _check(rc: int, what: str) -> None:
@@ -180,7 +202,8 @@ def _f32(t: torch.Tensor, name: str, device: torch.device) -> torch.Tensor:
     return t.contiguous()
 
 
-def gaussians_struct(means, scales, rotations, opacities, sh_coefficients, keep: list) -> sgs_gaussians:
+def gaussians_struct(means, scales, rotations, opacities, sh_coefficients, keep: list,
+                     cov3d_cache: torch.Tensor | None = None) -> sgs_gaussians:
     device = means.device
     if device.type != "cuda":
         raise ValueError("renderer not implemented in CPU mode: tensors must be on a ROCm device")
@@ -195,7 +218,31 @@ def gaussians_struct(means, scales, rotations, opacities, sh_coefficients, keep:
     if sh.dim() != 3 or sh.shape[0] != n or sh.shape[2] != 3:
         raise ValueError(f"sh_coefficients must be (N,B,3), got {tuple(sh.shape)}")
     keep.extend(arrays)
-    return sgs_gaussians(m.data_ptr(), s.data_ptr(), r.data_ptr(), o.data_ptr(), sh.data_ptr(), n, sh.shape[1])
+    cache = None
+    if cov3d_cache is not None:
+        if cov3d_cache.dtype != torch.float32 or cov3d_cache.shape != (n, 8) or cov3d_cache.device != device \
+                or not cov3d_cache.is_contiguous() or cov3d_cache.data_ptr() % 16:
+            raise ValueError("cov3d_cache must be the contiguous (N,8) float32 tensor precompute_cov3d returns")
+        keep.append(cov3d_cache)
+        cache = cov3d_cache.data_ptr()
+    return sgs_gaussians(m.data_ptr(), s.data_ptr(), r.data_ptr(), o.data_ptr(), sh.data_ptr(), n, sh.shape[1],
+                         cache)
+
+
+def precompute_cov3d(means, scales, rotations, opacities) -> torch.Tensor:
+    """(N,8) float32 cache of the view-independent projection terms (sgs_precompute_cov3d):
+    3D covariance, sigmoid opacity and log(255 opacity) per Gaussian. Pass it as
+    ``cov3d_cache=`` to render() / accumulate() / the batched renderers; recompute it when
+    scales, rotations or opacities change. Renders are bit-identical with and without it."""
+    keep: list = []
+    n = means.shape[0]
+    sh = torch.empty((n, 1, 3), dtype=torch.float32, device=means.device)
+    g = gaussians_struct(means, scales, rotations, opacities, sh, keep)
+    out = torch.empty((n, 8), dtype=torch.float32, device=means.device)
+    with torch.cuda.device(means.device):
+        _check(_lib.sgs_precompute_cov3d(C.byref(g), out.data_ptr(), torch.cuda.current_stream(means.device).cuda_stream),
+               "sgs_precompute_cov3d")
+    return out
 
 
 def settings_struct(w2c, cam_position, width, height, focal_x, focal_y, center_x, center_y, near_plane, far_plane,
@@ -227,13 +274,17 @@ def render(means, scales, rotations, opacities, sh_coefficients, w2c, cam_positi
            center_x, center_y, near_plane, far_plane, proper_antialiasing, supersampling_factor, use_gps,
            hybrid_scheduling, spatial_reuse, blue_noise, k_buffer_primitives, k_buffer_occlusion, history_image,
            history_depth, history_alpha, history_weight_mass, view_to_previous_view, temporal_mode, *,
-           frame_idx: int | None = None, tuning: Tuning | None = None, return_framebuffer: bool = False):
+           frame_idx: int | None = None, tuning: Tuning | None = None, return_framebuffer: bool = False,
+           cov3d_cache: torch.Tensor | None = None):
     """Same positional arguments and result as _C.render (render.cu:16-45): fresh
     (image (H,W,3), depth, alpha, weight_mass (H,W,1)) tensors. ``frame_idx`` defaults to the
-    process-global counter; ``return_framebuffer`` appends the packed framebuffer (P,) int64."""
+    process-global counter; ``return_framebuffer`` appends the packed framebuffer (P,) int64;
+    ``cov3d_cache`` is precompute_cov3d's output for these Gaussians. The tile schedule's
+    workspace is capped by ``tuning.max_workspace_mib`` (default: default_workspace_cap_mib),
+    above which the view takes the scatter schedule."""
     device = means.device
     keep: list = []
-    g = gaussians_struct(means, scales, rotations, opacities, sh_coefficients, keep)
+    g = gaussians_struct(means, scales, rotations, opacities, sh_coefficients, keep, cov3d_cache)
     s = settings_struct(w2c, cam_position, width, height, focal_x, focal_y, center_x, center_y, near_plane, far_plane,
                         proper_antialiasing, supersampling_factor, use_gps, hybrid_scheduling, spatial_reuse, device, keep)
     mode = int(temporal_mode)
@@ -273,8 +324,8 @@ def render(means, scales, rotations, opacities, sh_coefficients, w2c, cam_positi
             bn_t = blue_noise.to(device=device, dtype=torch.uint8).contiguous()
             keep.append(bn_t)
             bn, bn_numel = bn_t.data_ptr(), bn_t.numel()
-        tc = tuning.c() if tuning is not None else None
-        tp = C.byref(tc) if tc is not None else None
+        tc = _capped(tuning, device).c()
+        tp = C.byref(tc)
         nbytes = _lib.sgs_render_workspace_size(C.byref(s), g.n_gaussians, tp)
         ws = _workspace(nbytes, device)
         fidx = next_frame_idx() if frame_idx is None else int(frame_idx) & 0xFFFFFFFF
@@ -294,12 +345,12 @@ def accumulate(means, scales, rotations, opacities, sh_coefficients, w2c, cam_po
                focal_y, center_x, center_y, near_plane, far_plane, proper_antialiasing, supersampling_factor, use_gps,
                hybrid_scheduling, spatial_reuse=False, blue_noise=None, *, frame_idx: int, tuning: Tuning | None = None,
                framebuffer: torch.Tensor | None = None, counters: torch.Tensor | None = None,
-               workspace: torch.Tensor | None = None) -> torch.Tensor:
+               workspace: torch.Tensor | None = None, cov3d_cache: torch.Tensor | None = None) -> torch.Tensor:
     """Accumulation stage only (sgs_accumulate): returns the packed framebuffer as
     (H*ss, W*ss) int64 (uint64 bits). ``counters`` (4,) int64 receives the work counts."""
     device = means.device
     keep: list = []
-    g = gaussians_struct(means, scales, rotations, opacities, sh_coefficients, keep)
+    g = gaussians_struct(means, scales, rotations, opacities, sh_coefficients, keep, cov3d_cache)
     s = settings_struct(w2c, cam_position, width, height, focal_x, focal_y, center_x, center_y, near_plane, far_plane,
                         proper_antialiasing, supersampling_factor, use_gps, hybrid_scheduling, spatial_reuse, device, keep)
     ss = max(1, int(supersampling_factor))
@@ -309,8 +360,8 @@ def accumulate(means, scales, rotations, opacities, sh_coefficients, w2c, cam_po
             framebuffer = torch.empty((FH, FW), dtype=torch.int64, device=device)
         elif framebuffer.numel() != FH * FW or framebuffer.dtype != torch.int64 or not framebuffer.is_contiguous():
             raise ValueError("framebuffer must be a contiguous int64 tensor of (H*ss)*(W*ss) elements")
-        tc = tuning.c() if tuning is not None else None
-        tp = C.byref(tc) if tc is not None else None
+        tc = _capped(tuning, device).c()
+        tp = C.byref(tc)
         nbytes = _lib.sgs_accumulate_workspace_size(C.byref(s), g.n_gaussians, tp)
         if nbytes and (workspace is None or workspace.numel() < nbytes or workspace.data_ptr() % 256):
             workspace = _workspace(nbytes, device)

@@ -72,10 +72,11 @@ class BatchRenderer:
     per view, overwritten by every render)."""
 
     def __init__(self, means, scales, rotations, opacities, sh_coefficients, views: Sequence[BatchView],
-                 tuning: BatchTuning | None = None):
+                 tuning: BatchTuning | None = None, cov3d_cache: torch.Tensor | None = None):
         self.device = means.device
         self._keep: list = []
-        self.gaussians = _native.gaussians_struct(means, scales, rotations, opacities, sh_coefficients, self._keep)
+        self.gaussians = _native.gaussians_struct(means, scales, rotations, opacities, sh_coefficients, self._keep,
+                                                  cov3d_cache)
         self.n_views = len(views)
         self.views = list(views)
         self.tuning = tuning.c() if tuning is not None else None
@@ -213,7 +214,8 @@ class StreamedBatchRenderer:
     every group or a list with one per group (e.g. each group's instance capacity)."""
 
     def __init__(self, means, scales, rotations, opacities, sh_coefficients, views: Sequence[BatchView],
-                 tuning: BatchTuning | Sequence[BatchTuning] | None = None, streams: int = 2):
+                 tuning: BatchTuning | Sequence[BatchTuning] | None = None, streams: int = 2,
+                 cov3d_cache: torch.Tensor | None = None):
         import torch
         if streams < 1:
             raise ValueError("streams must be at least 1")
@@ -224,7 +226,8 @@ class StreamedBatchRenderer:
         if len(tunings) != len(self.groups):
             raise ValueError(f"{len(tunings)} tunings for {len(self.groups)} view groups")
         self.parts = [BatchRenderer(means, scales, rotations, opacities, sh_coefficients,
-                                    [self.views[i] for i in g], t) for g, t in zip(self.groups, tunings)]
+                                    [self.views[i] for i in g], t, cov3d_cache)
+                      for g, t in zip(self.groups, tunings)]
         self.streams = [torch.cuda.Stream(device=self.device) for _ in self.parts]
         self.outputs: list[History] = [None] * len(self.views)  # type: ignore[list-item]
         for g, r in zip(self.groups, self.parts):
@@ -280,7 +283,7 @@ class StreamedBatchRenderer:
 
 
 def render_batch(means, scales, rotations, opacities, sh_coefficients, views: Sequence[BatchView],
-                 tuning: BatchTuning | None = None) -> list[History]:
+                 tuning: BatchTuning | None = None, cov3d_cache: torch.Tensor | None = None) -> list[History]:
     """Renders every view (all of its samples) in one persistent launch; fresh outputs."""
-    r = BatchRenderer(means, scales, rotations, opacities, sh_coefficients, views, tuning)
+    r = BatchRenderer(means, scales, rotations, opacities, sh_coefficients, views, tuning, cov3d_cache)
     return r.render()

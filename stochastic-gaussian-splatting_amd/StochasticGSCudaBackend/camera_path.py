@@ -141,7 +141,8 @@ def benchmark_camera_path_batched(renderer: StochasticGSRenderer, views: list[Vi
     groups = []
     for g in range(0, len(views), max_views):
         bviews = batch_views_for_path(renderer, views[g:g + max_views], first_frame=g * max(renderer.SAMPLES_PER_PIXEL, 1))
-        groups.append(BatchRenderer(m.means, m.scales, m.rotations, m.opacities, m.sh_coefficients, bviews, tuning))
+        groups.append(BatchRenderer(m.means, m.scales, m.rotations, m.opacities, m.sh_coefficients, bviews, tuning,
+                                    getattr(m, "cov3d_cache", None)))
     for r in groups:
         r.render()
     torch.cuda.synchronize()

@@ -3,12 +3,16 @@
 ``StochasticGSModel(PLY_PATH=...).build()`` registers the buffers ``means``, ``scales``,
 ``rotations``, ``opacities`` and ``sh_coefficients`` (the names ``Renderer.py:43-46`` reads),
 in Morton order of the means, on the current ROCm device. Decoding, ordering and
-reordering run in libsgs.so (see ``ply.load_ply``).
+reordering run in libsgs.so (see ``ply.load_ply``). At load it also precomputes the
+view-independent projection terms (3D covariance, sigmoid opacity; the precompute
+README.md:139-142 suggests) into ``cov3d_cache``, which the renderer passes to every render;
+call ``refresh_cache()`` after changing the parameters.
 """
 from __future__ import annotations
 
 import torch
 
+from ._native import precompute_cov3d
 from .ply import load_ply
 
 
@@ -33,7 +37,12 @@ class StochasticGSModel(torch.nn.Module):
         self.register_buffer("rotations", arrays["rotations"])
         self.register_buffer("opacities", arrays["opacities"])
         self.register_buffer("sh_coefficients", arrays["sh"])
+        self.refresh_cache()
         return self
+
+    def refresh_cache(self) -> None:
+        """Recomputes ``cov3d_cache`` from the current scales, rotations and opacities."""
+        self.register_buffer("cov3d_cache", precompute_cov3d(self.means, self.scales, self.rotations, self.opacities))
 
     @classmethod
     def from_arrays(cls, means, scales, rotations, opacities, sh_coefficients, name: str | None = None):
@@ -42,6 +51,7 @@ class StochasticGSModel(torch.nn.Module):
         for k, v in (("means", means), ("scales", scales), ("rotations", rotations), ("opacities", opacities),
                      ("sh_coefficients", sh_coefficients)):
             m.register_buffer(k, v)
+        m.refresh_cache()
         return m
 
     @property

@@ -127,8 +127,17 @@ def render(
     *,
     frame_idx: int | None = None,
     tuning: Tuning | None = None,
+    cov3d_cache: torch.Tensor | None = None,
 ) -> History:
-    """Renders one stochastic sample and merges it into ``history`` (render.py:72-95)."""
+    """Renders one stochastic sample and merges it into ``history`` (render.py:72-95).
+
+    Stochastic transparency without hybrid scheduling accumulates by default with the tile
+    schedule: each (bin, Gaussian) instance is listed per framebuffer bin and each bin's list
+    is sorted into depth buckets so 16x16 tiles can stop early, where the reference scatters
+    sorting-free (README.md:6). The framebuffer is the same bit for bit; ``tuning`` selects
+    the scatter (``Tuning(schedule=2)``) or caps the tile schedule's per-call workspace
+    (``max_workspace_mib``, default a quarter of the device's memory), above which the view
+    scatters. ``cov3d_cache``: precompute_cov3d's output for these Gaussians, or None."""
     return History(*_C.render(
         means,
         scales,
@@ -143,4 +152,5 @@ def render(
         int(temporal_mode),
         frame_idx=frame_idx,
         tuning=tuning,
+        cov3d_cache=cov3d_cache,
     ))

@@ -114,7 +114,8 @@ class StochasticGSRenderer:
             frame, self.frame_idx = self.frame_idx, self.frame_idx + 1
         m = self.model
         self.history = render(m.means, m.scales, m.rotations, m.opacities, m.sh_coefficients, settings, self.k_buffer,
-                              self.history, view_to_previous_view, temporal_mode, frame_idx=frame)
+                              self.history, view_to_previous_view, temporal_mode, frame_idx=frame,
+                              cov3d_cache=getattr(m, "cov3d_cache", None))
 
     def _batched(self, settings: Settings) -> bool:
         return (not settings.use_gps and not settings.hybrid_scheduling and not settings.spatial_reuse
@@ -135,7 +136,8 @@ class StochasticGSRenderer:
                            view_to_previous_view=view_to_previous_view if temporal_mode == TemporalMode.REPROJECT
                            else None)
             m = self.model
-            self.history = render_batch(m.means, m.scales, m.rotations, m.opacities, m.sh_coefficients, [bv])[0]
+            self.history = render_batch(m.means, m.scales, m.rotations, m.opacities, m.sh_coefficients, [bv],
+                                        cov3d_cache=getattr(m, "cov3d_cache", None))[0]
             return
         for _ in range(spp):
             self._render_once(settings, view_to_previous_view, temporal_mode)

@@ -152,14 +152,15 @@ class Server:
                  max_views: int = 32, inflight: int = 2, chain: int = 3, max_wait: float = 0.0,
                  arena_bytes: int | None = None,
                  tuning: BatchTuning | None = None, keep_outputs: bool = False,
-                 clock: Callable[[], float] = time.perf_counter):
+                 clock: Callable[[], float] = time.perf_counter, cov3d_cache: torch.Tensor | None = None):
         if max_views < 1 or inflight < 1 or chain < 1:
             raise ValueError("max_views, inflight and chain must be >= 1")
         self.chain = chain
         self.max_wait = float(max_wait)
         self.device = means.device
         self._keep: list = []
-        self.gaussians = _native.gaussians_struct(means, scales, rotations, opacities, sh_coefficients, self._keep)
+        self.gaussians = _native.gaussians_struct(means, scales, rotations, opacities, sh_coefficients, self._keep,
+                                                  cov3d_cache)
         self.n_gaussians = int(self.gaussians.n_gaussians)
         self.streams = [_Stream(s, [_history(s, self.device) for _ in range(chain + 1)]) for s in streams]
         if len(self.streams) > (1 << (32 - STREAM_SHIFT)):

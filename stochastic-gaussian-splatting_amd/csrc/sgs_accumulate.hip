@@ -106,7 +106,7 @@ __global__ __launch_bounds__(kBlock) void st_scatter_kernel(SceneArgs sc, const 
     const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
     const float4 r1 = w2c[0], r2 = w2c[1], r3 = w2c[2];
     Projected p;
-    bool active = g < sc.n && project_gaussian(sc.means, sc.scales, sc.rotations, sc.opacities, g, r1, r2, r3, cam, p);
+    bool active = g < sc.n && project_scene(sc, g, r1, r2, r3, cam, p);
 
     uint32_t x0 = 0, y0 = 0, w = 0, count = 0;
     float msx = 0.0f, msy = 0.0f, o = 0.0f;
@@ -264,7 +264,7 @@ __global__ __launch_bounds__(kBlock) void gps_scatter_kernel(SceneArgs sc, const
     const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
     const float4 r1 = w2c[0], r2 = w2c[1], r3 = w2c[2];
     Projected p;
-    bool active = g < sc.n && project_gaussian(sc.means, sc.scales, sc.rotations, sc.opacities, g, r1, r2, r3, cam, p);
+    bool active = g < sc.n && project_scene(sc, g, r1, r2, r3, cam, p);
     const uint32_t n_projected = active ? 1u : 0u;
 
     uint32_t count = 0, seed_x = 0, seed_y = 0;

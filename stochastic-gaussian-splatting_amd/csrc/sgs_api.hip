@@ -78,6 +78,7 @@ static Tuning resolve_tuning(const sgs_tuning* t, uint32_t n_gaussians)
     r.max_tile_instances = (t && t->max_tile_instances) ? t->max_tile_instances : (1u << 24);
     r.max_gps_points = t ? t->max_gps_points : 0u;
     r.tiles = !(t && t->schedule == SGS_BATCH_SCATTER);
+    r.max_workspace_bytes = t ? (uint64_t)t->max_workspace_mib << 20 : 0u;
     // one sample of one view: 16x16 bins (larger bins leave small frames with few workgroups)
     r.bin_log = (t && t->bin_size) ? bin_log2(t->bin_size) : 0u;
     return r;
@@ -173,6 +174,18 @@ RenderLayout render_layout(const sgs_settings* s, const FbCamera& cam, const Tun
     return L;
 }
 
+// The tuning of one view's render: resolve_tuning, then, when the tile schedule's workspace
+// would exceed the caller's cap, the scatter schedule (no binning buffers; the same
+// framebuffer bit for bit)
+Tuning view_tuning(const sgs_settings* s, const FbCamera& cam, const sgs_tuning* tuning, uint32_t n_gaussians)
+{
+    Tuning t = resolve_tuning(tuning, n_gaussians);
+    if (t.max_workspace_bytes && single_tiles_allowed(s, t) &&
+        render_layout(s, cam, t, n_gaussians).total > t.max_workspace_bytes)
+        t.tiles = false;
+    return t;
+}
+
 int check_settings(const sgs_settings* s)
 {
     if (!s) return fail(SGS_ERROR_INVALID_ARGUMENT, "settings is NULL");
@@ -198,6 +211,8 @@ int check_gaussians(const sgs_gaussians* g)
         return fail(SGS_ERROR_INVALID_ARGUMENT, "total_sh_bases must be 1, 4, 9 or 16 (SH degree 0-3)");
     if ((b == 4 || b == 16) && reinterpret_cast<uintptr_t>(g->sh_coefficients) % 16u != 0)
         return fail(SGS_ERROR_INVALID_ARGUMENT, "sh_coefficients must be 16-byte aligned");
+    if (reinterpret_cast<uintptr_t>(g->cov3d) % 16u != 0)
+        return fail(SGS_ERROR_INVALID_ARGUMENT, "cov3d must be 16-byte aligned");
     return SGS_SUCCESS;
 }
 
@@ -218,7 +233,8 @@ int accumulate_tiles_single(const sgs_gaussians* g, const sgs_settings* s, const
 
 SceneArgs scene_args(const sgs_gaussians* g)
 {
-    return { g->means, g->scales, g->rotations, g->opacities, g->sh_coefficients, g->n_gaussians, g->total_sh_bases };
+    return { g->means, g->scales, g->rotations, g->opacities, g->sh_coefficients, g->n_gaussians, g->total_sh_bases,
+             reinterpret_cast<const float4*>(g->cov3d) };
 }
 
 // clear + accumulate into fb (tile_ws: the single-view tile section of the workspace, or NULL)
@@ -389,7 +405,7 @@ size_t sgs_render_workspace_size(const sgs_settings* settings, uint32_t n_gaussi
                                         settings->center_x, settings->center_y, settings->near_plane,
                                         settings->far_plane, settings->proper_antialiasing,
                                         settings->supersampling_factor);
-    return render_layout(settings, cam, resolve_tuning(tuning, n_gaussians), n_gaussians).total;
+    return render_layout(settings, cam, view_tuning(settings, cam, tuning, n_gaussians), n_gaussians).total;
 }
 
 int sgs_render(const sgs_gaussians* gaussians, const sgs_settings* s, const uint8_t* blue_noise,
@@ -403,9 +419,9 @@ int sgs_render(const sgs_gaussians* gaussians, const sgs_settings* s, const uint
                                 history, view_to_previous_view, temporal_mode, out, tuning)))
         return rc;
 
-    const Tuning t = resolve_tuning(tuning, gaussians->n_gaussians);
     const FbCamera cam = make_fb_camera(s->width, s->height, s->focal_x, s->focal_y, s->center_x, s->center_y,
                                         s->near_plane, s->far_plane, s->proper_antialiasing, s->supersampling_factor);
+    const Tuning t = view_tuning(s, cam, tuning, gaussians->n_gaussians);
     const RenderLayout L = render_layout(s, cam, t, gaussians->n_gaussians);
     if (!workspace || workspace_bytes < L.total)
         return fail(SGS_ERROR_WORKSPACE_TOO_SMALL,
@@ -461,6 +477,19 @@ int sgs_render_noalloc(const sgs_gaussians* gaussians, const sgs_settings* s, co
                              k_buffer_occlusion, history, view_to_previous_view, temporal_mode, out, stream);
 }
 
+int sgs_precompute_cov3d(const sgs_gaussians* gaussians, float* cov3d, void* stream)
+{
+    int rc;
+    if ((rc = check_gaussians(gaussians))) return rc;
+    if (gaussians->n_gaussians > 0 && !cov3d) return fail(SGS_ERROR_INVALID_ARGUMENT, "cov3d is NULL");
+    if (reinterpret_cast<uintptr_t>(cov3d) % 16u != 0)
+        return fail(SGS_ERROR_INVALID_ARGUMENT, "cov3d must be 16-byte aligned");
+    SceneArgs sc = scene_args(gaussians);
+    sc.cov3d = nullptr;  // computed from the raw parameters
+    const hipError_t e = launch_cov3d(sc, reinterpret_cast<float4*>(cov3d), static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? SGS_SUCCESS : hip_fail(e, "cov3d");
+}
+
 size_t sgs_accumulate_workspace_size(const sgs_settings* settings, uint32_t n_gaussians, const sgs_tuning* tuning)
 {
     if (!settings) return 0;
@@ -468,7 +497,7 @@ size_t sgs_accumulate_workspace_size(const sgs_settings* settings, uint32_t n_ga
                                         settings->center_x, settings->center_y, settings->near_plane,
                                         settings->far_plane, settings->proper_antialiasing,
                                         settings->supersampling_factor);
-    const Tuning t = resolve_tuning(tuning, n_gaussians);
+    const Tuning t = view_tuning(settings, cam, tuning, n_gaussians);
     const size_t hb = hybrid_bytes(settings, cam, t);
     if (single_tiles_allowed(settings, t)) return single_tile_layout(cam, n_gaussians, t.bin_log).total;
     return hb ? ((hb + 255u) & ~size_t(255u)) : 0;
@@ -485,9 +514,9 @@ int sgs_accumulate(const sgs_gaussians* gaussians, const sgs_settings* s, const 
     if (!framebuffer) return fail(SGS_ERROR_INVALID_ARGUMENT, "framebuffer is NULL");
     if (reinterpret_cast<uintptr_t>(framebuffer) % 16u != 0)
         return fail(SGS_ERROR_INVALID_ARGUMENT, "framebuffer must be 16-byte aligned");
-    const Tuning t = resolve_tuning(tuning, gaussians->n_gaussians);
     const FbCamera cam = make_fb_camera(s->width, s->height, s->focal_x, s->focal_y, s->center_x, s->center_y,
                                         s->near_plane, s->far_plane, s->proper_antialiasing, s->supersampling_factor);
+    const Tuning t = view_tuning(s, cam, tuning, gaussians->n_gaussians);
     const bool tiles = single_tiles_allowed(s, t);
     const size_t need = tiles ? single_tile_layout(cam, gaussians->n_gaussians, t.bin_log).total : hybrid_bytes(s, cam, t);
     if (need && (!workspace || workspace_bytes < need))

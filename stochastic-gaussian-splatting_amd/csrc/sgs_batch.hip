@@ -193,7 +193,7 @@ __global__ __launch_bounds__(kBatchBlock) void batch_accumulate_kernel(SceneArgs
         const uint32_t g = chunk * kBatchBlock + tid;
         Projected p;
         bool active =
-            g < sc.n && project_gaussian(sc.means, sc.scales, sc.rotations, sc.opacities, g, vd.r1, vd.r2, vd.r3,
+            g < sc.n && project_scene(sc, g, vd.r1, vd.r2, vd.r3,
                                          vd.cam, p);
         uint32_t x0 = 0, y0 = 0, w = 0, count = 0;
         float msx = 0.0f, msy = 0.0f, o = 0.0f;
@@ -446,6 +446,14 @@ __global__ __launch_bounds__(kBatchBlock) void cov3d_kernel(SceneArgs sc, float4
     Cov3d c = { 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f };
     const bool valid = gaussian_cov3d(sc.scales, sc.rotations, sc.opacities, g, c);
     store_cov3d(cache, g, valid, c);
+}
+
+hipError_t launch_cov3d(const SceneArgs& sc, float4* cache, hipStream_t stream)
+{
+    if (sc.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(cov3d_kernel, dim3((sc.n + kBatchBlock - 1u) / kBatchBlock), dim3(kBatchBlock), 0, stream, sc,
+                       cache);
+    return hipGetLastError();
 }
 
 // tile coordinates fit 16 bits (framebuffers are at most 65535 pixels wide); a pair with no
@@ -1191,12 +1199,14 @@ hipError_t launch_batch_tiles(const SceneArgs& sc, const ViewDesc* views, uint32
     uint64_t grid = (uint64_t)cus * (per_cu < 1 ? 1 : per_cu);
     if (grid > n_chunks) grid = n_chunks;
     if (grid < 1) grid = 1;
+    // the scene's own cache of the view-independent terms, or one computed here per render
+    const float4* cov3d = sc.cov3d ? sc.cov3d : tb.cov3d;
     if (sc.n > 0) {
         hipLaunchKernelGGL(chunk_centroid_kernel, dim3(n_chunks), dim3(kBatchBlock), 0, stream, sc, tb.centroids);
         hipLaunchKernelGGL(view_key_range_kernel, dim3(n_views), dim3(kRangeThreads), 0, stream, views, tb.centroids,
                            n_chunks, tb.view_keys);
-        hipLaunchKernelGGL(cov3d_kernel, dim3(n_chunks), dim3(kBatchBlock), 0, stream, sc, tb.cov3d);
-        hipLaunchKernelGGL((batch_bin_kernel<false>), dim3((unsigned)grid), dim3(kBatchBlock), 0, stream, sc, tb.cov3d,
+        if (!sc.cov3d) hipLaunchKernelGGL(cov3d_kernel, dim3(n_chunks), dim3(kBatchBlock), 0, stream, sc, tb.cov3d);
+        hipLaunchKernelGGL((batch_bin_kernel<false>), dim3((unsigned)grid), dim3(kBatchBlock), 0, stream, sc, cov3d,
                            views, n_views, tb.tile_count, tb.tile_cursor, tb.offsets,
                            tb.entries, tb.view_keys, tb.pair_rect, counters);
     }
@@ -1204,7 +1214,7 @@ hipError_t launch_batch_tiles(const SceneArgs& sc, const ViewDesc* views, uint32
                        CountInput{ tb.tile_count }, tb.offsets, nullptr, tb.n_tiles, tb.n_tiles, &counters->n_instances,
                        &counters->overflow, tb.capacity, &counters->total_instances);
     if (sc.n > 0) {
-        hipLaunchKernelGGL((batch_bin_kernel<true>), dim3((unsigned)grid), dim3(kBatchBlock), 0, stream, sc, tb.cov3d,
+        hipLaunchKernelGGL((batch_bin_kernel<true>), dim3((unsigned)grid), dim3(kBatchBlock), 0, stream, sc, cov3d,
                            views, n_views, tb.tile_count, tb.tile_cursor, tb.offsets,
                            tb.entries, tb.view_keys, tb.pair_rect, counters);
     }
@@ -1222,15 +1232,15 @@ hipError_t launch_batch_tiles(const SceneArgs& sc, const ViewDesc* views, uint32
     // once on the other's (measured faster than two launches over class-ordered tiles, whose
     // sequential tails cost more than the early returns).
     if (tb.has_single)
-        hipLaunchKernelGGL(batch_tile_raster_kernel<1>, raster_grid, dim3(kBatchBlock), 0, stream, sc, tb.cov3d,
+        hipLaunchKernelGGL(batch_tile_raster_kernel<1>, raster_grid, dim3(kBatchBlock), 0, stream, sc, cov3d,
                            views, n_views, raster_prefix, tb.offsets, tb.sorted_instances, tb.sorted_keys, counters,
                            fb);
     if (tb.has_multi && layers == 8u)
-        hipLaunchKernelGGL(batch_tile_raster_kernel<8>, raster_grid, dim3(kBatchBlock), 0, stream, sc, tb.cov3d,
+        hipLaunchKernelGGL(batch_tile_raster_kernel<8>, raster_grid, dim3(kBatchBlock), 0, stream, sc, cov3d,
                            views, n_views, raster_prefix, tb.offsets, tb.sorted_instances, tb.sorted_keys, counters,
                            fb);
     else if (tb.has_multi)
-        hipLaunchKernelGGL(batch_tile_raster_kernel<16>, raster_grid, dim3(kBatchBlock), 0, stream, sc, tb.cov3d,
+        hipLaunchKernelGGL(batch_tile_raster_kernel<16>, raster_grid, dim3(kBatchBlock), 0, stream, sc, cov3d,
                            views, n_views, raster_prefix, tb.offsets, tb.sorted_instances, tb.sorted_keys, counters,
                            fb);
     return hipGetLastError();

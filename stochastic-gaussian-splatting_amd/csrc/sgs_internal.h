@@ -37,7 +37,17 @@ struct SceneArgs {
     const float* sh;
     uint32_t n;
     uint32_t sh_bases;
+    const float4* cov3d;  // cached view-independent projection terms (store_cov3d layout), or NULL
 };
+
+// EWA projection of Gaussian g from its raw parameters or, when the scene carries it, from
+// the cached view-independent terms (bit-identical either way)
+__device__ __forceinline__ bool project_scene(const SceneArgs& sc, uint32_t g, const float4 r1, const float4 r2,
+                                              const float4 r3, const FbCamera& cam, Projected& o)
+{
+    if (sc.cov3d) return project_cached(sc.means, sc.cov3d, g, r1, r2, r3, cam, o);
+    return project_gaussian(sc.means, sc.scales, sc.rotations, sc.opacities, g, r1, r2, r3, cam, o);
+}
 
 // Resolved scheduling knobs (zero fields of sgs_tuning replaced by defaults).
 struct Tuning {
@@ -50,6 +60,7 @@ struct Tuning {
     uint32_t max_gps_points;  // 0 = reference heuristic
     bool tiles;               // single view: tile schedule when the settings allow it
     uint32_t bin_log;         // single-view tile schedule bins of (16 << bin_log) pixels
+    uint64_t max_workspace_bytes;  // single view: scatter when the tile workspace exceeds it (0: no cap)
 };
 
 // One binned large Gaussian of the hybrid ST path (64 bytes, 16-byte aligned).
@@ -228,6 +239,8 @@ struct TileBuffers {
 hipError_t launch_batch_tiles(const SceneArgs& sc, const ViewDesc* views, uint32_t n_views, const uint32_t* tile_prefix,
                               const uint32_t* raster_prefix, const TileBuffers& tb, uint32_t layers,
                               BatchCounters* counters, uint64_t* fb, hipEvent_t raster_event, hipStream_t stream);
+// the view-independent projection terms of every Gaussian (store_cov3d layout, 2 float4 each)
+hipError_t launch_cov3d(const SceneArgs& sc, float4* cache, hipStream_t stream);
 hipError_t launch_fill_u64_if(uint64_t* p, size_t n, uint64_t value, const uint32_t* run_if_set, hipStream_t stream);
 // device view table of a single view whose world-to-view matrix and camera position are in
 // device memory: `desc` with its rows and position taken from w2c / cam_position, then the

@@ -120,7 +120,7 @@ __global__ __launch_bounds__(kBlockSample) void sample_kernel(SceneArgs sc, cons
     // project the winner again to evaluate it at the four neighbours (resolve.cu:81-117)
     Projected p;
     bool projected = false;
-    if (has) projected = project_gaussian(sc.means, sc.scales, sc.rotations, sc.opacities, g, w2c[0], w2c[1], w2c[2], cam, p);
+    if (has) projected = project_scene(sc, g, w2c[0], w2c[1], w2c[2], cam, p);
     float alpha = 1.0f;
     float na[4] = { 1.0f, 1.0f, 1.0f, 1.0f };
     float sdepth = __builtin_huge_valf();

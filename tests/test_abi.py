@@ -159,6 +159,21 @@ def test_workspace_sizes(native):
     assert b"bin_size" in lib.sgs_last_error()
 
 
+def test_workspace_cap_selects_scatter(native):
+    """A tile-schedule workspace above max_workspace_mib gives the scatter schedule's size."""
+    lib = native.library()
+    s = _settings(native, width=1920, height=1080)
+    n = 1_000_000
+    tiles = lib.sgs_render_workspace_size(C.byref(s), n, C.byref(native.sgs_tuning(max_workspace_mib=1 << 20)))
+    scatter = lib.sgs_render_workspace_size(C.byref(s), n, C.byref(native.sgs_tuning(schedule=2)))
+    assert tiles > scatter
+    cap = (tiles >> 20) - 1   # just below the tile schedule's need
+    assert lib.sgs_render_workspace_size(C.byref(s), n, C.byref(native.sgs_tuning(max_workspace_mib=cap))) == scatter
+    assert lib.sgs_render_workspace_size(C.byref(s), n,
+                                         C.byref(native.sgs_tuning(max_workspace_mib=(tiles >> 20) + 1))) == tiles
+    assert lib.sgs_accumulate_workspace_size(C.byref(s), n, C.byref(native.sgs_tuning(max_workspace_mib=cap))) == 0
+
+
 def test_python_api_mirrors_reference():
     from StochasticGSCudaBackend import History, Settings, TemporalMode, create_k_buffer, render  # noqa: F401
     assert Settings._fields == ("w2c", "cam_position", "width", "height", "focal_x", "focal_y", "center_x",
