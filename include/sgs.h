@@ -192,10 +192,14 @@ int sgs_accumulate(const sgs_gaussians* gaussians, const sgs_settings* settings,
 
 /* ---------------------------------------------------------------------------------------
  * Batched multi-view rendering (no reference counterpart: the reference renders one view
- * per call from Python). Every view renders spp samples in one persistent launch; view v's
- * result equals spp successive sgs_render calls with frames frame_idx .. frame_idx+spp-1,
- * the first with temporal_mode and the rest STATIC (identical framebuffer winners).
- * Spatial reuse, GPS, hybrid scheduling and blue noise are single-view features.
+ * per call from Python). Every view renders spp samples; view v's result equals spp
+ * successive sgs_render calls with frames frame_idx .. frame_idx+spp-1, the first with
+ * temporal_mode and the rest STATIC (identical framebuffer winners and outputs). Views of
+ * stochastic transparency with white noise (hybrid scheduling or not: identical
+ * framebuffers) accumulate all their samples in the batch's shared schedule; views with
+ * GPS or blue noise accumulate each sample with the single-view scatter kernels into the
+ * batch's framebuffers; views with spatial reuse are resolved sample by sample with their
+ * k-buffer (the per-sample order the k-buffer's updates need), the others in one pass.
  * --------------------------------------------------------------------------------------- */
 typedef struct sgs_view {
     float w2c[16];                    /* host copy, row-major world->view */
@@ -221,6 +225,16 @@ typedef struct sgs_view {
                                          read by the bins' 16x16 raster tiles: 16, 32 or 64; 0 =
                                          16 for framebuffers (W ss x H ss) of at most 2^19
                                          pixels, else 32. No result bit depends on it. */
+    /* the single-view features of sgs_settings / sgs_render, per view (all zero: stochastic
+     * transparency, white noise, no spatial reuse) */
+    int32_t use_gps;
+    int32_t hybrid_scheduling;
+    int32_t spatial_reuse;
+    uint32_t max_gps_points;          /* as sgs_tuning.max_gps_points (0: the reference's cap) */
+    const uint8_t* blue_noise;        /* device (64,128,128) uint8 volume, or NULL */
+    uint64_t blue_noise_numel;        /* 0 or 64*128*128 */
+    int32_t* k_buffer_primitives;     /* spatial reuse: device (K,P,2) int32, updated in place */
+    float* k_buffer_occlusion;        /* spatial reuse: device (K,P,4) float32 */
 } sgs_view;
 
 /* Scheduling of the batched accumulation; zero fields take defaults. None of them changes

@@ -60,7 +60,10 @@ class sgs_view(C.Structure):
                 ("center_y", C.c_float), ("near_plane", C.c_float), ("far_plane", C.c_float),
                 ("proper_antialiasing", C.c_int32), ("supersampling_factor", C.c_int32), ("spp", C.c_uint32),
                 ("frame_idx", C.c_uint32), ("temporal_mode", C.c_int32), ("view_to_previous_view", C.c_float * 16),
-                ("history", sgs_history), ("out", sgs_history), ("bin_size", C.c_uint32)]
+                ("history", sgs_history), ("out", sgs_history), ("bin_size", C.c_uint32),
+                ("use_gps", C.c_int32), ("hybrid_scheduling", C.c_int32), ("spatial_reuse", C.c_int32),
+                ("max_gps_points", C.c_uint32), ("blue_noise", C.c_void_p), ("blue_noise_numel", C.c_uint64),
+                ("k_buffer_primitives", C.c_void_p), ("k_buffer_occlusion", C.c_void_p)]
 
 
 class sgs_batch_tuning(C.Structure):

@@ -53,18 +53,19 @@ class BatchView:
     # tile schedule bins of this view (16, 32 or 64 pixels; 0 = 16 for framebuffers of at most
     # 2^19 pixels, else 32); no result bit depends on it
     bin_size: int = 0
+    # the single-view features (settings.use_gps / hybrid_scheduling / spatial_reuse come from
+    # settings): the view's k-buffer (create_k_buffer) when spatial_reuse, updated in place like
+    # render()'s; a (64,128,128) uint8 blue-noise volume or None; the GPS point cap (0: the
+    # reference's)
+    k_buffer: tuple | None = None
+    blue_noise: torch.Tensor | None = None
+    max_gps_points: int = 0
 
 
 def _host(t, shape) -> np.ndarray:
     if isinstance(t, torch.Tensor):
         t = t.detach().to("cpu")
     return np.ascontiguousarray(np.asarray(t, np.float32).reshape(shape))
-
-
-def _check_settings(s: Settings) -> None:
-    if s.use_gps or s.hybrid_scheduling or s.spatial_reuse:
-        raise ValueError("batched rendering covers stochastic transparency without spatial reuse; "
-                         "render GPS, hybrid or spatial-reuse views with render()")
 
 
 class BatchRenderer:
@@ -86,7 +87,6 @@ class BatchRenderer:
         opts = dict(dtype=torch.float32, device=self.device)
         for i, v in enumerate(self.views):
             s = v.settings
-            _check_settings(s)
             H, W = int(s.height), int(s.width)
             out = History(torch.empty((H, W, 3), **opts), torch.empty((H, W, 1), **opts),
                           torch.empty((H, W, 1), **opts), torch.empty((H, W, 1), **opts))
@@ -103,6 +103,26 @@ class BatchRenderer:
             cv.spp = int(v.spp)
             cv.temporal_mode = int(v.temporal_mode)
             cv.bin_size = int(v.bin_size)
+            cv.use_gps = int(bool(s.use_gps))
+            cv.hybrid_scheduling = int(bool(s.hybrid_scheduling))
+            cv.spatial_reuse = int(bool(s.spatial_reuse))
+            cv.max_gps_points = int(v.max_gps_points)
+            if v.blue_noise is not None and v.blue_noise.numel() > 0:
+                bn = v.blue_noise.to(device=self.device, dtype=torch.uint8).contiguous()
+                if bn.numel() != 64 * 128 * 128:
+                    raise ValueError(f"view {i}: blue_noise must be a (64,128,128) uint8 volume")
+                self._keep.append(bn)
+                cv.blue_noise, cv.blue_noise_numel = bn.data_ptr(), bn.numel()
+            if s.spatial_reuse:
+                if v.k_buffer is None:
+                    raise ValueError(f"view {i}: spatial reuse needs the view's k-buffer (create_k_buffer)")
+                kp, ko = v.k_buffer
+                P = W * H * max(1, int(s.supersampling_factor)) ** 2
+                if (kp.shape != (_native.k_buffer_size, P, 2) or ko.shape != (_native.k_buffer_size, P, 4)
+                        or kp.dtype != torch.int32 or ko.dtype != torch.float32 or not kp.is_contiguous()
+                        or not ko.is_contiguous() or kp.device != self.device or ko.device != self.device):
+                    raise ValueError(f"view {i}: k-buffer does not match the settings (use create_k_buffer)")
+                cv.k_buffer_primitives, cv.k_buffer_occlusion = kp.data_ptr(), ko.data_ptr()
             v2p = np.eye(4, dtype=np.float32) if v.view_to_previous_view is None else _host(v.view_to_previous_view, 16)
             cv.view_to_previous_view[:] = np.asarray(v2p, np.float32).reshape(16).tolist()
             if int(v.temporal_mode) != 0:

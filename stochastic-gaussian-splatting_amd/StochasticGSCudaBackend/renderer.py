@@ -117,15 +117,12 @@ class StochasticGSRenderer:
                               self.history, view_to_previous_view, temporal_mode, frame_idx=frame,
                               cov3d_cache=getattr(m, "cov3d_cache", None))
 
-    def _batched(self, settings: Settings) -> bool:
-        return (not settings.use_gps and not settings.hybrid_scheduling and not settings.spatial_reuse
-                and blue_noise().numel() == 0)
-
     def _render_samples(self, settings: Settings, view_to_previous_view: torch.Tensor, temporal_mode: TemporalMode,
                         spp: int) -> None:
         """The SPP loop of Renderer.py:85-87 (first sample in temporal_mode, the rest STATIC),
-        as one batched render when the settings allow it."""
-        if spp > 1 and self._batched(settings):
+        as one batched render of the view's spp samples (every feature, the k-buffer updated
+        sample by sample; the same outputs bit for bit)."""
+        if spp > 1:
             from .batch import BatchView, render_batch
             frame = None
             if self.frame_idx is not None:
@@ -134,7 +131,8 @@ class StochasticGSRenderer:
             bv = BatchView(settings, spp=spp, frame_idx=frame, temporal_mode=temporal_mode,
                            history=None if reset else self.history,
                            view_to_previous_view=view_to_previous_view if temporal_mode == TemporalMode.REPROJECT
-                           else None)
+                           else None, k_buffer=self.k_buffer if settings.spatial_reuse else None,
+                           blue_noise=blue_noise())
             m = self.model
             self.history = render_batch(m.means, m.scales, m.rotations, m.opacities, m.sh_coefficients, [bv],
                                         cov3d_cache=getattr(m, "cov3d_cache", None))[0]

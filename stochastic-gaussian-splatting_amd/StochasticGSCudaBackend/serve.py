@@ -1,8 +1,9 @@
 """Serving loop: continuous batching of per-stream render requests over one resident scene.
 
 A *stream* is a client viewport (fixed resolution) whose frames are resolved with the
-spatio-temporal REPROJECT mode against that stream's own history (RESET for its first
-frame), as the reference's Renderer.py:73-87 does for one camera path. A *request* is one
+temporal REPROJECT mode against that stream's own history (RESET for its first frame), and
+with spatial reuse over the stream's own k-buffer when its StreamSpec asks for it (the
+spatio-temporal resolve), as the reference's Renderer.py:73-87 does for one camera path. A *request* is one
 frame of one stream: a camera pose and an spp count.
 
 The server keeps the Gaussians and a ring of ``chain + 1`` output buffers per stream
@@ -58,6 +59,8 @@ class StreamSpec:
     far_plane: float = 10000.0
     proper_antialiasing: bool = False
     supersampling_factor: int = 1
+    spatial_reuse: bool = False   # spatio-temporal resolve: the stream keeps a k-buffer on the device
+    use_gps: bool = False
 
 
 @dataclass
@@ -86,6 +89,7 @@ class Request:
 class _Stream:
     spec: StreamSpec
     ring: list               # chain + 1 History sets
+    k_buffer: tuple | None   # spatial reuse: the stream's k-buffer (create_k_buffer layout)
     head: int = 0            # ring slot of the latest admitted frame's output
     frames: int = 0          # frames submitted
     finished: int = 0        # frames finished (they finish in order)
@@ -141,6 +145,14 @@ def _history(spec: StreamSpec, device) -> History:
                    torch.zeros((H, W, 1), **opts))
 
 
+def _k_buffer(spec: StreamSpec, device) -> tuple | None:
+    if not spec.spatial_reuse:
+        return None
+    P = spec.width * spec.height * max(1, int(spec.supersampling_factor)) ** 2
+    return (torch.full((_native.k_buffer_size, P, 2), -1, dtype=torch.int32, device=device),
+            torch.zeros((_native.k_buffer_size, P, 4), dtype=torch.float32, device=device))
+
+
 def _hist_struct(h: History) -> _native.sgs_history:
     return _native.sgs_history(*[t.data_ptr() for t in h])
 
@@ -162,7 +174,8 @@ class Server:
         self.gaussians = _native.gaussians_struct(means, scales, rotations, opacities, sh_coefficients, self._keep,
                                                   cov3d_cache)
         self.n_gaussians = int(self.gaussians.n_gaussians)
-        self.streams = [_Stream(s, [_history(s, self.device) for _ in range(chain + 1)]) for s in streams]
+        self.streams = [_Stream(s, [_history(s, self.device) for _ in range(chain + 1)], _k_buffer(s, self.device))
+                        for s in streams]
         if len(self.streams) > (1 << (32 - STREAM_SHIFT)):
             raise ValueError(f"at most {1 << (32 - STREAM_SHIFT)} streams")
         self.max_views = max_views
@@ -235,6 +248,10 @@ class Server:
         hist, out = st.ring[(st.head + k) % n], st.ring[(st.head + k + 1) % n]
         cv.history = _hist_struct(hist) if req.mode != int(TemporalMode.RESET) else _native.sgs_history()
         cv.out = _hist_struct(out)
+        cv.use_gps = int(s.use_gps)
+        cv.spatial_reuse = int(s.spatial_reuse)
+        cv.k_buffer_primitives, cv.k_buffer_occlusion = ((st.k_buffer[0].data_ptr(), st.k_buffer[1].data_ptr())
+                                                         if st.k_buffer is not None else (None, None))
         req.out_buffers = out
 
     def _free_slot(self) -> _Slot | None:

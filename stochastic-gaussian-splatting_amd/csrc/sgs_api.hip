@@ -552,6 +552,7 @@ BatchTuning resolve_batch_tuning(const sgs_batch_tuning* t)
 }
 
 uint64_t view_fb_pixels(const sgs_view& v);
+bool view_routed(const sgs_view& v);
 
 // default capacity of the tile schedule's instance list (14 bytes of workspace each): per
 // view, 4 + 8 P / 2^21 per Gaussian for a framebuffer of P pixels (a Gaussian covers more
@@ -559,7 +560,8 @@ uint64_t view_fb_pixels(const sgs_view& v);
 uint32_t default_tile_instances(const sgs_view* views, uint32_t n_views, uint32_t n_gaussians)
 {
     uint64_t per = 0;
-    for (uint32_t v = 0; v < n_views; v++) per += 4ull + ((8ull * view_fb_pixels(views[v])) >> 21);
+    for (uint32_t v = 0; v < n_views; v++)
+        if (!view_routed(views[v])) per += 4ull + ((8ull * view_fb_pixels(views[v])) >> 21);
     const uint64_t want = per >= (1ull << 32) ? (1ull << 32) : per * n_gaussians;
     return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(want, 1ull << 16), 1ull << 31);
 }
@@ -588,13 +590,18 @@ uint32_t view_bin_log(const sgs_view& v)
     if (v.bin_size) return bin_log2(v.bin_size);
     return view_fb_pixels(v) <= kSmallBinMaxPixels ? 0u : 1u;
 }
+// views whose samples the single-view scatter kernels accumulate (GPS, blue noise): they have
+// no bins in the batch's tile schedule
+bool view_routed(const sgs_view& v) { return v.use_gps || (v.blue_noise && v.blue_noise_numel > 0); }
 uint32_t view_grid_w(const sgs_view& v)
 {
+    if (view_routed(v)) return 0u;
     const uint32_t bin = kRasterPixels << view_bin_log(v);
     return (uint32_t)((v.width * view_ss(v) + bin - 1u) / bin);
 }
 uint32_t view_grid_h(const sgs_view& v)
 {
+    if (view_routed(v)) return 0u;
     const uint32_t bin = kRasterPixels << view_bin_log(v);
     return (uint32_t)((v.height * view_ss(v) + bin - 1u) / bin);
 }
@@ -610,7 +617,8 @@ uint64_t view_raster_groups(const sgs_view& v)
 struct BatchLayout {
     size_t counters, ready, tile_count, tile_cursor, cleared_end, offsets, entry_buckets, entry_gaussians,
         sorted_instances, sorted_keys,
-        centroids, cov3d, pair_rect, view_keys, queue, table, fb, samples, total;
+        centroids, cov3d, pair_rect, view_keys, queue, table, fb, samples, reuse_samples, reuse_weights,
+        reuse_temp, total;
     uint64_t fb_words, n_samples, n_tiles, n_raster;
     uint32_t n_blocks, capacity;
 };
@@ -659,6 +667,17 @@ BatchLayout batch_layout(const sgs_view* views, uint32_t n, uint32_t n_gaussians
     L.table = a.take(table_bytes(n));
     L.fb = a.take(8u * L.fb_words);
     L.samples = a.take(16u * (L.n_samples ? L.n_samples : 1u));
+    // spatial-reuse views, one at a time: one sample's samples and reuse weights (16 B per
+    // framebuffer pixel each) and an intermediate history (24 B per image pixel)
+    uint64_t reuse_fb = 0, reuse_px = 0;
+    for (uint32_t v = 0; v < n; v++)
+        if (views[v].spatial_reuse) {
+            reuse_fb = std::max<uint64_t>(reuse_fb, view_fb_pixels(views[v]));
+            reuse_px = std::max<uint64_t>(reuse_px, (uint64_t)views[v].width * views[v].height);
+        }
+    L.reuse_samples = a.take(16u * reuse_fb);
+    L.reuse_weights = a.take(16u * reuse_fb);
+    L.reuse_temp = a.take(24u * reuse_px);
     L.total = a.offset;
     return L;
 }
@@ -694,6 +713,11 @@ int check_views(const sgs_view* views, uint32_t n)
             return fail(SGS_ERROR_INVALID_ARGUMENT, at + "invalid temporal mode " + std::to_string(v.temporal_mode));
         if (!v.out.image || !v.out.depth || !v.out.alpha || !v.out.weight_mass)
             return fail(SGS_ERROR_INVALID_ARGUMENT, at + "output pointers are NULL");
+        if (v.blue_noise_numel != 0 && v.blue_noise_numel != 64u * 128u * 128u)
+            return fail(SGS_ERROR_INVALID_ARGUMENT, at + "blue_noise must hold 0 or 64*128*128 elements");
+        if (v.blue_noise_numel != 0 && !v.blue_noise) return fail(SGS_ERROR_INVALID_ARGUMENT, at + "blue_noise is NULL");
+        if (v.spatial_reuse && (!v.k_buffer_primitives || !v.k_buffer_occlusion))
+            return fail(SGS_ERROR_INVALID_ARGUMENT, at + "spatial reuse needs the k-buffer");
         if (v.temporal_mode != SGS_TEMPORAL_RESET &&
             (!v.history.image || !v.history.depth || !v.history.alpha || !v.history.weight_mass))
             return fail(SGS_ERROR_INVALID_ARGUMENT, at + "STATIC and REPROJECT need a history");
@@ -725,6 +749,15 @@ int check_views(const sgs_view* views, uint32_t n)
                                 "; a history may only be all outputs of an earlier view");
         }
     }
+    // a k-buffer is updated sample by sample in view order: views may share one (consecutive
+    // frames of a stream) only in resolve order, which view_levels keeps for history chains
+    for (uint32_t i = 0; i < n; i++)
+        for (uint32_t j = i + 1; j < n; j++)
+            if (views[i].spatial_reuse && views[j].spatial_reuse &&
+                (views[i].k_buffer_primitives == views[j].k_buffer_primitives) != (views[i].k_buffer_occlusion ==
+                                                                                    views[j].k_buffer_occlusion))
+                return fail(SGS_ERROR_INVALID_ARGUMENT,
+                            "views " + std::to_string(i) + " and " + std::to_string(j) + " share half a k-buffer");
     // one rasteriser workgroup per 16x16 tile of every view's bins (at least as many as bins)
     uint64_t groups = 0;
     for (uint32_t i = 0; i < n; i++) groups += view_raster_groups(views[i]);
@@ -769,6 +802,8 @@ void encode_views(const sgs_view* views, uint32_t n, void* host)
         x.bin_log = view_bin_log(v);
         x.raster_begin = group;
         x.level = level[i];
+        x.route = view_routed(v) ? 1u : 0u;
+        x.reuse = v.spatial_reuse ? 1u : 0u;
         x.fb_offset = fb_off;
         x.samples_offset = smp_off;
         x.hist = { v.history.image, v.history.depth, v.history.alpha, v.history.weight_mass };
@@ -852,6 +887,51 @@ int accumulate_tiles_single(const sgs_gaussians* g, const sgs_settings* s, const
     e = launch_batch_accumulate(sc, desc, 1, fb, counters, reinterpret_cast<BigEntry*>(tile_ws + L.queue),
                                 reinterpret_cast<uint32_t*>(tile_ws + L.ready), bt, &counters->overflow, stream);
     if (e != hipSuccess) return hip_fail(e, "scatter fallback");
+    return SGS_SUCCESS;
+}
+
+// Views with spatial reuse at resolve level l: every sample in order, as sgs_render's loop
+// would run it (resolve.cu:355-444): sample_cu updates the view's k-buffer and writes the
+// sample's colours and reuse weights, resolve_cu folds them into the previous sample's result
+// (the view's history for sample 0, in the view's temporal mode; STATIC after). Results
+// alternate between the view's outputs and a workspace buffer so the last lands in the outputs.
+int resolve_reuse_views(const SceneArgs& sc, const sgs_view* views, uint32_t n_views, const char* table,
+                        const std::vector<uint32_t>& level, uint32_t l, const uint64_t* fb, char* ws,
+                        const BatchLayout& L, hipStream_t stream)
+{
+    uint64_t fb_off = 0;
+    for (uint32_t i = 0; i < n_views; i++) {
+        const sgs_view& v = views[i];
+        const uint64_t P = view_fb_pixels(v);
+        if (v.spatial_reuse && level[i] == l) {
+            const FbCamera cam = make_fb_camera(v.width, v.height, v.focal_x, v.focal_y, v.center_x, v.center_y,
+                                                v.near_plane, v.far_plane, v.proper_antialiasing, v.supersampling_factor);
+            const char* d = table + (size_t)i * sizeof(ViewDesc);
+            const float* w2c = reinterpret_cast<const float*>(d + offsetof(ViewDesc, r1));
+            const float* cam_pos = reinterpret_cast<const float*>(d + offsetof(ViewDesc, cam_x));
+            const float* v2p = reinterpret_cast<const float*>(d + offsetof(ViewDesc, v2p));
+            float4* smp = reinterpret_cast<float4*>(ws + L.reuse_samples);
+            float4* rw = reinterpret_cast<float4*>(ws + L.reuse_weights);
+            const size_t Q = (size_t)v.width * v.height;
+            float* tmp = reinterpret_cast<float*>(ws + L.reuse_temp);
+            const OutputPtrs out = { v.out.image, v.out.depth, v.out.alpha, v.out.weight_mass };
+            const OutputPtrs temp = { tmp, tmp + 3u * Q, tmp + 4u * Q, tmp + 5u * Q };
+            HistoryPtrs hist = { v.history.image, v.history.depth, v.history.alpha, v.history.weight_mass };
+            for (uint32_t k = 0; k < v.spp; k++) {
+                hipError_t e = launch_sample(sc, w2c, cam_pos, cam, fb + fb_off + (uint64_t)k * P, smp, rw,
+                                             reinterpret_cast<uint2*>(v.k_buffer_primitives),
+                                             reinterpret_cast<float4*>(v.k_buffer_occlusion), true, stream);
+                if (e != hipSuccess) return hip_fail(e, "batched spatial-reuse sample");
+                const OutputPtrs o = ((v.spp - 1u - k) % 2u == 0u) ? out : temp;
+                const int32_t mode = k == 0 ? v.temporal_mode : SGS_TEMPORAL_STATIC;
+                e = launch_resolve(cam, v.width, v.height, v.focal_x, v.focal_y, v.center_x, v.center_y, smp, rw,
+                                   hist, v2p, o, true, mode, stream);
+                if (e != hipSuccess) return hip_fail(e, "batched spatial-reuse resolve");
+                hist = { o.image, o.depth, o.alpha, o.weight_mass };
+            }
+        }
+        fb_off += P * v.spp;
+    }
     return SGS_SUCCESS;
 }
 
@@ -963,13 +1043,46 @@ int sgs_render_batch(const sgs_gaussians* gaussians, const sgs_view* views, uint
         e = launch_batch_accumulate(sc, desc, n_views, fb, counters, queue, ready, t, nullptr, stream);
         if (e != hipSuccess) return hip_fail(e, "batched accumulation");
     }
+    // views with GPS or blue noise: each sample by the single-view scatter kernels, as
+    // sgs_render would (cleared framebuffer, the sample's frame, its blue-noise slice)
+    {
+        uint64_t fb_off = 0;
+        for (uint32_t i = 0; i < n_views; i++) {
+            const sgs_view& v = views[i];
+            const uint64_t P = view_fb_pixels(v);
+            if (view_routed(v)) {
+                const FbCamera cam = make_fb_camera(v.width, v.height, v.focal_x, v.focal_y, v.center_x, v.center_y,
+                                                    v.near_plane, v.far_plane, v.proper_antialiasing,
+                                                    v.supersampling_factor);
+                const float* w2c = reinterpret_cast<const float*>(table + (size_t)i * sizeof(ViewDesc) +
+                                                                  offsetof(ViewDesc, r1));
+                for (uint32_t smp = 0; smp < v.spp; smp++) {
+                    uint64_t* fbs = fb + fb_off + (uint64_t)smp * P;
+                    const uint32_t frame = v.frame_idx + smp;
+                    e = launch_fill_u64(fbs, P, kEmpty, stream);
+                    if (e != hipSuccess) return hip_fail(e, "framebuffer clear");
+                    if (v.use_gps) {
+                        e = launch_gps_scatter(sc, w2c, cam, fbs, frame, gps_max_points(cam, v.max_gps_points),
+                                               t.sequential_threshold, BinArgs{}, nullptr, stream);
+                    } else {
+                        // one blue-noise slice per frame, rotated by the golden ratio per pass (render.cu:77-82)
+                        e = launch_st_scatter(sc, w2c, cam, fbs, frame, v.blue_noise + (size_t)(frame % 64u) * 128u * 128u,
+                                              (frame / 64u) * 0x9E3779B9u, t.sequential_threshold, BinArgs{},
+                                              nullptr, stream);
+                    }
+                    if (e != hipSuccess) return hip_fail(e, "batched view scatter");
+                }
+            }
+            fb_off += P * v.spp;
+        }
+    }
     if ((e = mark(3)) != hipSuccess) return hip_fail(e, "stage event");
     // sample 0 of REPROJECT views needs the full sample buffer (3x3 neighbourhood moments)
     uint64_t fb_off = 0, smp_off = 0;
     for (uint32_t i = 0; i < n_views; i++) {
         const sgs_view& v = views[i];
         const uint64_t P = view_fb_pixels(v);
-        if (v.temporal_mode == SGS_TEMPORAL_REPROJECT) {
+        if (v.temporal_mode == SGS_TEMPORAL_REPROJECT && !v.spatial_reuse) {
             const FbCamera cam = make_fb_camera(v.width, v.height, v.focal_x, v.focal_y, v.center_x, v.center_y,
                                                 v.near_plane, v.far_plane, v.proper_antialiasing,
                                                 v.supersampling_factor);
@@ -978,8 +1091,8 @@ int sgs_render_batch(const sgs_gaussians* gaussians, const sgs_view* views, uint
                               reinterpret_cast<const float*>(d + offsetof(ViewDesc, cam_x)), cam, fb + fb_off,
                               samples + smp_off, nullptr, nullptr, nullptr, false, stream);
             if (e != hipSuccess) return hip_fail(e, "batched sample");
-            smp_off += P;
         }
+        if (v.temporal_mode == SGS_TEMPORAL_REPROJECT) smp_off += P;
         fb_off += P * v.spp;
     }
     const std::vector<uint32_t> level = view_levels(views, n_views);
@@ -987,6 +1100,7 @@ int sgs_render_batch(const sgs_gaussians* gaussians, const sgs_view* views, uint
     for (uint32_t l = 0; l < levels; l++) {
         e = launch_batch_resolve(sc, desc, n_views, prefix, L.n_blocks, fb, samples, l, stream);
         if (e != hipSuccess) return hip_fail(e, "batched resolve");
+        if ((rc = resolve_reuse_views(sc, views, n_views, table, level, l, fb, ws, L, stream))) return rc;
     }
     if ((e = mark(4)) != hipSuccess) return hip_fail(e, "stage event");
     return SGS_SUCCESS;

@@ -189,6 +189,7 @@ __global__ __launch_bounds__(kBatchBlock) void batch_accumulate_kernel(SceneArgs
             v = item - chunk * n_views;
         }
         const ViewDesc& vd = views[v];
+        if (vd.route) continue;  // accumulated by the single-view scatter kernels
         const ViewRun vr = view_run(vd, fb);
         const uint32_t g = chunk * kBatchBlock + tid;
         Projected p;
@@ -511,6 +512,11 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
         uint2 next_rect = make_uint2(kNoTiles, kNoTiles);
         if (kFill && g < sc.n) next_rect = pair_rect[g];
         for (uint32_t v = 0; v < n_views; v++) {
+            if (views[v].route) {
+                // a view without bins: only keep the fill pass's one-view-ahead record read going
+                if (kFill && g < sc.n && v + 1u < n_views) next_rect = pair_rect[(size_t)(v + 1u) * sc.n + g];
+                continue;
+            }
             if (tid == 0) {
                 s_wx0 = 0xffffffffu;
                 s_wy0 = 0xffffffffu;

@@ -132,6 +132,9 @@ struct alignas(16) ViewDesc {
     uint32_t bin_log;           // bins of (16 << bin_log) pixels
     uint32_t raster_begin;      // first rasteriser workgroup (16x16 tile) of this view
     uint32_t level;             // resolve order: 1 + the level of the view whose output is this view's history
+    uint32_t route;             // 1: samples accumulated one by one by the single-view scatter kernels
+                                // (GPS, blue noise); the batch's binning, rasteriser and scatter skip it
+    uint32_t reuse;             // 1: spatial reuse, resolved sample by sample (the batched resolve skips it)
     uint64_t fb_offset;         // first framebuffer word of sample 0 (samples are P words apart)
     uint64_t samples_offset;    // REPROJECT views: first float4 of the sample-0 buffer
     HistoryPtrs hist;

@@ -558,8 +558,9 @@ __global__ __launch_bounds__(kBlockResolve) void batch_resolve_kernel(SceneArgs 
         else hi = mid - 1u;
     }
     const ViewDesc& vd = views[lo];
-    // views whose history is another view's output of the batch resolve in a later launch
-    if (vd.level != level) return;
+    // views whose history is another view's output of the batch resolve in a later launch;
+    // views with spatial reuse are resolved sample by sample
+    if (vd.level != level || vd.reuse) return;
     // 16x16 tile t of the view's image pixels, row-major over ceil(W/16) tiles per row
     const uint32_t t = blockIdx.x - block_prefix[lo], tw = (vd.width + kResolveTile - 1u) / kResolveTile;
     const uint32_t px = (t % tw) * kResolveTile + threadIdx.x % kResolveTile;

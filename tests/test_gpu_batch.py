@@ -307,10 +307,20 @@ def test_streamed_batches_equal_one_batch(gpu, streams):
     assert [drawn[i + 1] - drawn[i] for i in range(len(auto) - 1)] == steps
 
 
-def test_batch_rejects_unsupported_views(gpu):
+def test_batch_rejects_incomplete_feature_views(gpu):
+    """Spatial reuse without the view's k-buffer, a k-buffer of the wrong size and a blue-noise
+    volume of the wrong size are refused before anything runs."""
+    import torch
     from StochasticGSCudaBackend.batch import BatchView, render_batch
     arrays = small_scene(10)
     t = to_device(arrays)
-    s = settings_of(small_camera(16, 16))._replace(use_gps=True)
-    with pytest.raises(ValueError):
-        render_batch(t["means"], t["scales"], t["rotations"], t["opacities"], t["sh"], [BatchView(s)])
+    s = settings_of(small_camera(16, 16))._replace(spatial_reuse=True)
+    args = (t["means"], t["scales"], t["rotations"], t["opacities"], t["sh"])
+    with pytest.raises(ValueError, match="k-buffer"):
+        render_batch(*args, [BatchView(s)])
+    small = (torch.full((4, 10, 2), -1, dtype=torch.int32, device="cuda"), torch.zeros((4, 10, 4), device="cuda"))
+    with pytest.raises(ValueError, match="k-buffer"):
+        render_batch(*args, [BatchView(s, k_buffer=small)])
+    with pytest.raises(ValueError, match="blue_noise"):
+        render_batch(*args, [BatchView(settings_of(small_camera(16, 16)),
+                                       blue_noise=torch.zeros(100, dtype=torch.uint8, device="cuda"))])
