@@ -79,6 +79,9 @@ def parse() -> argparse.Namespace:
                     help="batches on the GPU at once, one serving run per value (the first is the headline)")
     ap.add_argument("--serve-window-ms", type=float, default=35.0,
                     help="batching window: a free slot waits until the oldest request is this old")
+    ap.add_argument("--serve-temporal-only", action="store_true",
+                    help="streams resolve with REPROJECT alone instead of the spatio-temporal resolve (spatial "
+                         "reuse + REPROJECT)")
     ap.add_argument("--only", choices=["plain", "batched", "all"], default="all",
                     help="run only one path; the JSON line then reports that path")
     ap.add_argument("--schedule", choices=["tiles", "scatter"], default="tiles",
@@ -700,7 +703,7 @@ SERVE_SIZES = [(1280, 720), (1920, 1080), (2560, 1440)]
 SERVE_SPP = [1, 2, 8]
 
 
-def serving_streams(n_streams: int, seed: int = 3):
+def serving_streams(n_streams: int, seed: int = 3, spatial_reuse: bool = True):
     import numpy as np
     from StochasticGSCudaBackend.serve import StreamSpec
     from StochasticGSCudaBackend.synthetic import camera, orbit_eye
@@ -710,7 +713,7 @@ def serving_streams(n_streams: int, seed: int = 3):
         w, h = SERVE_SIZES[int(rng.integers(len(SERVE_SIZES)))]
         c = camera(orbit_eye(0.0, 3.0), w, h)
         specs.append(StreamSpec(w, h, c["focal_x"], c["focal_y"], c["center_x"], c["center_y"], c["near_plane"],
-                                c["far_plane"]))
+                                c["far_plane"], spatial_reuse=spatial_reuse))
         # orbit phase, angular speed (rad/s), radius, height amplitude
         paths.append((rng.uniform(0, 2 * math.pi), rng.uniform(0.2, 0.6) * rng.choice([-1, 1]),
                       rng.uniform(2.6, 3.4), rng.uniform(-0.3, 0.3)))
@@ -744,7 +747,8 @@ def bench_serving(args, scene, dev, inflight: int, with_plain: bool):
     import torch
     from StochasticGSCudaBackend import _native
     from StochasticGSCudaBackend.serve import Server, latency_stats
-    specs, paths = serving_streams(args.serve_streams)
+    reuse = not args.serve_temporal_only
+    specs, paths = serving_streams(args.serve_streams, spatial_reuse=reuse)
     server = Server(scene["means"], scene["scales"], scene["rotations"], scene["opacities"], scene["sh"], specs,
                     max_views=args.serve_streams, inflight=inflight, max_wait=args.serve_window_ms * 1e-3)
     warm = serving_requests(specs, paths, [0.0] * args.serve_streams, seed=10)
@@ -771,6 +775,7 @@ def bench_serving(args, scene, dev, inflight: int, with_plain: bool):
     log(f"serving (inflight {inflight}) at {rate:.1f} req/s: p50 {stats['p50_ms']} ms, p99 {stats['p99_ms']} ms, "
         f"{len(done) / batches:.1f} requests per batch")
     out = {"streams": args.serve_streams, "inflight": inflight, "window_ms": args.serve_window_ms,
+           "resolve": "spatio-temporal (spatial reuse + REPROJECT)" if reuse else "temporal (REPROJECT)",
            "saturated_req_per_s": round(sat_rate, 1), "offered_req_per_s": round(rate, 1),
            "completed_req_per_s": round(len(done) / run_s, 1), "msamples_per_s": round(samples / run_s / 1e6, 1),
            "requests": len(done), "requests_per_batch": round(len(done) / max(batches, 1), 2),
@@ -778,9 +783,8 @@ def bench_serving(args, scene, dev, inflight: int, with_plain: bool):
            "p99_over_p50": round(stats["p99_ms"] / max(stats["p50_ms"], 1e-9), 3), "overflowed_batches": overflowed}
     if with_plain:
         sample_reqs = load[:min(len(load), 96)]
-        hist_cache = {}
+        hist_cache, kb_cache = {}, {}
         service = []
-        kb = (torch.empty((0, 0, 2), dtype=torch.int32, device=dev), torch.empty((0, 0, 4), device=dev))
         eye4 = torch.eye(4, device=dev)
         noise = torch.empty(0, dtype=torch.uint8, device=dev)
         torch.cuda.synchronize()
@@ -790,6 +794,13 @@ def bench_serving(args, scene, dev, inflight: int, with_plain: bool):
                 hist_cache[(sp.width, sp.height)] = tuple(torch.zeros((sp.height, sp.width, c), device=dev)
                                                           for c in (3, 1, 1, 1))
             hist = hist_cache[(sp.width, sp.height)]
+            if (sp.width, sp.height) not in kb_cache:
+                P, K = sp.width * sp.height, _native.k_buffer_size
+                kb_cache[(sp.width, sp.height)] = (
+                    (torch.full((K, P, 2), -1, dtype=torch.int32, device=dev), torch.zeros((K, P, 4), device=dev))
+                    if reuse else (torch.empty((0, 0, 2), dtype=torch.int32, device=dev),
+                                   torch.empty((0, 0, 4), device=dev)))
+            kb = kb_cache[(sp.width, sp.height)]
             w2c = torch.from_numpy(np.asarray(r.w2c, np.float32)).to(dev)
             cp = torch.from_numpy(np.asarray(r.cam_position, np.float32)).to(dev)
             torch.cuda.synchronize()
@@ -798,7 +809,7 @@ def bench_serving(args, scene, dev, inflight: int, with_plain: bool):
             for k in range(r.spp):
                 h = _native.render(scene["means"], scene["scales"], scene["rotations"], scene["opacities"],
                                    scene["sh"], w2c, cp, sp.width, sp.height, sp.focal_x, sp.focal_y, sp.center_x,
-                                   sp.center_y, sp.near_plane, sp.far_plane, False, 1, False, False, False, noise,
+                                   sp.center_y, sp.near_plane, sp.far_plane, False, 1, False, False, reuse, noise,
                                    kb[0], kb[1], *h, eye4, 2 if k == 0 else 1, frame_idx=r.frame_idx + k)
             torch.cuda.synchronize()
             service.append(time.perf_counter() - ts)

@@ -8,8 +8,9 @@
   SPP loop of render() calls (Renderer.py:85-87); for a few views 256x256 framebuffer crops
   bit-exact against the oracle and the resolve of the full framebuffer within 1e-5.
 * Serving over the same scene: 32 camera streams at 720p/1080p/1440p, spp in {1, 2, 8},
-  Poisson arrivals, REPROJECT with per-stream history, two batches in flight: every frame of
-  a sample of streams equals rendering that stream alone, bit for bit.
+  Poisson arrivals, the spatio-temporal resolve (spatial reuse over each stream's k-buffer,
+  REPROJECT against its history), two batches in flight: every frame of a sample of streams
+  equals rendering that stream alone, outputs and k-buffers bit for bit.
 """
 from __future__ import annotations
 
@@ -21,6 +22,7 @@ import pytest
 
 from conftest import oracle_scene
 from test_gpu_batch import assert_close, plain_spp, settings_of
+from test_gpu_batch_features import empty_k_buffer, plain_loop
 from test_gpu_parity import _gpu_render, gpu_fb, print_case, to_device
 
 pytestmark = pytest.mark.gpu
@@ -132,8 +134,9 @@ def test_c4_batched_full_size(gpu, oracle_mod, scene6m):
 
 def test_c5_serving_full_size(gpu, scene6m):
     """32 streams over the 6M scene, a few hundred Poisson requests, two batches in flight:
-    each sampled stream's frames equal the stream rendered alone (RESET, then REPROJECT
-    against its previous frame)."""
+    each sampled stream's frames equal the stream rendered alone with spatial reuse over its
+    own k-buffer (RESET, then REPROJECT against its previous frame), and the served k-buffer
+    equals the one the alone loop leaves."""
     import torch
     from StochasticGSCudaBackend.serve import Request, Server, StreamSpec
     from StochasticGSCudaBackend.synthetic import camera, orbit_eye
@@ -146,7 +149,7 @@ def test_c5_serving_full_size(gpu, scene6m):
         w, h = sizes[int(rng.integers(3))]
         c = camera(orbit_eye(0.0, 3.0), w, h)
         specs.append(StreamSpec(w, h, c["focal_x"], c["focal_y"], c["center_x"], c["center_y"], c["near_plane"],
-                                c["far_plane"]))
+                                c["far_plane"], spatial_reuse=True))
         paths.append((rng.uniform(0, 2 * math.pi), rng.uniform(0.2, 0.6) * rng.choice([-1, 1]),
                       rng.uniform(2.6, 3.4)))
     times = np.cumsum(rng.exponential(1.0 / 300.0, n_req))
@@ -154,7 +157,8 @@ def test_c5_serving_full_size(gpu, scene6m):
     for tt in times:
         s = int(rng.integers(n_streams))
         phase, omega, radius = paths[s]
-        c = camera(orbit_eye(phase + omega * tt, radius, 0.1 * math.sin(tt)), specs[s].width, specs[s].height)
+        c = camera(orbit_eye(phase + omega * tt, radius, 0.1 * math.sin(tt)), specs[s].width, specs[s].height,
+                   spatial_reuse=True)
         reqs.append(Request(stream=s, w2c=c["w2c"], cam_position=c["cam_position"], spp=int(rng.choice([1, 2, 8])),
                             arrival=float(tt)))
         cams[id(reqs[-1])] = c
@@ -174,6 +178,7 @@ def test_c5_serving_full_size(gpu, scene6m):
     for s in sample:
         frames = sorted(by_stream[s], key=lambda r: r.frame)
         hist, prev = None, None
+        kb = empty_k_buffer(cams[id(frames[0])])
         for r in frames:
             c = cams[id(r)]
             v2p = None
@@ -181,9 +186,12 @@ def test_c5_serving_full_size(gpu, scene6m):
                 v2p = (np.asarray(prev["w2c"], np.float64) @ np.linalg.inv(np.asarray(c["w2c"], np.float64))).astype(
                     np.float32)
             print_case(test="c5_alone", stream=s, frame=r.frame, spp=r.spp, width=c["width"])
-            ref = plain_spp(gpu, t, c, r.frame_idx, r.spp, mode=0 if r.frame == 0 else 2, history=hist, v2p=v2p)
+            ref = plain_loop(gpu, t, c, r.frame_idx, r.spp, mode=0 if r.frame == 0 else 2, history=hist, v2p=v2p,
+                             kb=kb)
             torch.cuda.synchronize()
             assert_close(r.output, ref)
             hist, prev = ref, c
+        k_served = server.streams[s].k_buffer
+        assert torch.equal(k_served[0], kb[0]) and torch.equal(k_served[1], kb[1])
     del server
     torch.cuda.empty_cache()
