@@ -38,6 +38,8 @@ class TuneResult:
     views: list | None = None  # tune_batch: the views with the chosen bin sizes
     streams: int = 1  # tune_batch: view groups on concurrent streams (1: one BatchRenderer)
     part_tunings: list | None = None  # tune_batch with streams > 1: one BatchTuning per group
+    # tune_batch's tile instance capacities are measured on the tuned poses; a render from
+    # other poses that exceeds them falls back to the (bit-identical, much slower) scatter path
 
 
 def coordinate_search(evaluate: Callable[[dict], float], space: dict[str, Sequence], start: dict,
@@ -157,7 +159,14 @@ def tune_batch(means, scales, rotations, opacities, sh_coefficients, views, *, r
                space: dict | None = None, capacity_margin: float = 1.25) -> TuneResult:
     """Fastest BatchTuning for rendering ``views`` (BatchView list) in one call; the tile
     instance capacity of the result (of each group's, with streams > 1) is the measured
-    instance count x ``capacity_margin``."""
+    instance count x ``capacity_margin``.
+
+    That capacity belongs to the tuned camera poses: other poses over the same scene can need
+    more tile instances, and a batch that exceeds the capacity still renders the same bits but
+    through the scatter fallback, an order of magnitude slower. Re-tune when the poses change
+    materially, check ``tile_stats()[2]`` (overflow) after a render, or clear the capacity
+    (``max_tile_instances=0``: the library's per-pixel default, usually larger) to trade
+    memory for headroom."""
     from .batch import BatchRenderer, BatchTuning, StreamedBatchRenderer
     space = dict(BATCH_SPACE if space is None else space)
 
