@@ -374,6 +374,85 @@ __device__ __forceinline__ bool gaussian_tiles_reg(float mx, float my, float mz,
     return true;
 }
 
+// A superset of the tile range gaussian_tiles_reg gives, at about half its cost, for the
+// counting pass. The near/far test and the numerators of the projected mean are the exact
+// operations; the divisions by the depth go through the hardware reciprocal (within 1 ulp),
+// the diagonal of the 2D covariance is evaluated with fused multiply-adds, the determinant and
+// antialiasing culls are skipped (a culled Gaussian only costs instances), and the AABB is
+// widened by bounds on the distance of every approximate value from the exact one:
+// * x = X / depth: the reciprocal and the product are within 2^-21 |x| of the exact quotient.
+// * cov_xx = a^T C a with a = j11 (r1 - xc r3): both evaluations, from Jacobians within 2^-20
+//   |j11| (|r1| + |xc| |r3|) + |j11| |r3| dx of each other, differ by at most
+//   tr(C) j11^2 S (2^-18 S + 2 dx |r3|) with S >= |r1| + (|xc| + dx) |r3|: by Cauchy-Schwarz,
+//   |u^T C v| <= |u| |v| tr(C) for the positive semidefinite C, and the roundings of the two
+//   six-operation dot products add at most 12 2^-24 |a|^2 tr(C).
+// * the cutoff: the cached log(255 opacity) is at least the per-view value (antialiasing only
+//   lowers the opacity); the half-width sqrt((cov_xx + kernel) cutoff) is scaled by 1 + 2^-16.
+// * the mean and the edges: 2^-20 (|x fx| + |mean| + half-width) covers the quotient's error
+//   through the mean and every later rounding.
+// Row norms are bounded by their L1 norms. A Gaussian whose bounds are not finite takes the
+// exact path. The rasteriser projects exactly and clips every instance to its exact AABB, so
+// an extra tile changes no result.
+__device__ __forceinline__ bool gaussian_tiles_bound(float mx, float my, float mz, const Cov3d& cv, float trace,
+                                                     bool valid, const ViewDesc& vd, uint32_t shift, TileRange& t,
+                                                     bool& exact)
+{
+    exact = false;
+    if (!valid) return false;
+    const float4 r1 = vd.r1, r2 = vd.r2, r3 = vd.r3;
+    const FbCamera& cam = vd.cam;
+    const float depth = r3.x * mx + r3.y * my + r3.z * mz + r3.w;
+    if (!(depth >= cam.near_plane && depth <= cam.far_plane)) return false;
+    const float X = r1.x * mx + r1.y * my + r1.z * mz + r1.w;
+    const float Y = r2.x * mx + r2.y * my + r2.z * mz + r2.w;
+    const float rz = __builtin_amdgcn_rcpf(depth);
+    const float x = X * rz, y = Y * rz;
+    const float dx = fabsf(x) * 0x1p-21f, dy = fabsf(y) * 0x1p-21f;
+    const float xc = fmaxf(cam.clip_left, fminf(x, cam.clip_right));
+    const float yc = fmaxf(cam.clip_top, fminf(y, cam.clip_bottom));
+    const float j11 = cam.focal_x * rz, j22 = cam.focal_y * rz;
+    const float j13 = -j11 * xc, j23 = -j22 * yc;
+    const float a1 = fmaf(j13, r3.x, j11 * r1.x), a2 = fmaf(j13, r3.y, j11 * r1.y), a3 = fmaf(j13, r3.z, j11 * r1.z);
+    const float b1 = fmaf(j23, r3.x, j22 * r2.x), b2 = fmaf(j23, r3.y, j22 * r2.y), b3 = fmaf(j23, r3.z, j22 * r2.z);
+    const float p1 = fmaf(a3, cv.c13, fmaf(a2, cv.c12, a1 * cv.c11));
+    const float p2 = fmaf(a3, cv.c23, fmaf(a2, cv.c22, a1 * cv.c12));
+    const float p3 = fmaf(a3, cv.c33, fmaf(a2, cv.c23, a1 * cv.c13));
+    const float q1 = fmaf(b3, cv.c13, fmaf(b2, cv.c12, b1 * cv.c11));
+    const float q2 = fmaf(b3, cv.c23, fmaf(b2, cv.c22, b1 * cv.c12));
+    const float q3 = fmaf(b3, cv.c33, fmaf(b2, cv.c23, b1 * cv.c13));
+    const float cxx = fmaf(p3, a3, fmaf(p2, a2, p1 * a1));
+    const float cyy = fmaf(q3, b3, fmaf(q2, b2, q1 * b1));
+    // L1 bounds of the rotation rows' Euclidean norms, rounded up
+    const float n1 = (fabsf(r1.x) + fabsf(r1.y) + fabsf(r1.z)) * (1.0f + 0x1p-20f);
+    const float n2 = (fabsf(r2.x) + fabsf(r2.y) + fabsf(r2.z)) * (1.0f + 0x1p-20f);
+    const float n3 = (fabsf(r3.x) + fabsf(r3.y) + fabsf(r3.z)) * (1.0f + 0x1p-20f);
+    const float sx = fmaf(fabsf(xc) + dx, n3, n1), sy = fmaf(fabsf(yc) + dy, n3, n2);
+    const float tr = trace * (1.0f + 0x1p-16f);
+    const float exx = tr * (j11 * j11) * sx * fmaf(sx, 0x1p-18f, 2.0f * dx * n3);
+    const float eyy = tr * (j22 * j22) * sy * fmaf(sy, 0x1p-18f, 2.0f * dy * n3);
+    const float kernel = (cam.proper_antialiasing ? 0.1f : 0.3f) * cam.ss * cam.ss;
+    const float cut = fmaf(fmaxf(2.0f * cv.pt, 0.0f), 1.0f + 0x1p-16f, 0x1p-20f);
+    const float ex = __builtin_amdgcn_sqrtf(((cxx + exx) + kernel) * cut) * (1.0f + 0x1p-16f);
+    const float ey = __builtin_amdgcn_sqrtf(((cyy + eyy) + kernel) * cut) * (1.0f + 0x1p-16f);
+    const float mean_x = x * cam.focal_x + cam.center_x, mean_y = y * cam.focal_y + cam.center_y;
+    const float wx = ex + fmaf(fabsf(x * cam.focal_x) + fabsf(mean_x) + ex, 0x1p-20f, 0x1p-20f);
+    const float wy = ey + fmaf(fabsf(y * cam.focal_y) + fabsf(mean_y) + ey, 0x1p-20f, 0x1p-20f);
+    const float bx0 = mean_x - wx, bx1 = mean_x + wx, by0 = mean_y - wy, by1 = mean_y + wy;
+    if (!(wx <= 3.0e38f && wy <= 3.0e38f && fabsf(mean_x) <= 3.0e38f && fabsf(mean_y) <= 3.0e38f)) {
+        exact = true;
+        return false;
+    }
+    if (bx0 >= cam.width || bx1 < 0.0f || by0 >= cam.height || by1 < 0.0f) return false;
+    const uint32_t x0 = (uint32_t)floorf(fmaxf(bx0, 0.0f)), x1 = (uint32_t)ceilf(fminf(bx1, cam.width));
+    const uint32_t y0 = (uint32_t)floorf(fmaxf(by0, 0.0f)), y1 = (uint32_t)ceilf(fminf(by1, cam.height));
+    if (x1 <= x0 || y1 <= y0) return false;
+    t.tx0 = x0 >> shift;
+    t.tx1 = (x1 - 1u) >> shift;
+    t.ty0 = y0 >> shift;
+    t.ty1 = (y1 - 1u) >> shift;
+    return true;
+}
+
 // Depth buckets of the per-bin sort (tile_sort_kernel): the fill pass stores each instance's
 // bucket, the sort counts and places by it.
 constexpr uint32_t kSortBins = 1024;
@@ -398,13 +477,35 @@ __device__ __forceinline__ uint32_t bucket_floor(uint32_t b, uint2 vk)
 // their wave's lanes with one global atomic per tile.
 constexpr uint32_t kSmallTiles = 32;
 
+// Bins of 2^b x 2^b tiles: an instance word carries, above the Gaussian index, the
+// sub-rectangle [x0, x1] x [y0, y1] of its bin's tiles that the pair's tile rectangle touches
+// (4b bits), so each tile's rasteriser projects only the instances that reach it. Stored when
+// the index leaves the room (n <= 2^(32 - 4b)); returns the bit position (32: not stored).
+__host__ __device__ __forceinline__ uint32_t subrect_shift(uint32_t bin_log, uint32_t n)
+{
+    if (bin_log == 0u) return 32u;
+    const uint32_t s = 32u - 4u * bin_log;
+    return (uint64_t)n <= (1ull << s) ? s : 32u;
+}
+
+// instance word of pair (t16: its rectangle of 16x16 tiles) in bin (bx, by)
+__device__ __forceinline__ uint32_t instance_word(uint32_t g, uint32_t bx, uint32_t by, const TileRange& t16,
+                                                  uint32_t bin_log, uint32_t shift)
+{
+    if (shift >= 32u) return g;
+    const uint32_t m = (1u << bin_log) - 1u;
+    const uint32_t x0 = max(t16.tx0, bx << bin_log) & m, x1 = min(t16.tx1, (bx << bin_log) | m) & m;
+    const uint32_t y0 = max(t16.ty0, by << bin_log) & m, y1 = min(t16.ty1, (by << bin_log) | m) & m;
+    return g | ((x0 | (x1 << bin_log) | (y0 << (2u * bin_log)) | (y1 << (3u * bin_log))) << shift);
+}
+
 template <bool kFill>
-__device__ __forceinline__ void bin_direct(uint32_t gt, uint32_t g, uint32_t bucket, uint32_t* tile_count,
+__device__ __forceinline__ void bin_direct(uint32_t gt, uint32_t word, uint32_t bucket, uint32_t* tile_count,
                                            uint32_t* tile_cursor, const uint32_t* offsets, const EntryArrays& entries)
 {
     if constexpr (kFill) {
         const uint32_t slot = offsets[gt] + atomicAdd(&tile_cursor[gt], 1u);
-        entries.store(slot, bucket, g);
+        entries.store(slot, bucket, word);
     } else {
         atomicAdd(&tile_count[gt], 1u);
     }
@@ -491,8 +592,10 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
     // chunk on one address; the per-chunk work is nearly uniform). A lane loads its
     // Gaussian's parameters once and bins it for every view in turn, so the parameter
     // stream is read from memory once per pass instead of once per view. The counting pass
-    // projects and records each (view, Gaussian) pair's bin rectangle; the fill pass reads it
-    // back instead of projecting again, with the depth key recomputed from the mean.
+    // records each (view, Gaussian) pair's rectangle of 16x16 tiles (a superset of the exact
+    // one, from gaussian_tiles_bound); the fill pass reads it back instead of projecting again,
+    // with the depth key recomputed from the mean. Both bin the pair into the bins holding
+    // those tiles.
     for (uint32_t chunk = blockIdx.x; chunk < n_chunks; chunk += gridDim.x) {
         const uint32_t g = chunk * kBatchBlock + tid;
         float mx = 0.0f, my = 0.0f, mz = 0.0f;
@@ -507,6 +610,7 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
             cv = { a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w };
         }
         const bool valid = g < sc.n && !(cv.opacity < 0.0f);
+        const float trace = cv.c11 + cv.c22 + cv.c33;
         // the fill pass reads each view's pair record one view ahead, so the load of view v + 1
         // is in flight while view v is binned
         uint2 next_rect = make_uint2(kNoTiles, kNoTiles);
@@ -525,8 +629,9 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
             }
             __syncthreads();
             const ViewDesc& vd = views[v];
-            const uint32_t bin0 = vd.tile_begin, gw = vd.grid_w;
-            TileRange tr = { 0u, 0u, 0u, 0u };
+            const uint32_t bin0 = vd.tile_begin, gw = vd.grid_w, bl = vd.bin_log;
+            const uint32_t ms = subrect_shift(bl, sc.n);
+            TileRange t16 = { 0u, 0u, 0u, 0u };
             uint32_t key = 0;
             bool has = false;
             const size_t pair = (size_t)v * sc.n + g;
@@ -534,17 +639,20 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
                 const uint2 r = next_rect;
                 if (g < sc.n && v + 1u < n_views) next_rect = pair_rect[pair + sc.n];
                 has = r.x != kNoTiles;
-                tr = { r.x & 0xffffu, r.y & 0xffffu, r.x >> 16, r.y >> 16 };
+                t16 = { r.x & 0xffffu, r.y & 0xffffu, r.x >> 16, r.y >> 16 };
                 // the depth key again from the mean in registers (the operations of
                 // gaussian_tiles_reg, so the same bits), stored as its sort bucket
                 const float4 r3 = vd.r3;
                 key = sort_bucket(f2u(r3.x * mx + r3.y * my + r3.z * mz + r3.w), view_keys[v]);
             } else {
-                has = gaussian_tiles_reg(mx, my, mz, cv, valid, vd, kRasterLog2 + vd.bin_log, tr, key);
+                bool exact = false;
+                has = gaussian_tiles_bound(mx, my, mz, cv, trace, valid, vd, kRasterLog2, t16, exact);
+                if (exact) has = gaussian_tiles_reg(mx, my, mz, cv, valid, vd, kRasterLog2, t16, key);
                 if (g < sc.n)
-                    pair_rect[pair] = has ? make_uint2(tr.tx0 | (tr.ty0 << 16), tr.tx1 | (tr.ty1 << 16))
+                    pair_rect[pair] = has ? make_uint2(t16.tx0 | (t16.ty0 << 16), t16.tx1 | (t16.ty1 << 16))
                                           : make_uint2(kNoTiles, kNoTiles);
             }
+            const TileRange tr = { t16.tx0 >> bl, t16.tx1 >> bl, t16.ty0 >> bl, t16.ty1 >> bl };  // bins
             const uint32_t tw = has ? tr.tx1 - tr.tx0 + 1u : 0u;
             const uint32_t n_t = has ? tw * (tr.ty1 - tr.ty0 + 1u) : 0u;
             const bool small = has && n_t <= kSmallTiles;
@@ -575,9 +683,11 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
                 big &= big - 1ull;
                 const uint32_t bx0 = bcast(tr.tx0, l), by0 = bcast(tr.ty0, l), bw = bcast(tw, l), bn = bcast(n_t, l);
                 const uint32_t bg = bcast(g, l), bkey = bcast(key, l);
+                const TileRange b16 = { bcast(t16.tx0, l), bcast(t16.tx1, l), bcast(t16.ty0, l), bcast(t16.ty1, l) };
                 for (uint32_t k = lane; k < bn; k += 64u) {
                     const uint32_t ty = by0 + k / bw, tx = bx0 + k % bw;
-                    bin_direct<kFill>(bin0 + ty * gw + tx, bg, bkey, tile_count, tile_cursor, offsets, entries);
+                    bin_direct<kFill>(bin0 + ty * gw + tx, instance_word(bg, tx, ty, b16, bl, ms), bkey, tile_count,
+                                      tile_cursor, offsets, entries);
                 }
             }
             __syncthreads();
@@ -622,15 +732,15 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
                                 for (uint32_t tx = tr.tx0; tx <= tr.tx1; tx++) {
                                     const uint32_t i = ((ty - wy0) * wt + tx - wx0) * kHistCopies + hc;
                                     const uint32_t slot = hbase[i] + atomicAdd(&hist[i], 1u);
-                                    entries.store(slot, key, g);
+                                    entries.store(slot, key, instance_word(g, tx, ty, t16, bl, ms));
                                 }
                     }
                 } else if (small) {
                     // spread-out chunk: one global atomic per instance
                     for (uint32_t ty = tr.ty0; ty <= tr.ty1; ty++)
                         for (uint32_t tx = tr.tx0; tx <= tr.tx1; tx++)
-                            bin_direct<kFill>(bin0 + ty * gw + tx, g, key, tile_count, tile_cursor,
-                                              offsets, entries);
+                            bin_direct<kFill>(bin0 + ty * gw + tx, instance_word(g, tx, ty, t16, bl, ms), key,
+                                              tile_count, tile_cursor, offsets, entries);
                 }
             }
             __syncthreads();
@@ -942,6 +1052,8 @@ __global__ __launch_bounds__(kBatchBlock, kLayers <= 8u ? 5 : 3) void batch_tile
     __shared__ float s_ca[kBatchBlock], s_cb[kBatchBlock], s_cc[kBatchBlock];
     __shared__ uint32_t s_rect[kBatchBlock], s_end[kBatchBlock];
     __shared__ uint32_t s_wave[kBatchBlock / 64u];
+    __shared__ uint32_t s_g[kBatchBlock];            // the batch's instances that reach this tile
+    __shared__ uint32_t s_mine[kBatchBlock / 64u];   // their count per wave
     if (ctr->overflow) return;
     // Workgroups are handed to the eight XCDs round-robin (workgroup i to XCD i % 8), each XCD
     // with its own L2. Within every 32 workgroups, the four that share an XCD take four
@@ -975,6 +1087,9 @@ __global__ __launch_bounds__(kBatchBlock, kLayers <= 8u ? 5 : 3) void batch_tile
     const uint32_t ptx = ox + tid % kTile, pty = oy + tid / kTile;
     const bool inside = ptx < W && pty < H;
     uint64_t* fbv = fb + vd.fb_offset;
+    // the tile's position in its bin, for the instances' stored sub-rectangles
+    const uint32_t ms = subrect_shift(bl, sc.n), bm = (1u << bl) - 1u;
+    const uint32_t qx = q & bm, qy = q >> bl;
 
     for (uint32_t s0 = 0; s0 < vd.spp; s0 += kLayers) {
         const uint32_t G = min(kLayers, vd.spp - s0);
@@ -982,6 +1097,25 @@ __global__ __launch_bounds__(kBatchBlock, kLayers <= 8u ? 5 : 3) void batch_tile
         for (uint32_t s = 0; s < G; s++) layer[s][tid] = inside ? kEmpty : 0ull;
         const uint32_t fs_group = (vd.frame0 + s0) * 1973u;
         for (uint32_t b0 = begin; b0 < end; b0 += kBatchBlock) {
+            const uint32_t nb = min(kBatchBlock, end - b0);
+            // The batch's instances whose sub-rectangle holds this tile (every instance when
+            // none is stored) are compacted to the front, so the projection below runs on as
+            // few waves as they fill: a bin's tiles share its list, and about half of a list's
+            // Gaussians miss any one of its tiles.
+            uint32_t gi = 0;
+            bool mine = false;
+            if (tid < nb) {
+                const uint32_t w = instances[b0 + tid];
+                gi = ms < 32u ? w & ((1u << ms) - 1u) : w;
+                mine = true;
+                if (ms < 32u) {
+                    const uint32_t sr = w >> ms;
+                    mine = qx >= (sr & bm) && qx <= ((sr >> bl) & bm) && qy >= ((sr >> (2u * bl)) & bm) &&
+                           qy <= (sr >> (3u * bl));
+                }
+            }
+            const uint64_t mine_mask = __ballot(mine);
+            if (lane == 0u) s_mine[tid / 64u] = (uint32_t)__popcll(mine_mask);
             __syncthreads();
             // pixel and sub-tile maxima of the words so far
             {
@@ -994,16 +1128,22 @@ __global__ __launch_bounds__(kBatchBlock, kLayers <= 8u ? 5 : 3) void batch_tile
                 m = max(m, shfl_xor_u64(m, 32));
                 if ((lane & 51u) == 0u) s_sub[(tid / 64u) * (kTile / kSub) + (lane & 15u) / kSub] = m;
             }
+            uint32_t n_mine = 0, slot = (uint32_t)__popcll(mine_mask & ((1ull << lane) - 1ull));
+            for (uint32_t w = 0; w < kBatchBlock / 64u; w++) {
+                const uint32_t c = s_mine[w];
+                if (w < tid / 64u) slot += c;
+                n_mine += c;
+            }
+            if (mine) s_g[slot] = gi;
             __syncthreads();
             // bucket floors ascend: once the nearest remaining one lies behind every word of
             // the tile, no later instance can win a sample
             uint64_t tile_max = 0;
             for (uint32_t q = 0; q < kSubtiles; q++) tile_max = max(tile_max, s_sub[q]);
             if (keys[b0] > (uint32_t)(tile_max >> 32)) break;
-            const uint32_t nb = min(kBatchBlock, end - b0);
             uint32_t cnt = 0, rect = 0;
-            if (tid < nb) {
-                const uint32_t g = instances[b0 + tid];
+            if (tid < n_mine) {
+                const uint32_t g = s_g[tid];
                 Projected p;
                 if (g < sc.n && project_cached(sc.means, cov3d, g, vd.r1, vd.r2, vd.r3, vd.cam, p)) {
                     const Aabb b = pixel_aabb(p, vd.cam);
@@ -1039,9 +1179,10 @@ __global__ __launch_bounds__(kBatchBlock, kLayers <= 8u ? 5 : 3) void batch_tile
                     s_msx[tid] = p.mean_x - 0.5f;
                     s_msy[tid] = p.mean_y - 0.5f;
                     s_o[tid] = p.opacity;
-                    // log2(kMinAlpha / opacity) less a margin of 0.01 that dwarfs the error of
-                    // this estimate, so no fragment passing the exact alpha test is ever below it
-                    s_thr[tid] = (log_f(kMinAlpha) - log_f(p.opacity)) * kLog2e - 0.01f;
+                    // log2(kMinAlpha / opacity) = -log2(255 opacity) (power_threshold is
+                    // log(255 opacity)) less a margin of 0.01 that dwarfs the error of this
+                    // estimate, so no fragment passing the exact alpha test is ever below it
+                    s_thr[tid] = -p.power_threshold * kLog2e - 0.01f;
                     s_ca[tid] = ce.a;
                     s_cb[tid] = ce.b;
                     s_cc[tid] = ce.c;
