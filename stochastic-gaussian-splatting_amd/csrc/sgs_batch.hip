@@ -1045,7 +1045,9 @@ __global__ __launch_bounds__(kBatchBlock, kLayers <= 8u ? 5 : 3) void batch_tile
     __shared__ unsigned long long layer[kLayers][kTile * kTile];
     __shared__ uint64_t s_frag[kBatchBlock];
     __shared__ uint64_t s_sub[kSubtiles];
-    __shared__ uint32_t s_pmax[kTile * kTile];  // depth bits of each pixel's largest word
+    // depth bits of each pixel's largest word (the padding word is read, and ignored, for the
+    // missing second pixel of a pair at the tile's last pixel)
+    __shared__ uint32_t s_pmax[kTile * kTile + 1u];
     __shared__ uint16_t s_queue[kBatchBlock / 64u][192];  // per-wave (instance << 8 | pixel) queue
     __shared__ float s_msx[kBatchBlock], s_msy[kBatchBlock], s_o[kBatchBlock];
     __shared__ float s_thr[kBatchBlock];  // exponent below which a fragment's alpha is < 1/255
@@ -1098,24 +1100,23 @@ __global__ __launch_bounds__(kBatchBlock, kLayers <= 8u ? 5 : 3) void batch_tile
         const uint32_t fs_group = (vd.frame0 + s0) * 1973u;
         for (uint32_t b0 = begin; b0 < end; b0 += kBatchBlock) {
             const uint32_t nb = min(kBatchBlock, end - b0);
-            // The batch's instances whose sub-rectangle holds this tile (every instance when
-            // none is stored) are compacted to the front, so the projection below runs on as
-            // few waves as they fill: a bin's tiles share its list, and about half of a list's
-            // Gaussians miss any one of its tiles.
+            // With sub-rectangles stored, the batch's instances whose sub-rectangle holds this
+            // tile are compacted to the front, so the projection below runs on as few waves as
+            // they fill: a bin's tiles share its list, and about half of a list's Gaussians miss
+            // any one of its tiles. Without (bins of one tile), every instance is projected.
             uint32_t gi = 0;
             bool mine = false;
-            if (tid < nb) {
-                const uint32_t w = instances[b0 + tid];
-                gi = ms < 32u ? w & ((1u << ms) - 1u) : w;
-                mine = true;
-                if (ms < 32u) {
-                    const uint32_t sr = w >> ms;
+            uint64_t mine_mask = 0;
+            if (ms < 32u) {
+                if (tid < nb) {
+                    const uint32_t w = instances[b0 + tid], sr = w >> ms;
+                    gi = w & ((1u << ms) - 1u);
                     mine = qx >= (sr & bm) && qx <= ((sr >> bl) & bm) && qy >= ((sr >> (2u * bl)) & bm) &&
                            qy <= (sr >> (3u * bl));
                 }
+                mine_mask = __ballot(mine);
+                if (lane == 0u) s_mine[tid / 64u] = (uint32_t)__popcll(mine_mask);
             }
-            const uint64_t mine_mask = __ballot(mine);
-            if (lane == 0u) s_mine[tid / 64u] = (uint32_t)__popcll(mine_mask);
             __syncthreads();
             // pixel and sub-tile maxima of the words so far
             {
@@ -1128,13 +1129,18 @@ __global__ __launch_bounds__(kBatchBlock, kLayers <= 8u ? 5 : 3) void batch_tile
                 m = max(m, shfl_xor_u64(m, 32));
                 if ((lane & 51u) == 0u) s_sub[(tid / 64u) * (kTile / kSub) + (lane & 15u) / kSub] = m;
             }
-            uint32_t n_mine = 0, slot = (uint32_t)__popcll(mine_mask & ((1ull << lane) - 1ull));
-            for (uint32_t w = 0; w < kBatchBlock / 64u; w++) {
-                const uint32_t c = s_mine[w];
-                if (w < tid / 64u) slot += c;
-                n_mine += c;
+            uint32_t n_mine = nb;
+            if (ms < 32u) {
+                uint32_t slot = __builtin_amdgcn_mbcnt_hi((uint32_t)(mine_mask >> 32),
+                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)mine_mask, 0u));
+                n_mine = 0;
+                for (uint32_t w = 0; w < kBatchBlock / 64u; w++) {
+                    const uint32_t c = s_mine[w];
+                    if (w < tid / 64u) slot += c;
+                    n_mine += c;
+                }
+                if (mine) s_g[slot] = gi;
             }
-            if (mine) s_g[slot] = gi;
             __syncthreads();
             // bucket floors ascend: once the nearest remaining one lies behind every word of
             // the tile, no later instance can win a sample
@@ -1143,7 +1149,7 @@ __global__ __launch_bounds__(kBatchBlock, kLayers <= 8u ? 5 : 3) void batch_tile
             if (keys[b0] > (uint32_t)(tile_max >> 32)) break;
             uint32_t cnt = 0, rect = 0;
             if (tid < n_mine) {
-                const uint32_t g = s_g[tid];
+                const uint32_t g = ms < 32u ? s_g[tid] : instances[b0 + tid];
                 Projected p;
                 if (g < sc.n && project_cached(sc.means, cov3d, g, vd.r1, vd.r2, vd.r3, vd.cam, p)) {
                     const Aabb b = pixel_aabb(p, vd.cam);
@@ -1228,49 +1234,52 @@ __global__ __launch_bounds__(kBatchBlock, kLayers <= 8u ? 5 : 3) void batch_tile
             uint16_t* queue = s_queue[tid / 64u];
             uint32_t count = 0;  // wave-uniform
             for (uint32_t it = 0; it < per; it++, k++) {
-                bool cand = false, cand2 = false;
-                uint32_t entry = 0;
-                if (k < kend) {
-                    if (k == jend) {
-                        while (s_end[j] <= k) j++;
-                        jend = s_end[j];
-                        rect_j = s_rect[j];
-                        rw = ((rect_j >> 8) & 0xfu) + 1u;
-                        rp = (rw + 1u) >> 1;
-                        q = 0;
-                        r = 0;
-                        fdepth = (uint32_t)(s_frag[j] >> 32);
-                        msx = s_msx[j];
-                        msy = s_msy[j];
-                        thr = s_thr[j];
-                        ce = { s_ca[j], s_cb[j], s_cc[j] };
-                    }
-                    const uint32_t lx = (rect_j & 0xfu) + 2u * r, ly = ((rect_j >> 4) & 0xfu) + q;
-                    const uint32_t l = ly * kTile + lx;
-                    const bool two = 2u * r + 1u < rw;
-                    if (++r == rp) {
-                        r = 0;
-                        q++;
-                    }
-                    // some sample word of the pixel was behind the fragment at the batch start,
-                    // and the fragment lies inside the instance's alpha >= 1/255 ellipse (the
-                    // exponent is the one the alpha evaluation computes)
-                    const float dy = (float)(oy + ly) - msy;
-                    const float e = fragment_exponent(ce, (float)(ox + lx) - msx, dy);
-                    cand = fdepth <= s_pmax[l] && !(e < thr);
-                    if (two) {
-                        const float e2 = fragment_exponent(ce, (float)(ox + lx + 1u) - msx, dy);
-                        cand2 = fdepth <= s_pmax[l + 1u] && !(e2 < thr);
-                    }
-                    entry = (j << 8) | l;
+                // lanes past their range compute on stale (in-bounds) state and queue nothing
+                const bool valid = k < kend;
+                if (valid && k == jend) {
+                    while (s_end[j] <= k) j++;
+                    jend = s_end[j];
+                    rect_j = s_rect[j];
+                    rw = ((rect_j >> 8) & 0xfu) + 1u;
+                    rp = (rw + 1u) >> 1;
+                    q = 0;
+                    r = 0;
+                    fdepth = (uint32_t)(s_frag[j] >> 32);
+                    msx = s_msx[j];
+                    msy = s_msy[j];
+                    thr = s_thr[j];
+                    ce = { s_ca[j], s_cb[j], s_cc[j] };
                 }
+                // a lane whose range ended may sit one row below its rectangle: the row is taken
+                // modulo the tile so its reads stay inside s_pmax (valid lanes have ly < 16)
+                const uint32_t lx = (rect_j & 0xfu) + 2u * r, ly = (((rect_j >> 4) & 0xfu) + q) & (kTile - 1u);
+                const uint32_t l = ly * kTile + lx;
+                const bool two = 2u * r + 1u < rw;
+                if (valid && ++r == rp) {
+                    r = 0;
+                    q++;
+                }
+                // some sample word of the pixel was behind the fragment at the batch start,
+                // and the fragment lies inside the instance's alpha >= 1/255 ellipse (the
+                // exponent is the one the alpha evaluation computes; pixel coordinates below
+                // 2^24 are exact floats, so px + 1.0f is (float)(px + 1))
+                const float dy = (float)(oy + ly) - msy;
+                const float fx = (float)(ox + lx);
+                const float e = fragment_exponent(ce, fx - msx, dy);
+                const float e2 = fragment_exponent(ce, (fx + 1.0f) - msx, dy);
+                const bool cand = valid && fdepth <= s_pmax[l] && !(e < thr);
+                const bool cand2 = valid && two && fdepth <= s_pmax[l + 1u] && !(e2 < thr);
                 const uint64_t m = __ballot(cand), m2 = __ballot(cand2);
                 if (!(m | m2)) continue;
-                const uint64_t below = (1ull << lane) - 1ull;
-                const uint32_t n1 = (uint32_t)__popcll(m);
-                if (cand) queue[count + (uint32_t)__popcll(m & below)] = (uint16_t)entry;
-                if (cand2) queue[count + n1 + (uint32_t)__popcll(m2 & below)] = (uint16_t)(entry + 1u);
-                count += n1 + (uint32_t)__popcll(m2);
+                // queue slots: the candidates of lower lanes first (mbcnt counts the set bits of
+                // a mask below this lane)
+                const uint32_t entry = (j << 8) | l;
+                const uint32_t s1 = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                const uint32_t s2 = __builtin_amdgcn_mbcnt_hi((uint32_t)(m2 >> 32),
+                                                              __builtin_amdgcn_mbcnt_lo((uint32_t)m2, 0u));
+                if (cand) queue[count + s1] = (uint16_t)entry;
+                if (cand2) queue[count + (uint32_t)__popcll(m) + s2] = (uint16_t)(entry + 1u);
+                count += (uint32_t)__popcll(m) + (uint32_t)__popcll(m2);
                 // count < 64 before this step and at most 128 entries were added: at most two
                 // full queues to evaluate, then fewer than 64 entries move to the front
                 if (count >= 64u) {
