@@ -574,8 +574,15 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
                                                                 uint2* __restrict__ pair_rect,
                                                                 BatchCounters* __restrict__ ctr)
 {
+    // window of the small Gaussians (x0, x1, y0, y1): one buffer reset at each view start in
+    // the counting pass; in the fill pass two, a view's buffer being reset while the next one
+    // is in use, so no barrier precedes a view's window atomics (each layout measured the
+    // faster one for its pass)
     __shared__ uint32_t s_wx0, s_wx1, s_wy0, s_wy1;
+    __shared__ uint32_t s_win[2][4];
     __shared__ unsigned long long s_total, s_visible;
+    // Counters of the window's bins. The counting pass clears them at each view; in the fill pass
+    // they are zero between views, as each bin's slots are counted back down to zero.
     __shared__ uint32_t hist[kWindow * kHistCopies];
     __shared__ uint32_t hbase[kWindow * kHistCopies];
     if constexpr (kFill) {
@@ -588,6 +595,12 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
         s_total = 0ull;
         s_visible = 0ull;
     }
+    if constexpr (kFill) {
+        if (tid < 8u) s_win[tid >> 2][tid & 3u] = (tid & 1u) ? 0u : 0xffffffffu;
+        for (uint32_t i = tid; i < kWindow * kHistCopies; i += kBatchBlock) hist[i] = 0u;
+        __syncthreads();
+    }
+    uint32_t wb = 0;  // window buffer of the next binned view
     // Chunks are dealt round-robin (a shared claim counter would serialise one atomic per
     // chunk on one address; the per-chunk work is nearly uniform). A lane loads its
     // Gaussian's parameters once and bins it for every view in turn, so the parameter
@@ -621,13 +634,15 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
                 if (kFill && g < sc.n && v + 1u < n_views) next_rect = pair_rect[(size_t)(v + 1u) * sc.n + g];
                 continue;
             }
-            if (tid == 0) {
-                s_wx0 = 0xffffffffu;
-                s_wy0 = 0xffffffffu;
-                s_wx1 = 0u;
-                s_wy1 = 0u;
+            if constexpr (!kFill) {
+                if (tid == 0) {
+                    s_wx0 = 0xffffffffu;
+                    s_wy0 = 0xffffffffu;
+                    s_wx1 = 0u;
+                    s_wy1 = 0u;
+                }
+                __syncthreads();
             }
-            __syncthreads();
             const ViewDesc& vd = views[v];
             const uint32_t bin0 = vd.tile_begin, gw = vd.grid_w, bl = vd.bin_log;
             const uint32_t ms = subrect_shift(bl, sc.n);
@@ -663,10 +678,10 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
                 const uint32_t mx1 = wave_max_u32(small ? tr.tx1 : 0u);
                 const uint32_t my1 = wave_max_u32(small ? tr.ty1 : 0u);
                 if (lane == 0u && mx0 <= mx1) {
-                    atomicMin(&s_wx0, mx0);
-                    atomicMax(&s_wx1, mx1);
-                    atomicMin(&s_wy0, my0);
-                    atomicMax(&s_wy1, my1);
+                    atomicMin(kFill ? &s_win[wb][0] : &s_wx0, mx0);
+                    atomicMax(kFill ? &s_win[wb][1] : &s_wx1, mx1);
+                    atomicMin(kFill ? &s_win[wb][2] : &s_wy0, my0);
+                    atomicMax(kFill ? &s_win[wb][3] : &s_wy1, my1);
                 }
             }
             if constexpr (!kFill) {
@@ -691,13 +706,22 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
                 }
             }
             __syncthreads();
-            const uint32_t wx0 = s_wx0, wx1 = s_wx1, wy0 = s_wy0, wy1 = s_wy1;
+            const uint32_t wx0 = kFill ? s_win[wb][0] : s_wx0, wx1 = kFill ? s_win[wb][1] : s_wx1;
+            const uint32_t wy0 = kFill ? s_win[wb][2] : s_wy0, wy1 = kFill ? s_win[wb][3] : s_wy1;
+            // every wave has read the previous view's window (before this barrier): reset it for
+            // the next view (ordered before that view's atomics by the barriers below)
+            if constexpr (kFill) {
+                if (tid < 4u) s_win[wb ^ 1u][tid] = (tid & 1u) ? 0u : 0xffffffffu;
+                wb ^= 1u;
+            }
             if (wx0 <= wx1) {
                 const uint32_t wt = wx1 - wx0 + 1u, ht = wy1 - wy0 + 1u;
                 if (wt * ht <= kWindow) {
                     const uint32_t nw = wt * ht;
-                    for (uint32_t i = tid; i < nw * kHistCopies; i += kBatchBlock) hist[i] = 0u;
-                    __syncthreads();
+                    if constexpr (!kFill) {
+                        for (uint32_t i = tid; i < nw * kHistCopies; i += kBatchBlock) hist[i] = 0u;
+                        __syncthreads();
+                    }
                     if (small)
                         for (uint32_t ty = tr.ty0; ty <= tr.ty1; ty++)
                             for (uint32_t tx = tr.tx0; tx <= tr.tx1; tx++)
@@ -719,7 +743,6 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
                             for (uint32_t q = 0; q < kHistCopies; q++) {
                                 hbase[i * kHistCopies + q] = b;
                                 b += cc[q];
-                                hist[i * kHistCopies + q] = 0u;
                             }
                         } else {
                             atomicAdd(&tile_count[gt], c);
@@ -730,8 +753,9 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
                         if (small)
                             for (uint32_t ty = tr.ty0; ty <= tr.ty1; ty++)
                                 for (uint32_t tx = tr.tx0; tx <= tr.tx1; tx++) {
+                                    // the copy's count down to zero: slots hbase .. hbase + count - 1
                                     const uint32_t i = ((ty - wy0) * wt + tx - wx0) * kHistCopies + hc;
-                                    const uint32_t slot = hbase[i] + atomicAdd(&hist[i], 1u);
+                                    const uint32_t slot = hbase[i] + atomicSub(&hist[i], 1u) - 1u;
                                     entries.store(slot, key, instance_word(g, tx, ty, t16, bl, ms));
                                 }
                     }
