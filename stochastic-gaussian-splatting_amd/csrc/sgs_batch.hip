@@ -993,13 +993,6 @@ __device__ __forceinline__ uint32_t block_inclusive_scan(uint32_t x, uint32_t* w
     return x;
 }
 
-__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t x, int m)
-{
-    const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)x, m, 64);
-    const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(x >> 32), m, 64);
-    return ((uint64_t)hi << 32) | lo;
-}
-
 // The tile's pixels are 16 sub-tiles of 4x4; a wave holds rows 4w..4w+3, i.e. sub-tiles
 // (0..3, w). kSubtiles maxima bound the framebuffer words an instance can still lower.
 constexpr uint32_t kSub = 4, kSubtiles = (kTile / kSub) * (kTile / kSub);
@@ -1055,7 +1048,7 @@ __device__ __forceinline__ void tile_fragment(uint32_t entry, unsigned long long
 // kLayers: samples held in LDS per pass over the tile's instances (1 for views of one sample,
 // whose LDS then holds one framebuffer word per pixel)
 template <uint32_t kLayers>
-__global__ __launch_bounds__(kBatchBlock, kLayers <= 8u ? 5 : 3) void batch_tile_raster_kernel(SceneArgs sc,
+__global__ __launch_bounds__(kBatchBlock, kLayers == 1u ? 7 : kLayers <= 8u ? 5 : 3) void batch_tile_raster_kernel(SceneArgs sc,
                                                                         const float4* __restrict__ cov3d,
                                                                         const ViewDesc* __restrict__ views,
                                                                         uint32_t n_views,
@@ -1068,7 +1061,7 @@ __global__ __launch_bounds__(kBatchBlock, kLayers <= 8u ? 5 : 3) void batch_tile
 {
     __shared__ unsigned long long layer[kLayers][kTile * kTile];
     __shared__ uint64_t s_frag[kBatchBlock];
-    __shared__ uint64_t s_sub[kSubtiles];
+    __shared__ uint32_t s_sub[kSubtiles];  // depth bits of each 4x4 sub-tile's largest word
     // depth bits of each pixel's largest word (the padding word is read, and ignored, for the
     // missing second pixel of a pair at the tile's last pixel)
     __shared__ uint32_t s_pmax[kTile * kTile + 1u];
@@ -1142,15 +1135,16 @@ __global__ __launch_bounds__(kBatchBlock, kLayers <= 8u ? 5 : 3) void batch_tile
                 if (lane == 0u) s_mine[tid / 64u] = (uint32_t)__popcll(mine_mask);
             }
             __syncthreads();
-            // pixel and sub-tile maxima of the words so far
+            // pixel and sub-tile maxima of the words' depths so far (the depth half of a word
+            // orders it before its index half)
             {
-                uint64_t m = 0;
-                for (uint32_t s = 0; s < G; s++) m = max(m, (uint64_t)layer[s][tid]);
-                s_pmax[tid] = (uint32_t)(m >> 32);
-                m = max(m, shfl_xor_u64(m, 1));
-                m = max(m, shfl_xor_u64(m, 2));
-                m = max(m, shfl_xor_u64(m, 16));
-                m = max(m, shfl_xor_u64(m, 32));
+                uint32_t m = 0;
+                for (uint32_t s = 0; s < G; s++) m = max(m, (uint32_t)(layer[s][tid] >> 32));
+                s_pmax[tid] = m;
+                m = max(m, (uint32_t)__shfl_xor((int)m, 1, 64));
+                m = max(m, (uint32_t)__shfl_xor((int)m, 2, 64));
+                m = max(m, (uint32_t)__shfl_xor((int)m, 16, 64));
+                m = max(m, (uint32_t)__shfl_xor((int)m, 32, 64));
                 if ((lane & 51u) == 0u) s_sub[(tid / 64u) * (kTile / kSub) + (lane & 15u) / kSub] = m;
             }
             uint32_t n_mine = nb;
@@ -1168,9 +1162,9 @@ __global__ __launch_bounds__(kBatchBlock, kLayers <= 8u ? 5 : 3) void batch_tile
             __syncthreads();
             // bucket floors ascend: once the nearest remaining one lies behind every word of
             // the tile, no later instance can win a sample
-            uint64_t tile_max = 0;
+            uint32_t tile_max = 0;
             for (uint32_t q = 0; q < kSubtiles; q++) tile_max = max(tile_max, s_sub[q]);
-            if (keys[b0] > (uint32_t)(tile_max >> 32)) break;
+            if (keys[b0] > tile_max) break;
             uint32_t cnt = 0, rect = 0;
             if (tid < n_mine) {
                 const uint32_t g = ms < 32u ? s_g[tid] : instances[b0 + tid];
@@ -1181,12 +1175,14 @@ __global__ __launch_bounds__(kBatchBlock, kLayers <= 8u ? 5 : 3) void batch_tile
                     uint32_t cy0 = max(b.y0, oy), cy1 = min(b.y1, oy + kTile);
                     const uint64_t frag = fragment_key(p.depth, g);
                     if (cx1 > cx0 && cy1 > cy0) {
-                        // an instance can only win in sub-tiles holding a word behind it: the
-                        // walk covers the bounding box of those sub-tiles within its rectangle
+                        // an instance can only win in sub-tiles holding a word behind it (a word
+                        // at its depth or farther): the walk covers the bounding box of those
+                        // sub-tiles within its rectangle
+                        const uint32_t fd = (uint32_t)(frag >> 32);
                         uint32_t lx0 = kTile, lx1 = 0, ly0 = kTile, ly1 = 0;
                         for (uint32_t sy = (cy0 - oy) / kSub; sy <= (cy1 - 1u - oy) / kSub; sy++)
                             for (uint32_t sx = (cx0 - ox) / kSub; sx <= (cx1 - 1u - ox) / kSub; sx++)
-                                if (frag < s_sub[sy * (kTile / kSub) + sx]) {
+                                if (fd <= s_sub[sy * (kTile / kSub) + sx]) {
                                     lx0 = min(lx0, sx * kSub);
                                     lx1 = max(lx1, sx * kSub + kSub);
                                     ly0 = min(ly0, sy * kSub);
