@@ -103,6 +103,8 @@ def parse() -> argparse.Namespace:
                          "--pmc WRITE_SIZE; --pmc SQ counters) -> profiles/c4_batched_kernels.{json,md}")
     ap.add_argument("--summarize-trace", nargs=2, metavar=("DIR", "OUT"),
                     help="rocprofv3 kernel-trace directory -> markdown table per (kernel, grid)")
+    ap.add_argument("--trace-title", help="title of the --summarize-trace table")
+    ap.add_argument("--trace-command", help="the traced command, quoted above the --summarize-trace table")
     return ap.parse_args()
 
 
@@ -170,12 +172,19 @@ def counter_groups(pmc_dir: Path) -> dict:
     return groups
 
 
-def summarize_trace(trace_dir: str, out_md: str) -> None:
+def summarize_trace(trace_dir: str, out_md: str, title: str | None = None, command: str | None = None) -> None:
     """Markdown table of a kernel trace: every (kernel, grid) group with its dispatch count
-    and average / median duration, longest total first."""
+    and average / median duration, longest total first (under a title and the command that
+    made the trace, when given)."""
     groups = trace_groups(Path(trace_dir))
     rows = sorted(groups.items(), key=lambda kv: -sum(kv[1]))
-    lines = ["| kernel | grid (lanes) | dispatches | average ms | median ms | total ms |", "|---|---|---|---|---|---|"]
+    lines = []
+    if title:
+        lines += [f"# {title}", ""]
+    if command:
+        lines += ["Command (repository root):", "", f"    {command}", "",
+                  f"    python3 bench.py --summarize-trace <trace dir> {out_md}", ""]
+    lines += ["| kernel | grid (lanes) | dispatches | average ms | median ms | total ms |", "|---|---|---|---|---|---|"]
     for (name, grid), d in rows[:40]:
         lines.append(f"| `{name}` | {grid:,} | {len(d)} | {statistics.mean(d) / 1e6:.4f} | "
                      f"{statistics.median(d) / 1e6:.4f} | {sum(d) / 1e6:.2f} |")
@@ -1018,7 +1027,7 @@ def main() -> int:
         summarize_profile(*args.summarize_profile)
         return 0
     if args.summarize_trace:
-        summarize_trace(*args.summarize_trace)
+        summarize_trace(*args.summarize_trace, title=args.trace_title, command=args.trace_command)
         return 0
     if args.gpus != 1:
         print(json.dumps({"error": "this renderer runs on one GPU; --gpus must be 1 (views and requests would shard "
