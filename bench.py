@@ -481,7 +481,7 @@ def kernel_table(profile, views, n_instances, visible_pairs, scene_n):
         # every bucket and index read once, a 4 B sorted index written per instance
         "sgs::tile_sort_kernel": 10.0 * n_instances,
     }
-    raster_algo = 44.0 * visible_pairs + 8.0 * n_instances + 8.0 * words
+    raster_algo = 44.0 * visible_pairs + 4.0 * n_instances + 8.0 * words
     rows = []
     for name, e in profile["kernels"].items():
         per = e["dispatches"] / max(profile.get("renders") or 1, 1)
@@ -507,10 +507,11 @@ def roofline_raster(profile, views, raster_ms: list, n_instances: int, visible_p
     and its 8-layer launch for the others, timed together with HIP events). It is bound by
     VALU issue: wave64 VALU instructions per launch (SQ_INSTS_VALU of the profile summary)
     over the live launch time, against 256 CUs x one wave-instruction per cycle x 2.4 GHz.
-    Beside it the HBM view: algorithmic bytes (44 B per visible (view, Gaussian) pair, 8 B
-    per list entry, 8 B per framebuffer word written) and the measured bytes (profile)."""
+    Beside it the HBM view: algorithmic bytes (44 B per visible (view, Gaussian) pair, a 4 B
+    instance word per list entry, 8 B per framebuffer word written) and the measured bytes
+    (profile)."""
     words = samples_per_step(views)
-    algo = 44.0 * visible_pairs + 8.0 * n_instances + 8.0 * words
+    algo = 44.0 * visible_pairs + 4.0 * n_instances + 8.0 * words
     t = sum(raster_ms) / len(raster_ms) * 1e-3
     out = {"kernel": "sgs::batch_tile_raster_kernel<1> + <8> (one stage)", "avg_launch_ms": round(t * 1e3, 3)}
     ks = [profile["kernels"].get(k) for k in RASTER_KERNELS] if profile else [None]
