@@ -1388,7 +1388,7 @@ hipError_t launch_batch_tiles(const SceneArgs& sc, const ViewDesc* views, uint32
     }
     hipLaunchKernelGGL((exclusive_scan_kernel<CountInput>), dim3(1), dim3(kScanThreads), 0, stream,
                        CountInput{ tb.tile_count }, tb.offsets, nullptr, tb.n_tiles, tb.n_tiles, &counters->n_instances,
-                       &counters->overflow, tb.capacity, &counters->total_instances);
+                       &counters->overflow, tb.capacity, nullptr);
     if (sc.n > 0) {
         hipLaunchKernelGGL((batch_bin_kernel<true>), dim3((unsigned)grid), dim3(kBatchBlock), 0, stream, sc, cov3d,
                            views, n_views, tb.tile_count, tb.tile_cursor, tb.offsets,

@@ -9,9 +9,9 @@ constexpr uint32_t kScanThreads = 1024;
 
 // Exclusive scan of n values (n from the device when n_device is set, capped at capacity):
 // out[0..n] with out[n] the total; the total also goes to total_out, and overflow_out is set
-// to total > overflow_limit, judged on *total64 instead when given (a 64-bit total counted
-// elsewhere, so a 32-bit wrap of the scan cannot hide an overflow). One workgroup of 1024
-// threads, 4096 values per iteration.
+// to total > overflow_limit, judged on the 64-bit sum of the values (so a 32-bit wrap of the
+// scan cannot hide an overflow), which also goes to total64_out when given. One workgroup of
+// 1024 threads, 4096 values per iteration.
 struct CountInput {
     const uint32_t* values;
     __device__ uint32_t operator()(uint32_t i) const { return values[i]; }
@@ -24,13 +24,15 @@ __global__ __launch_bounds__(kScanThreads) void exclusive_scan_kernel(Input in, 
                                                                       uint32_t* __restrict__ total_out,
                                                                       uint32_t* __restrict__ overflow_out,
                                                                       uint32_t overflow_limit,
-                                                                      const unsigned long long* __restrict__ total64)
+                                                                      unsigned long long* __restrict__ total64_out)
 {
     __shared__ uint32_t wave_sums[kScanThreads / 64];
+    __shared__ unsigned long long wave_sums64[kScanThreads / 64];
     uint32_t n = n_device ? *n_device : n_host;
     if (n > capacity) n = capacity;
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     uint32_t carry = 0;
+    unsigned long long sum64 = 0;  // this thread's values, summed without wrap
     for (uint32_t base = 0; base < n; base += kScanThreads * 4u) {
         uint32_t v[4];
         uint32_t local = 0;
@@ -39,6 +41,7 @@ __global__ __launch_bounds__(kScanThreads) void exclusive_scan_kernel(Input in, 
             const uint32_t i = base + tid * 4u + (uint32_t)k;
             v[k] = i < n ? in(i) : 0u;
             local += v[k];
+            sum64 += v[k];
         }
         // inclusive wave scan of the per-thread totals
         uint32_t incl = local;
@@ -65,13 +68,20 @@ __global__ __launch_bounds__(kScanThreads) void exclusive_scan_kernel(Input in, 
         carry += block_total;
         __syncthreads();
     }
+    for (int off = 32; off > 0; off >>= 1) {
+        const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)sum64, off, 64);
+        const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(sum64 >> 32), off, 64);
+        sum64 += ((unsigned long long)hi << 32) | lo;
+    }
+    if (lane == 0u) wave_sums64[wave] = sum64;
+    __syncthreads();
     if (tid == 0) {
+        unsigned long long t = 0;
+        for (uint32_t w = 0; w < kScanThreads / 64; w++) t += wave_sums64[w];
         out[n] = carry;
         if (total_out) *total_out = carry;
-        if (overflow_out) {
-            const unsigned long long t = total64 ? *total64 : (unsigned long long)carry;
-            *overflow_out = t > (unsigned long long)overflow_limit ? 1u : 0u;
-        }
+        if (total64_out) *total64_out = t;
+        if (overflow_out) *overflow_out = t > (unsigned long long)overflow_limit ? 1u : 0u;
     }
 }
 
