@@ -324,3 +324,32 @@ def test_batch_rejects_incomplete_feature_views(gpu):
     with pytest.raises(ValueError, match="blue_noise"):
         render_batch(*args, [BatchView(settings_of(small_camera(16, 16)),
                                        blue_noise=torch.zeros(100, dtype=torch.uint8, device="cuda"))])
+
+
+def test_batch_bins_without_subrectangles(gpu):
+    """More Gaussians than a 64-pixel bin's instance word leaves index bits for (2^24): the
+    fill stores no tile sub-rectangles and every tile of a bin takes every instance; the
+    result still equals the scatter schedule and the plain SPP loop (16-pixel bins, no
+    sub-rectangles either way)."""
+    import torch
+    from StochasticGSCudaBackend.batch import BatchRenderer, BatchTuning, BatchView
+    n = (1 << 24) + 1
+    arrays = small_scene(n, seed=48, log_scale_mean=-6.5, log_scale_std=0.5, sh_degree=0)
+    t = to_device(arrays)
+    del arrays
+    cams, spps, frames = [small_camera(80, 60, angle=0.7), small_camera(40, 30, angle=2.2)], [3, 1], [11, 90]
+    views = [BatchView(settings_of(c), spp=k, frame_idx=f, bin_size=64) for c, k, f in zip(cams, spps, frames)]
+    print_case(test="batch_no_subrect", n=n)
+    tiles = BatchRenderer(t["means"], t["scales"], t["rotations"], t["opacities"], t["sh"], views)
+    scatter = BatchRenderer(t["means"], t["scales"], t["rotations"], t["opacities"], t["sh"], views,
+                            BatchTuning(schedule=2))
+    tiles.render()
+    scatter.render()
+    torch.cuda.synchronize()
+    n_inst, pairs, overflow = tiles.tile_stats()
+    assert not overflow and n_inst > 0
+    for v, (c, k, f) in enumerate(zip(cams, spps, frames)):
+        assert_close(tiles.outputs[v], scatter.outputs[v])
+        assert_close(tiles.outputs[v], plain_spp(gpu, t, c, f, k))
+    del tiles, scatter, t
+    torch.cuda.empty_cache()
