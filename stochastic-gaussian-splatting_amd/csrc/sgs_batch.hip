@@ -456,8 +456,8 @@ __device__ __forceinline__ bool gaussian_tiles_bound(float mx, float my, float m
 // Depth buckets of the per-bin sort (tile_sort_kernel): the fill pass stores each instance's
 // bucket, the sort counts and places by it.
 constexpr uint32_t kSortBins = 1024;
-constexpr uint32_t kSortInFlight = 16;  // entries per lane in flight in the two-pass path of larger tiles
 constexpr uint32_t kSortRegKeys = 32;  // entries per lane kept in registers for tiles of <= 8192
+constexpr uint32_t kSortInFlight = 16;  // entries per lane in flight in the two-pass path of larger tiles
 
 __device__ __forceinline__ uint32_t sort_bucket(uint32_t key, uint2 vk)
 {
@@ -859,6 +859,60 @@ __global__ __launch_bounds__(kRangeThreads) void view_key_range_kernel(const Vie
     }
 }
 
+// ---- order of the sort's workgroups ----
+// Bins by descending list length class (bit length of the length, 2^15 entries and above in
+// one class; order within a class is irrelevant), so the longest lists start first instead of
+// extending the sort's tail. Written over the bin counts, dead once the scan has turned them
+// into offsets. Two launches over 1024-bin blocks: class counts, then placement; a wave
+// counts and places each distinct class of its bins once.
+constexpr uint32_t kOrderThreads = 1024, kOrderClasses = 17;
+
+__device__ __forceinline__ uint32_t order_class(const uint32_t* offsets, uint32_t b)
+{
+    const uint32_t len = offsets[b + 1u] - offsets[b];
+    return min(len ? 32u - (uint32_t)__clz(len) : 0u, kOrderClasses - 1u);
+}
+
+template <bool kPlace>
+__global__ __launch_bounds__(kOrderThreads) void bin_order_kernel(const uint32_t* __restrict__ offsets,
+                                                                   uint32_t n_bins, uint32_t* __restrict__ order,
+                                                                   BatchCounters* __restrict__ ctr)
+{
+    __shared__ uint32_t s_count[kOrderClasses], s_base[kOrderClasses];
+    if (ctr->overflow) return;
+    const uint32_t tid = threadIdx.x, lane = lane_id();
+    const uint32_t b = blockIdx.x * kOrderThreads + tid;
+    const uint32_t c = b < n_bins ? order_class(offsets, b) : kOrderClasses;
+    if (tid < kOrderClasses) s_count[tid] = 0u;
+    __syncthreads();
+    // this block's bins per class
+    uint32_t rank = 0;  // among this block's bins of the same class
+    uint64_t todo = __ballot(b < n_bins);
+    while (todo) {
+        const int leader = __builtin_ctzll(todo);
+        const uint32_t lc = (uint32_t)__shfl((int)c, leader, 64);
+        const uint64_t same = __ballot(c == lc) & todo;
+        todo &= ~same;
+        uint32_t first = 0;
+        if ((int)lane == leader) first = atomicAdd(&s_count[lc], (uint32_t)__popcll(same));
+        first = (uint32_t)__shfl((int)first, leader, 64);
+        if (c == lc) rank = first + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+    }
+    __syncthreads();
+    if constexpr (!kPlace) {
+        if (tid < kOrderClasses && s_count[tid]) atomicAdd(&ctr->order_count[tid], s_count[tid]);
+    } else {
+        // the block's slots: after the longer classes and the blocks that reserved before it
+        if (tid < kOrderClasses) {
+            uint32_t base = 0;
+            for (uint32_t k = tid + 1u; k < kOrderClasses; k++) base += ctr->order_count[k];
+            s_base[tid] = base + (s_count[tid] ? atomicAdd(&ctr->order_cursor[tid], s_count[tid]) : 0u);
+        }
+        __syncthreads();
+        if (b < n_bins) order[s_base[c] + rank] = b;
+    }
+}
+
 // ---- per-tile counting sort by depth ----
 // One workgroup per tile: the tile's instances are written bucket by bucket, nearest first.
 // Bucket 0 takes keys below the view's kmin; bucket b >= 1 takes (key - kmin) >> shift = b - 1
@@ -867,7 +921,8 @@ __global__ __launch_bounds__(kRangeThreads) void view_key_range_kernel(const Vie
 // that position on, read where the rasteriser's batches of 256 start; other positions of
 // keys_out are not written.
 
-__global__ __launch_bounds__(kBatchBlock) void tile_sort_kernel(uint32_t n_views, const uint32_t* __restrict__ tile_prefix,
+__global__ __launch_bounds__(kBatchBlock) void tile_sort_kernel(uint32_t n_views, const uint32_t* __restrict__ order,
+                                                                const uint32_t* __restrict__ tile_prefix,
                                                                 const uint2* __restrict__ view_keys,
                                                                 const uint32_t* __restrict__ offsets,
                                                                 const EntryArrays entries_in,
@@ -878,7 +933,7 @@ __global__ __launch_bounds__(kBatchBlock) void tile_sort_kernel(uint32_t n_views
     __shared__ uint32_t hist[kSortBins];
     __shared__ uint32_t s_wave[kBatchBlock / 64u];
     if (ctr->overflow) return;
-    const uint32_t gt = blockIdx.x;
+    const uint32_t gt = order[blockIdx.x];
     const uint32_t begin = offsets[gt], end = offsets[gt + 1u];
     if (end <= begin) return;
     uint32_t lo = 0, hi = n_views - 1u;
@@ -1394,8 +1449,13 @@ hipError_t launch_batch_tiles(const SceneArgs& sc, const ViewDesc* views, uint32
                            views, n_views, tb.tile_count, tb.tile_cursor, tb.offsets,
                            tb.entries, tb.view_keys, tb.pair_rect, counters);
     }
-    hipLaunchKernelGGL(tile_sort_kernel, dim3(tb.n_tiles), dim3(kBatchBlock), 0, stream, n_views, tile_prefix,
-                       tb.view_keys, tb.offsets, tb.entries, tb.sorted_keys, tb.sorted_instances, counters);
+    const dim3 order_grid((tb.n_tiles + kOrderThreads - 1u) / kOrderThreads);
+    hipLaunchKernelGGL(bin_order_kernel<false>, order_grid, dim3(kOrderThreads), 0, stream, tb.offsets, tb.n_tiles,
+                       tb.tile_count, counters);
+    hipLaunchKernelGGL(bin_order_kernel<true>, order_grid, dim3(kOrderThreads), 0, stream, tb.offsets, tb.n_tiles,
+                       tb.tile_count, counters);
+    hipLaunchKernelGGL(tile_sort_kernel, dim3(tb.n_tiles), dim3(kBatchBlock), 0, stream, n_views, tb.tile_count,
+                       tile_prefix, tb.view_keys, tb.offsets, tb.entries, tb.sorted_keys, tb.sorted_instances, counters);
     if (raster_event) {
         e = hipEventRecord(raster_event, stream);
         if (e != hipSuccess) return e;

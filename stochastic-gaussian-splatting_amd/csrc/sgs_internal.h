@@ -152,7 +152,9 @@ struct BatchCounters {
     uint32_t overflow;          // instances exceed the capacity: the scatter schedule runs instead
     unsigned long long total_instances;  // 64-bit instance total of the counting pass
     unsigned long long visible_pairs;    // (view, Gaussian) pairs with at least one tile
-    uint32_t pad[54];
+    uint32_t order_count[17];   // bins per list-length class (bit length, 16 and above merged)
+    uint32_t order_cursor[17];  // placement cursors of the sort's bin order, per class
+    uint32_t pad[20];
 };
 static_assert(sizeof(BatchCounters) == 256, "BatchCounters layout");
 
@@ -224,7 +226,7 @@ struct EntryArrays {
 };
 
 struct TileBuffers {
-    uint32_t* tile_count;       // [n_tiles] instances per tile (zeroed by the caller)
+    uint32_t* tile_count;       // [n_tiles] instances per tile (zeroed by the caller); after the scan, the sort's bin order
     uint32_t* tile_cursor;      // [n_tiles] fill cursors (zeroed by the caller)
     uint32_t* offsets;          // [n_tiles + 1] exclusive scan of tile_count
     EntryArrays entries;        // [capacity] depth bucket and Gaussian index of each (view, bin, Gaussian) instance
