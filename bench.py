@@ -50,7 +50,8 @@ COPY_CALIBRATION_BYTES = 1 << 30
 SOURCE_DIRS = [REPO / "stochastic-gaussian-splatting_amd" / "csrc", REPO / "include"]
 
 # kernels of one batched render with the tile schedule, in launch order
-RASTER_KERNELS = ["sgs::batch_tile_raster_kernel<1u>", "sgs::batch_tile_raster_kernel<8u>"]
+RASTER_KERNELS = ["sgs::batch_tile_raster_kernel<1u, 16u>", "sgs::batch_tile_raster_kernel<8u, 16u>",
+                  "sgs::batch_tile_raster_kernel<16u, 8u>"]
 BATCH_KERNELS = ["sgs::chunk_centroid_kernel", "sgs::view_key_range_kernel", "sgs::cov3d_kernel",
                  "sgs::batch_bin_kernel<false>", "sgs::exclusive_scan_kernel<sgs::CountInput>",
                  "sgs::batch_bin_kernel<true>", "sgs::tile_sort_kernel", *RASTER_KERNELS,
@@ -86,8 +87,9 @@ def parse() -> argparse.Namespace:
                     help="run only one path; the JSON line then reports that path")
     ap.add_argument("--schedule", choices=["tiles", "scatter"], default="tiles",
                     help="batched accumulation: LDS tiles (default) or persistent scatter")
-    ap.add_argument("--tile-samples", type=int, choices=[8, 16], default=8,
-                    help="samples per LDS pass of the tile rasteriser (with --no-autotune)")
+    ap.add_argument("--tile-samples", type=int, choices=[0, 8, 16], default=0,
+                    help="samples per LDS pass of the tile rasteriser (with --no-autotune; 0: 8 for views of up "
+                         "to 8 samples, 16 per half tile above)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="use --schedule/--tile-samples instead of tuning them on the workload first")
     ap.add_argument("--view-major", action="store_true",
@@ -503,8 +505,9 @@ def kernel_table(profile, views, n_instances, visible_pairs, scene_n):
 
 
 def roofline_raster(profile, views, raster_ms: list, n_instances: int, visible_pairs: int, copy_gbs: float):
-    """The dominant kernel: the tile rasteriser (its one-layer launch for views of one sample
-    and its 8-layer launch for the others, timed together with HIP events). It is bound by
+    """The dominant kernel: the tile rasteriser (its one-layer launch for views of one sample,
+    its 8-layer launch over whole tiles for views of up to 8 samples and its 16-layer launch
+    over half tiles for views of more, timed together with HIP events). It is bound by
     VALU issue: wave64 VALU instructions per launch (SQ_INSTS_VALU of the profile summary)
     over the live launch time, against 256 CUs x one wave-instruction per cycle x 2.4 GHz.
     Beside it the HBM view: algorithmic bytes (44 B per visible (view, Gaussian) pair, a 4 B
@@ -513,7 +516,8 @@ def roofline_raster(profile, views, raster_ms: list, n_instances: int, visible_p
     words = samples_per_step(views)
     algo = 44.0 * visible_pairs + 4.0 * n_instances + 8.0 * words
     t = sum(raster_ms) / len(raster_ms) * 1e-3
-    out = {"kernel": "sgs::batch_tile_raster_kernel<1> + <8> (one stage)", "avg_launch_ms": round(t * 1e3, 3)}
+    out = {"kernel": "sgs::batch_tile_raster_kernel<1, 16> + <8, 16> + <16, 8> (one stage)",
+           "avg_launch_ms": round(t * 1e3, 3)}
     ks = [profile["kernels"].get(k) for k in RASTER_KERNELS] if profile else [None]
     if all(k is not None and "SQ_INSTS_VALU" in k["counters"] for k in ks):
         valu = sum(k["counters"]["SQ_INSTS_VALU"] for k in ks)
@@ -896,7 +900,7 @@ def bench_batched(args, scene, views, S, plain_out0, dev):
         bviews = res.views
         streams, part_tunings = res.streams, res.part_tunings
         args.schedule = "tiles" if tuning.schedule in (0, SCHEDULE_TILES) else "scatter"
-        args.tile_samples = tuning.tile_samples or 8
+        args.tile_samples = tuning.tile_samples
         tuned = {"best": dict(tuning._asdict()), "streams": streams,
                  "part_max_tile_instances": [int(t.max_tile_instances) for t in part_tunings] if part_tunings else None,
                  "view_bin_sizes": sorted({v.bin_size for v in bviews}),

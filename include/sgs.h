@@ -254,8 +254,11 @@ typedef struct sgs_batch_tuning {
                                          and view 4 + 8 P / 2^21 for a framebuffer of P pixels,
                                          within [2^16, 2^31]); a batch that needs more runs the
                                          scatter schedule */
-    uint32_t tile_samples;            /* samples held in LDS per pass over a tile (8 or 16;
-                                         default 8) */
+    uint32_t tile_samples;            /* samples held in LDS per pass over the instances of a
+                                         view of several samples: 8 (passes over whole 16x16
+                                         tiles) or 16 (passes over half tiles, two workgroups
+                                         per tile); 0 (default): 8 for views of up to 8
+                                         samples, 16 above */
 } sgs_batch_tuning;
 
 /* SGS_BATCH_TILES: every (view, Gaussian) is binned into the framebuffer bins (the view's

@@ -547,7 +547,7 @@ BatchTuning resolve_batch_tuning(const sgs_batch_tuning* t)
     r.blocks_per_cu = t ? t->blocks_per_cu : 0u;
     r.schedule = (t && t->schedule == SGS_BATCH_SCATTER) ? 1u : 0u;
     r.max_tile_instances = t ? t->max_tile_instances : 0u;
-    r.tile_samples = (t && t->tile_samples == 16u) ? 16u : 8u;
+    r.tile_samples = t ? t->tile_samples : 0u;
     return r;
 }
 
@@ -873,8 +873,7 @@ int accumulate_tiles_single(const sgs_gaussians* g, const sgs_settings* s, const
     tb.n_tiles = L.n_tiles;
     tb.capacity = L.capacity;
     tb.n_raster = prefixes.w[5];
-    tb.has_single = true;
-    tb.has_multi = false;
+    tb.spp_min = tb.spp_max = 1u;  // one view of one sample
     const SceneArgs sc = scene_args(g);
     const uint32_t* dev_prefix = reinterpret_cast<const uint32_t*>(tile_ws + L.table + L.views_bytes);
     e = launch_batch_tiles(sc, desc, 1, dev_prefix + 2, dev_prefix + 4, tb, 8u, counters, fb, nullptr, stream);
@@ -1026,8 +1025,12 @@ int sgs_render_batch(const sgs_gaussians* gaussians, const sgs_view* views, uint
         tb.n_tiles = (uint32_t)L.n_tiles;
         tb.capacity = L.capacity;
         tb.n_raster = (uint32_t)L.n_raster;
-        tb.has_single = tb.has_multi = false;
-        for (uint32_t v = 0; v < n_views; v++) (views[v].spp == 1u ? tb.has_single : tb.has_multi) = true;
+        tb.spp_min = 0xffffffffu;
+        tb.spp_max = 0u;
+        for (uint32_t v = 0; v < n_views; v++) {
+            tb.spp_min = std::min(tb.spp_min, views[v].spp);
+            tb.spp_max = std::max(tb.spp_max, views[v].spp);
+        }
         e = launch_batch_tiles(sc, desc, n_views, tile_prefix, raster_prefix, tb, t.tile_samples, counters, fb,
                                stage_events ? static_cast<hipEvent_t>(stage_events[2]) : nullptr, stream);
         if (e != hipSuccess) return hip_fail(e, "batched tiles");

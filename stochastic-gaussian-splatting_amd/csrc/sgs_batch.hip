@@ -1048,15 +1048,15 @@ __device__ __forceinline__ uint32_t block_inclusive_scan(uint32_t x, uint32_t* w
     return x;
 }
 
-// The tile's pixels are 16 sub-tiles of 4x4; a wave holds rows 4w..4w+3, i.e. sub-tiles
-// (0..3, w). kSubtiles maxima bound the framebuffer words an instance can still lower.
-constexpr uint32_t kSub = 4, kSubtiles = (kTile / kSub) * (kTile / kSub);
-// One queued fragment (instance j << 8 | tile pixel l): alpha once, then every sample word of
-// the pixel the fragment still lies in front of gets its hash test and, on acceptance, the
-// LDS atomicMin. All kLayers words are read (rows at and beyond G are masked off), in groups of
-// up to four.
-template <uint32_t kLayers>
-__device__ __forceinline__ void tile_fragment(uint32_t entry, unsigned long long (*layer)[kTile * kTile],
+// A workgroup's pixels (16 x kRows: a 16x16 tile, or half of one) are sub-tiles of 4x4; a wave
+// holds rows 4w..4w+3, i.e. sub-tiles (0..3, w). Their maxima bound the framebuffer words an
+// instance can still lower.
+constexpr uint32_t kSub = 4;
+// One queued fragment (instance j << 8 | pixel l): alpha once, then every sample word of the
+// pixel the fragment still lies in front of gets its hash test and, on acceptance, the LDS
+// atomicMin. The words of the G samples of the pass are read in groups of up to four.
+template <uint32_t kLayers, uint32_t kPix>
+__device__ __forceinline__ void tile_fragment(uint32_t entry, unsigned long long (*layer)[kPix],
                                               const uint64_t* s_frag, const float* s_msx, const float* s_msy,
                                               const float* s_o, const float* s_ca, const float* s_cb,
                                               const float* s_cc, uint32_t ox, uint32_t oy, uint32_t G,
@@ -1066,44 +1066,49 @@ __device__ __forceinline__ void tile_fragment(uint32_t entry, unsigned long long
     const uint32_t px = ox + (l & 0xfu), py = oy + (l >> 4);
     const ConicExp2 ce = { s_ca[j], s_cb[j], s_cc[j] };
     const float alpha = fragment_alpha(ce, s_o[j], (float)px - s_msx[j], (float)py - s_msy[j]);
-    if (alpha < kMinAlpha) return;
+    // a NaN alpha accepts no sample (alpha_threshold is 0), so it returns here as well
+    if (!(alpha >= kMinAlpha)) return;
     const uint64_t fj = s_frag[j];
     const uint32_t fd = (uint32_t)(fj >> 32);
-    uint32_t live = 0;
-    // only the depth halves of the words are read: a sample is live if its word's depth is not
+    // (h >> 9) < T  <=>  h <= 512 T - 1 for 1 <= T <= 2^23 (alpha <= 1; T = 2^23 wraps to all
+    // ones, which accepts every h as (h >> 9) < 2^23 does)
+    const uint32_t lim = (min(alpha_threshold(alpha), 1u << 23) << 9) - 1u;
+    // px, py < 2^16, so the 24-bit multiplies give the full products of rand_uniform's seed
+    const uint32_t base = (uint32_t)fj ^ __umul24(px, 9277u) ^ __umul24(py, 2699u);
+    // Only the depth halves of the words are read: a sample is tested if its word's depth is not
     // in front of the fragment (equal depths are resolved by the atomicMin), and words only
-    // decrease, so a stale read never drops a winner
+    // decrease, so a stale read never drops a winner. The comparison is the branch condition
+    // itself (a lane mask), and a sample no lane tests is skipped as a whole; each sample's
+    // frame term is a wave-uniform constant.
     constexpr uint32_t kGroup = kLayers < 4u ? kLayers : 4u;
 #pragma unroll
     for (uint32_t s0 = 0; s0 < kLayers; s0 += kGroup) {
+        if (s0 >= G) break;
         uint32_t w[kGroup];
 #pragma unroll
         for (uint32_t i = 0; i < kGroup; i++)
             w[i] = __hip_atomic_load(reinterpret_cast<uint32_t*>(&layer[s0 + i][l]) + 1, __ATOMIC_RELAXED,
                                      __HIP_MEMORY_SCOPE_WORKGROUP);
 #pragma unroll
-        for (uint32_t i = 0; i < kGroup; i++) live |= fd <= w[i] ? 1u << (s0 + i) : 0u;
-    }
-    live &= (1u << G) - 1u;
-    if (!live) return;
-    const uint32_t T = alpha_threshold(alpha);
-    // px, py < 2^16, so the 24-bit multiplies give the full products of rand_uniform's seed
-    const uint32_t base = (uint32_t)fj ^ __umul24(px, 9277u) ^ __umul24(py, 2699u);
-    // samples unrolled: each sample's frame term is a wave-uniform constant, and a sample no
-    // lane of the wave holds live is skipped as a whole
-#pragma unroll
-    for (uint32_t s = 0; s < kLayers; s++) {
-        if (live & (1u << s)) {
-            const uint32_t h = pcg_hash(base ^ (fs_group + s * 1973u));
-            if ((h >> 9u) < T) atomicMin(&layer[s][l], (unsigned long long)fj);
+        for (uint32_t i = 0; i < kGroup; i++) {
+            const uint32_t s = s0 + i;
+            if (s >= G) break;
+            if (fd <= w[i]) {
+                const uint32_t h = pcg_hash(base ^ (fs_group + s * 1973u));
+                if (h <= lim) atomicMin(&layer[s][l], (unsigned long long)fj);
+            }
         }
     }
 }
 
-// kLayers: samples held in LDS per pass over the tile's instances (1 for views of one sample,
-// whose LDS then holds one framebuffer word per pixel)
-template <uint32_t kLayers>
-__global__ __launch_bounds__(kBatchBlock, kLayers == 1u ? 7 : kLayers <= 8u ? 5 : 3) void batch_tile_raster_kernel(SceneArgs sc,
+// kLayers: samples held in LDS per pass over the instances (1 for views of one sample, whose
+// LDS then holds one framebuffer word per pixel). kRows: rows of the workgroup's pixels, 16 for
+// a whole 16x16 tile, 8 for half of one (two workgroups per tile, the same LDS for twice the
+// samples: a view of 16 samples then walks each instance's pixels and evaluates each fragment's
+// alpha once instead of once per pass of 8). Only views with lo <= spp <= hi are rasterised
+// here; the others return at once.
+template <uint32_t kLayers, uint32_t kRows>
+__global__ __launch_bounds__(kBatchBlock, kLayers == 1u ? 7 : 5) void batch_tile_raster_kernel(SceneArgs sc,
                                                                         const float4* __restrict__ cov3d,
                                                                         const ViewDesc* __restrict__ views,
                                                                         uint32_t n_views,
@@ -1112,14 +1117,19 @@ __global__ __launch_bounds__(kBatchBlock, kLayers == 1u ? 7 : kLayers <= 8u ? 5 
                                                                         const uint32_t* __restrict__ instances,
                                                                         const uint32_t* __restrict__ keys,
                                                                         const BatchCounters* __restrict__ ctr,
-                                                                        uint64_t* __restrict__ fb)
+                                                                        uint64_t* __restrict__ fb, uint32_t spp_lo,
+                                                                        uint32_t spp_hi)
 {
-    __shared__ unsigned long long layer[kLayers][kTile * kTile];
+    static_assert(kRows == kTile || kRows * 2u == kTile, "a workgroup covers a tile or half of one");
+    static_assert(kLayers * kTile * kRows <= 2048u, "16 KB of framebuffer words per workgroup");
+    constexpr uint32_t kPix = kTile * kRows;
+    constexpr uint32_t kSubtiles = (kTile / kSub) * (kRows / kSub);
+    __shared__ unsigned long long layer[kLayers][kPix];
     __shared__ uint64_t s_frag[kBatchBlock];
     __shared__ uint32_t s_sub[kSubtiles];  // depth bits of each 4x4 sub-tile's largest word
     // depth bits of each pixel's largest word (the padding word is read, and ignored, for the
-    // missing second pixel of a pair at the tile's last pixel)
-    __shared__ uint32_t s_pmax[kTile * kTile + 1u];
+    // missing second pixel of a pair at the last pixel)
+    __shared__ uint32_t s_pmax[kPix + 1u];
     __shared__ uint16_t s_queue[kBatchBlock / 64u][192];  // per-wave (instance << 8 | pixel) queue
     __shared__ float s_msx[kBatchBlock], s_msy[kBatchBlock], s_o[kBatchBlock];
     __shared__ float s_thr[kBatchBlock];  // exponent below which a fragment's alpha is < 1/255
@@ -1130,12 +1140,15 @@ __global__ __launch_bounds__(kBatchBlock, kLayers == 1u ? 7 : kLayers <= 8u ? 5 
     __shared__ uint32_t s_mine[kBatchBlock / 64u];   // their count per wave
     if (ctr->overflow) return;
     // Workgroups are handed to the eight XCDs round-robin (workgroup i to XCD i % 8), each XCD
-    // with its own L2. Within every 32 workgroups, the four that share an XCD take four
-    // consecutive tiles (a 32-pixel bin's four tiles, as views start at multiples of 4), so a
-    // bin's list and its Gaussians are fetched into one L2: blk is the tile in view order.
-    const uint32_t j = blockIdx.x & 31u;
-    const uint32_t blk = (blockIdx.x & ~31u) | ((j & 7u) << 2) | (j >> 3);
-    if (blk >= raster_prefix[n_views]) return;  // the grid is a multiple of 32
+    // with its own L2. Within every 8 x kPer workgroups, the kPer that share an XCD take kPer
+    // consecutive slots (a 32-pixel bin's four tiles, or their eight halves, as views start at
+    // multiples of 4 tiles), so a bin's list and its Gaussians are fetched into one L2: wg is the
+    // workgroup in view order, blk its tile and half the half of the tile.
+    constexpr uint32_t kPer = kRows == kTile ? 4u : 8u, kPerLog = kRows == kTile ? 2u : 3u;
+    const uint32_t j = blockIdx.x & (8u * kPer - 1u);
+    const uint32_t wg = (blockIdx.x & ~(8u * kPer - 1u)) | ((j & 7u) << kPerLog) | (j >> 3);
+    const uint32_t blk = kRows == kTile ? wg : wg >> 1, half = kRows == kTile ? 0u : wg & 1u;
+    if (blk >= raster_prefix[n_views]) return;  // the grid is a multiple of 8 x kPer
     // tile -> view (last v with raster_prefix[v] <= blk) -> (bin lt of the view, 16x16 tile q
     // of the bin's 2^b x 2^b)
     uint32_t lo = 0, hi = n_views - 1u;
@@ -1145,21 +1158,20 @@ __global__ __launch_bounds__(kBatchBlock, kLayers == 1u ? 7 : kLayers <= 8u ? 5 
         else hi = mid - 1u;
     }
     const ViewDesc& vd = views[lo];
-    // one-sample views are rasterised by the one-layer kernel, the others by the layered one
-    if ((kLayers == 1u) != (vd.spp == 1u)) return;
+    if (vd.spp < spp_lo || vd.spp > spp_hi) return;  // a view of another launch
     const uint32_t bl = vd.bin_log, r = blk - raster_prefix[lo];
     const uint32_t lt = r >> (2u * bl), q = r & ((1u << (2u * bl)) - 1u);
     const uint32_t gt = vd.tile_begin + lt;
     const uint32_t gw = vd.grid_w;
     const uint32_t ox = ((lt % gw) << (kRasterLog2 + bl)) + (q & ((1u << bl) - 1u)) * kTile;
-    const uint32_t oy = ((lt / gw) << (kRasterLog2 + bl)) + (q >> bl) * kTile;
+    const uint32_t oy = ((lt / gw) << (kRasterLog2 + bl)) + (q >> bl) * kTile + half * kRows;
     const uint32_t W = vd.cam.width_i, H = vd.cam.height_i;
     if (ox >= W || oy >= H) return;  // a tile of an edge bin beyond the framebuffer, or padding
     const size_t P = (size_t)W * H;
     const uint32_t begin = offsets[gt], end = offsets[gt + 1u];
     const uint32_t tid = threadIdx.x, lane = lane_id();
-    const uint32_t ptx = ox + tid % kTile, pty = oy + tid / kTile;
-    const bool inside = ptx < W && pty < H;
+    // whether pixel p (row-major, 16 per row) of the workgroup lies inside the view
+    const auto inside = [&](uint32_t p) { return ox + p % kTile < W && oy + p / kTile < H; };
     uint64_t* fbv = fb + vd.fb_offset;
     // the tile's position in its bin, for the instances' stored sub-rectangles
     const uint32_t ms = subrect_shift(bl, sc.n), bm = (1u << bl) - 1u;
@@ -1168,7 +1180,7 @@ __global__ __launch_bounds__(kBatchBlock, kLayers == 1u ? 7 : kLayers <= 8u ? 5 
     for (uint32_t s0 = 0; s0 < vd.spp; s0 += kLayers) {
         const uint32_t G = min(kLayers, vd.spp - s0);
         // pixels outside the view hold 0 so they never keep an instance alive
-        for (uint32_t s = 0; s < G; s++) layer[s][tid] = inside ? kEmpty : 0ull;
+        for (uint32_t i = tid; i < G * kPix; i += kBatchBlock) layer[i / kPix][i % kPix] = inside(i % kPix) ? kEmpty : 0ull;
         const uint32_t fs_group = (vd.frame0 + s0) * 1973u;
         for (uint32_t b0 = begin; b0 < end; b0 += kBatchBlock) {
             const uint32_t nb = min(kBatchBlock, end - b0);
@@ -1192,7 +1204,7 @@ __global__ __launch_bounds__(kBatchBlock, kLayers == 1u ? 7 : kLayers <= 8u ? 5 
             __syncthreads();
             // pixel and sub-tile maxima of the words' depths so far (the depth half of a word
             // orders it before its index half)
-            {
+            if (tid < kPix) {  // whole waves
                 uint32_t m = 0;
                 for (uint32_t s = 0; s < G; s++) m = max(m, (uint32_t)(layer[s][tid] >> 32));
                 s_pmax[tid] = m;
@@ -1227,7 +1239,7 @@ __global__ __launch_bounds__(kBatchBlock, kLayers == 1u ? 7 : kLayers <= 8u ? 5 
                 if (g < sc.n && project_cached(sc.means, cov3d, g, vd.r1, vd.r2, vd.r3, vd.cam, p)) {
                     const Aabb b = pixel_aabb(p, vd.cam);
                     uint32_t cx0 = max(b.x0, ox), cx1 = min(b.x1, ox + kTile);
-                    uint32_t cy0 = max(b.y0, oy), cy1 = min(b.y1, oy + kTile);
+                    uint32_t cy0 = max(b.y0, oy), cy1 = min(b.y1, oy + kRows);
                     const uint64_t frag = fragment_key(p.depth, g);
                     if (cx1 > cx0 && cy1 > cy0) {
                         // an instance can only win in sub-tiles holding a word behind it (a word
@@ -1326,8 +1338,8 @@ __global__ __launch_bounds__(kBatchBlock, kLayers == 1u ? 7 : kLayers <= 8u ? 5 
                     ce = { s_ca[j], s_cb[j], s_cc[j] };
                 }
                 // a lane whose range ended may sit one row below its rectangle: the row is taken
-                // modulo the tile so its reads stay inside s_pmax (valid lanes have ly < 16)
-                const uint32_t lx = (rect_j & 0xfu) + 2u * r, ly = (((rect_j >> 4) & 0xfu) + q) & (kTile - 1u);
+                // modulo kRows so its reads stay inside s_pmax (valid lanes have ly < kRows)
+                const uint32_t lx = (rect_j & 0xfu) + 2u * r, ly = (((rect_j >> 4) & 0xfu) + q) & (kRows - 1u);
                 const uint32_t l = ly * kTile + lx;
                 const bool two = 2u * r + 1u < rw;
                 if (valid && ++r == rp) {
@@ -1359,11 +1371,11 @@ __global__ __launch_bounds__(kBatchBlock, kLayers == 1u ? 7 : kLayers <= 8u ? 5 
                 // full queues to evaluate, then fewer than 64 entries move to the front
                 if (count >= 64u) {
                     __builtin_amdgcn_wave_barrier();
-                    tile_fragment<kLayers>(queue[lane], layer, s_frag, s_msx, s_msy, s_o, s_ca, s_cb, s_cc, ox, oy,
+                    tile_fragment<kLayers, kPix>(queue[lane], layer, s_frag, s_msx, s_msy, s_o, s_ca, s_cb, s_cc, ox, oy,
                                            G, fs_group);
                     uint32_t head = 64u;
                     if (count >= 128u) {
-                        tile_fragment<kLayers>(queue[64u + lane], layer, s_frag, s_msx, s_msy, s_o, s_ca, s_cb, s_cc,
+                        tile_fragment<kLayers, kPix>(queue[64u + lane], layer, s_frag, s_msx, s_msy, s_o, s_ca, s_cb, s_cc,
                                                ox, oy, G, fs_group);
                         head = 128u;
                     }
@@ -1377,12 +1389,15 @@ __global__ __launch_bounds__(kBatchBlock, kLayers == 1u ? 7 : kLayers <= 8u ? 5 
             }
             __builtin_amdgcn_wave_barrier();
             if (lane < count)
-                tile_fragment<kLayers>(queue[lane], layer, s_frag, s_msx, s_msy, s_o, s_ca, s_cb, s_cc, ox, oy, G,
+                tile_fragment<kLayers, kPix>(queue[lane], layer, s_frag, s_msx, s_msy, s_o, s_ca, s_cb, s_cc, ox, oy, G,
                                        fs_group);
         }
         __syncthreads();
-        if (inside)
-            for (uint32_t s = 0; s < G; s++) fbv[(size_t)(s0 + s) * P + (size_t)pty * W + ptx] = layer[s][tid];
+        for (uint32_t i = tid; i < G * kPix; i += kBatchBlock) {
+            const uint32_t p = i % kPix;
+            if (inside(p))
+                fbv[(size_t)(s0 + i / kPix) * P + (size_t)(oy + p / kTile) * W + ox + p % kTile] = layer[i / kPix][p];
+        }
         __syncthreads();
     }
 }
@@ -1461,24 +1476,30 @@ hipError_t launch_batch_tiles(const SceneArgs& sc, const ViewDesc* views, uint32
         if (e != hipSuccess) return e;
     }
     // one workgroup per 16x16 tile, 4^b per bin of (16 << b) pixels (the caller keeps the total
-    // below 2^31; it equals raster_prefix[n_views]), rounded up to a multiple of 32
+    // below 2^31; it equals raster_prefix[n_views]), rounded up to a multiple of 32; two per tile
+    // for the half-tile kernel, rounded up to a multiple of 64
     const dim3 raster_grid((tb.n_raster + 31u) & ~31u);
-    // Views of one sample take the one-layer kernel (less LDS per workgroup), the others the
-    // kernel with `layers` samples per pass. Both launches span every view's tiles and return at
-    // once on the other's (measured faster than two launches over class-ordered tiles, whose
-    // sequential tails cost more than the early returns).
-    if (tb.has_single)
-        hipLaunchKernelGGL(batch_tile_raster_kernel<1>, raster_grid, dim3(kBatchBlock), 0, stream, sc, cov3d,
+    const dim3 half_grid((2u * tb.n_raster + 63u) & ~63u);
+    // Views of one sample take the one-layer kernel (least LDS per workgroup); the others take
+    // 8 samples per pass over whole tiles or 16 per pass over half tiles: by default 8 up to 8
+    // samples and 16 above, `layers` 8 or 16 for every view of several samples. Every launch
+    // spans all views' tiles and returns at once on the others' (measured faster than launches
+    // over class-ordered tiles, whose sequential tails cost more than the early returns).
+    const uint32_t kAll = 0xffffffffu;
+    const uint32_t mid_hi = layers == 8u ? kAll : layers == 16u ? 1u : 8u;
+    const uint32_t many_lo = layers == 16u ? 2u : layers == 8u ? kAll : 9u;
+    if (tb.spp_max >= 1u && tb.spp_min <= 1u)
+        hipLaunchKernelGGL((batch_tile_raster_kernel<1, kTile>), raster_grid, dim3(kBatchBlock), 0, stream, sc, cov3d,
                            views, n_views, raster_prefix, tb.offsets, tb.sorted_instances, tb.sorted_keys, counters,
-                           fb);
-    if (tb.has_multi && layers == 8u)
-        hipLaunchKernelGGL(batch_tile_raster_kernel<8>, raster_grid, dim3(kBatchBlock), 0, stream, sc, cov3d,
+                           fb, 1u, 1u);
+    if (tb.spp_max >= 2u && tb.spp_min <= mid_hi)
+        hipLaunchKernelGGL((batch_tile_raster_kernel<8, kTile>), raster_grid, dim3(kBatchBlock), 0, stream, sc, cov3d,
                            views, n_views, raster_prefix, tb.offsets, tb.sorted_instances, tb.sorted_keys, counters,
-                           fb);
-    else if (tb.has_multi)
-        hipLaunchKernelGGL(batch_tile_raster_kernel<16>, raster_grid, dim3(kBatchBlock), 0, stream, sc, cov3d,
-                           views, n_views, raster_prefix, tb.offsets, tb.sorted_instances, tb.sorted_keys, counters,
-                           fb);
+                           fb, 2u, mid_hi);
+    if (tb.spp_max >= many_lo)
+        hipLaunchKernelGGL((batch_tile_raster_kernel<16, kTile / 2u>), half_grid, dim3(kBatchBlock), 0, stream, sc,
+                           cov3d, views, n_views, raster_prefix, tb.offsets, tb.sorted_instances, tb.sorted_keys,
+                           counters, fb, many_lo, kAll);
     return hipGetLastError();
 }
 

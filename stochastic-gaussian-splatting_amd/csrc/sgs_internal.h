@@ -171,7 +171,7 @@ static_assert(sizeof(BigEntry) == 64, "BigEntry layout");
 struct BatchTuning {
     uint32_t schedule;          // 0: LDS tiles, 1: scatter with global atomics
     uint32_t max_tile_instances;
-    uint32_t tile_samples;      // samples per LDS pass of the tile rasteriser (8 or 16)
+    uint32_t tile_samples;      // samples per LDS pass of the tile rasteriser (0: 8 or 16 by view, 8, 16)
     uint32_t sequential_threshold;
     uint32_t big_threshold;     // fragments above which a Gaussian goes to the queue
     uint32_t entry_fragments;   // fragments per queue entry
@@ -238,7 +238,7 @@ struct TileBuffers {
     uint2* view_keys;           // [n_views] depth-key bucketing of each view (kmin, shift)
     uint32_t n_tiles, capacity;  // bins of the batch; instance capacity
     uint32_t n_raster;           // rasteriser workgroups: 16x16 tiles of every view's bins
-    bool has_single, has_multi;  // views of one sample / of several samples in the batch
+    uint32_t spp_min, spp_max;   // fewest and most samples of the batch's views
 };
 // tile_prefix / raster_prefix: first bin / rasteriser workgroup of each view, n_views + 1 each
 hipError_t launch_batch_tiles(const SceneArgs& sc, const ViewDesc* views, uint32_t n_views, const uint32_t* tile_prefix,

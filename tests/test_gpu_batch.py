@@ -75,6 +75,8 @@ def make_views():
     dict(),                                                   # defaults (tile schedule)
     dict(max_tile_instances=16),                              # tile list overflow: scatter fallback
     dict(tile_samples=8),                                     # 8 samples per LDS pass
+    dict(spps=[16, 9, 8, 1, 12]),                             # 8 per pass up to 8, 16 per half tile above
+    dict(spps=[16, 9, 8, 1, 12], bins=[32, 16, 64, 32, 16]),
     dict(bins=[16, 16, 16, 16, 16]),                          # one raster tile per bin
     dict(bins=[32, 64, 16, 64, 32], tile_samples=16),         # bin sizes mixed in one launch
     dict(bins=[64, 64, 64, 64, 64]),                          # 4x4 raster tiles per bin
@@ -91,8 +93,9 @@ def test_batch_equals_plain_spp_loop(gpu, tuning):
     cams, spps, frames = make_views()
     tuning = dict(tuning)
     bins = tuning.pop("bins", [0] * len(cams))
+    spps = tuning.pop("spps", spps)
     views = [BatchView(settings_of(c), spp=n, frame_idx=f, bin_size=b) for c, n, f, b in zip(cams, spps, frames, bins)]
-    print_case(test="batch_vs_plain", bins=bins, **tuning)
+    print_case(test="batch_vs_plain", bins=bins, spps=spps, **tuning)
     r = BatchRenderer(t["means"], t["scales"], t["rotations"], t["opacities"], t["sh"], views, BatchTuning(**tuning))
     outs = r.render()
     torch.cuda.synchronize()
