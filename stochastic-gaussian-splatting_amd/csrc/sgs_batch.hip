@@ -1127,16 +1127,18 @@ __global__ __launch_bounds__(kBatchBlock, kLayers == 1u ? 7 : 5) void batch_tile
     __shared__ unsigned long long layer[kLayers][kPix];
     __shared__ uint64_t s_frag[kBatchBlock];
     __shared__ uint32_t s_sub[kSubtiles];  // depth bits of each 4x4 sub-tile's largest word
-    // depth bits of each pixel's largest word (the padding word is read, and ignored, for the
-    // missing second pixel of a pair at the last pixel)
-    __shared__ uint32_t s_pmax[kPix + 1u];
-    __shared__ uint16_t s_queue[kBatchBlock / 64u][192];  // per-wave (instance << 8 | pixel) queue
+    // depth bits of each pixel's largest word (the 17 padding words are read, and ignored, for
+    // the missing pixels of a 2x2 quad at the last row and column)
+    __shared__ uint32_t s_pmax[kPix + 17u];
+    __shared__ uint16_t s_queue[kBatchBlock / 64u][320];  // per-wave (instance << 8 | pixel) queue
     __shared__ float s_msx[kBatchBlock], s_msy[kBatchBlock], s_o[kBatchBlock];
     __shared__ float s_thr[kBatchBlock];  // exponent below which a fragment's alpha is < 1/255
     __shared__ float s_ca[kBatchBlock], s_cb[kBatchBlock], s_cc[kBatchBlock];
     __shared__ uint32_t s_rect[kBatchBlock], s_end[kBatchBlock];
     __shared__ uint32_t s_wave[kBatchBlock / 64u];
-    __shared__ uint32_t s_g[kBatchBlock];            // the batch's instances that reach this tile
+    // the batch's instances that reach this tile, in s_end: the projection reads them before the
+    // scan's barrier, and s_end is written after it
+    uint32_t* const s_g = s_end;
     __shared__ uint32_t s_mine[kBatchBlock / 64u];   // their count per wave
     if (ctr->overflow) return;
     // Workgroups are handed to the eight XCDs round-robin (workgroup i to XCD i % 8), each XCD
@@ -1260,11 +1262,13 @@ __global__ __launch_bounds__(kBatchBlock, kLayers == 1u ? 7 : 5) void batch_tile
                             cx1 = min(cx1, ox + lx1);
                             cy0 = max(cy0, oy + ly0);
                             cy1 = min(cy1, oy + ly1);
-                            // the walk visits horizontal pixel pairs: rp per row of rw pixels
-                            const uint32_t rw = cx1 - cx0, rh = cy1 - cy0, rp = (rw + 1u) >> 1;
-                            rect = (cx0 - ox) | ((cy0 - oy) << 4) | ((rw - 1u) << 8) |
-                                   (((65536u + rp - 1u) / rp) << 12);
-                            cnt = rp * rh;
+                            // the walk visits 2x2 pixel quads: qw per row of quads, qh rows
+                            // (rw, rh <= 16); ceil(1024 / qw) turns the walk's f / qw (f < 64)
+                            // into a multiply and shift
+                            const uint32_t rw = cx1 - cx0, rh = cy1 - cy0, qw = (rw + 1u) >> 1;
+                            rect = (cx0 - ox) | ((cy0 - oy) << 4) | ((rw - 1u) << 8) | ((rh - 1u) << 12) |
+                                   (((1024u + qw - 1u) / qw) << 16);
+                            cnt = qw * ((rh + 1u) >> 1);
                         }
                     }
                     const ConicExp2 ce = conic_exp2(p.conic_x, p.conic_y, p.conic_z);
@@ -1282,20 +1286,20 @@ __global__ __launch_bounds__(kBatchBlock, kLayers == 1u ? 7 : 5) void batch_tile
                 }
             }
             s_rect[tid] = rect;
-            // pixel pairs of the batch's live instances, flattened over the workgroup's lanes
+            // pixel quads of the batch's live instances, flattened over the workgroup's lanes
             const uint32_t incl = block_inclusive_scan(cnt, s_wave);
             s_end[tid] = incl;
             __syncthreads();
             const uint32_t total = s_end[kBatchBlock - 1u];
-            // Each lane walks a contiguous range of the batch's pixel pairs (an instance's
-            // rectangle advances without division; the second pixel of a pair lies beyond an
-            // odd-width row's end and is skipped). Fragments that some sample word of their
-            // pixel still lies behind go to the wave's LDS queue; alpha and the samples are
-            // evaluated on full queues of 64, so no lane idles on a rejected fragment.
+            // Each lane walks a contiguous range of the batch's 2x2 pixel quads (an instance's
+            // rectangle advances without division; the quad's pixels beyond an odd width or
+            // height are skipped). Fragments that some sample word of their pixel still lies
+            // behind go to the wave's LDS queue; alpha and the samples are evaluated on full
+            // queues of 64, so no lane idles on a rejected fragment.
             const uint32_t per = (total + kBatchBlock - 1u) / kBatchBlock;
             uint32_t k = tid * per;
             const uint32_t kend = min(k + per, total);
-            uint32_t j = 0, jend = 0, rect_j = 0, rw = 1, rp = 1, q = 0, r = 0, fdepth = 0;
+            uint32_t j = 0, jend = 0, rect_j = 0, qw = 1, q = 0, r = 0, fdepth = 0;
             float msx = 0.0f, msy = 0.0f, thr = 0.0f;
             ConicExp2 ce = { 0.0f, 0.0f, 0.0f };
             if (k < kend) {
@@ -1308,10 +1312,9 @@ __global__ __launch_bounds__(kBatchBlock, kLayers == 1u ? 7 : 5) void batch_tile
                 jend = s_end[j];
                 const uint32_t f = k - (j ? s_end[j - 1u] : 0u);
                 rect_j = s_rect[j];
-                rw = ((rect_j >> 8) & 0xfu) + 1u;
-                rp = (rw + 1u) >> 1;
-                q = (f * (rect_j >> 12)) >> 16;  // f / rp for f < 128, rp <= 8
-                r = f - q * rp;
+                qw = (((rect_j >> 8) & 0xfu) + 2u) >> 1;
+                q = (f * (rect_j >> 16)) >> 10;  // f / qw for f < 64, qw <= 8
+                r = f - q * qw;
                 fdepth = (uint32_t)(s_frag[j] >> 32);
                 msx = s_msx[j];
                 msy = s_msy[j];
@@ -1327,8 +1330,7 @@ __global__ __launch_bounds__(kBatchBlock, kLayers == 1u ? 7 : 5) void batch_tile
                     while (s_end[j] <= k) j++;
                     jend = s_end[j];
                     rect_j = s_rect[j];
-                    rw = ((rect_j >> 8) & 0xfu) + 1u;
-                    rp = (rw + 1u) >> 1;
+                    qw = (((rect_j >> 8) & 0xfu) + 2u) >> 1;
                     q = 0;
                     r = 0;
                     fdepth = (uint32_t)(s_frag[j] >> 32);
@@ -1337,48 +1339,63 @@ __global__ __launch_bounds__(kBatchBlock, kLayers == 1u ? 7 : 5) void batch_tile
                     thr = s_thr[j];
                     ce = { s_ca[j], s_cb[j], s_cc[j] };
                 }
-                // a lane whose range ended may sit one row below its rectangle: the row is taken
-                // modulo kRows so its reads stay inside s_pmax (valid lanes have ly < kRows)
-                const uint32_t lx = (rect_j & 0xfu) + 2u * r, ly = (((rect_j >> 4) & 0xfu) + q) & (kRows - 1u);
+                // a lane whose range ended may sit one quad row below its rectangle: the row is
+                // taken modulo kRows so its reads stay inside s_pmax (valid lanes have ly < kRows)
+                const uint32_t lx = (rect_j & 0xfu) + 2u * r, ly = (((rect_j >> 4) & 0xfu) + 2u * q) & (kRows - 1u);
                 const uint32_t l = ly * kTile + lx;
-                const bool two = 2u * r + 1u < rw;
-                if (valid && ++r == rp) {
+                const bool two_x = 2u * r + 1u < ((rect_j >> 8) & 0xfu) + 1u;
+                const bool two_y = 2u * q + 1u < ((rect_j >> 12) & 0xfu) + 1u;
+                if (valid && ++r == qw) {
                     r = 0;
                     q++;
                 }
                 // some sample word of the pixel was behind the fragment at the batch start,
                 // and the fragment lies inside the instance's alpha >= 1/255 ellipse (the
-                // exponent is the one the alpha evaluation computes; pixel coordinates below
-                // 2^24 are exact floats, so px + 1.0f is (float)(px + 1))
-                const float dy = (float)(oy + ly) - msy;
-                const float fx = (float)(ox + lx);
-                const float e = fragment_exponent(ce, fx - msx, dy);
-                const float e2 = fragment_exponent(ce, (fx + 1.0f) - msx, dy);
-                const bool cand = valid && fdepth <= s_pmax[l] && !(e < thr);
-                const bool cand2 = valid && two && fdepth <= s_pmax[l + 1u] && !(e2 < thr);
-                const uint64_t m = __ballot(cand), m2 = __ballot(cand2);
-                if (!(m | m2)) continue;
+                // exponents are the ones the alpha evaluation computes; pixel coordinates below
+                // 2^24 are exact floats, so p + 1.0f is (float)(p + 1))
+                const float fx = (float)(ox + lx), fy = (float)(oy + ly);
+                const float dx0 = fx - msx, dx1 = (fx + 1.0f) - msx;
+                const float dy0 = fy - msy, dy1 = (fy + 1.0f) - msy;
+                // Candidate masks: every test is one comparison whose lane mask (ballot) is taken
+                // on its own and the masks are combined with scalar ANDs (a ballot of a combined
+                // condition would be materialised per lane and compared again). Every lane of the
+                // wave is active here.
+                const uint64_t mv = __builtin_amdgcn_ballot_w64(valid);
+                const uint64_t mx = __builtin_amdgcn_ballot_w64(two_x), my = __builtin_amdgcn_ballot_w64(two_y);
+                const uint64_t m00 = mv & __builtin_amdgcn_ballot_w64(fdepth <= s_pmax[l]) &
+                                     __builtin_amdgcn_ballot_w64(!(fragment_exponent(ce, dx0, dy0) < thr));
+                const uint64_t m01 = mv & mx & __builtin_amdgcn_ballot_w64(fdepth <= s_pmax[l + 1u]) &
+                                     __builtin_amdgcn_ballot_w64(!(fragment_exponent(ce, dx1, dy0) < thr));
+                const uint64_t m10 = mv & my & __builtin_amdgcn_ballot_w64(fdepth <= s_pmax[l + kTile]) &
+                                     __builtin_amdgcn_ballot_w64(!(fragment_exponent(ce, dx0, dy1) < thr));
+                const uint64_t m11 = mv & mx & my & __builtin_amdgcn_ballot_w64(fdepth <= s_pmax[l + kTile + 1u]) &
+                                     __builtin_amdgcn_ballot_w64(!(fragment_exponent(ce, dx1, dy1) < thr));
+                if (!(m00 | m01 | m10 | m11)) continue;
                 // queue slots: the candidates of lower lanes first (mbcnt counts the set bits of
-                // a mask below this lane)
+                // a mask below this lane), the quad's four pixels one after another
+                const auto below = [](uint64_t m) {
+                    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                };
                 const uint32_t entry = (j << 8) | l;
-                const uint32_t s1 = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-                const uint32_t s2 = __builtin_amdgcn_mbcnt_hi((uint32_t)(m2 >> 32),
-                                                              __builtin_amdgcn_mbcnt_lo((uint32_t)m2, 0u));
-                if (cand) queue[count + s1] = (uint16_t)entry;
-                if (cand2) queue[count + (uint32_t)__popcll(m) + s2] = (uint16_t)(entry + 1u);
-                count += (uint32_t)__popcll(m) + (uint32_t)__popcll(m2);
-                // count < 64 before this step and at most 128 entries were added: at most two
-                // full queues to evaluate, then fewer than 64 entries move to the front
+                const uint32_t n00 = (uint32_t)__popcll(m00), n01 = (uint32_t)__popcll(m01);
+                const uint32_t n10 = (uint32_t)__popcll(m10), n11 = (uint32_t)__popcll(m11);
+                if (__builtin_amdgcn_inverse_ballot_w64(m00)) queue[count + below(m00)] = (uint16_t)entry;
+                if (__builtin_amdgcn_inverse_ballot_w64(m01)) queue[count + n00 + below(m01)] = (uint16_t)(entry + 1u);
+                if (__builtin_amdgcn_inverse_ballot_w64(m10))
+                    queue[count + n00 + n01 + below(m10)] = (uint16_t)(entry + kTile);
+                if (__builtin_amdgcn_inverse_ballot_w64(m11))
+                    queue[count + n00 + n01 + n10 + below(m11)] = (uint16_t)(entry + kTile + 1u);
+                count += n00 + n01 + n10 + n11;
+                // count < 64 before this step and at most 256 entries were added: full queues
+                // of 64 are evaluated, then fewer than 64 entries move to the front
                 if (count >= 64u) {
                     __builtin_amdgcn_wave_barrier();
-                    tile_fragment<kLayers, kPix>(queue[lane], layer, s_frag, s_msx, s_msy, s_o, s_ca, s_cb, s_cc, ox, oy,
-                                           G, fs_group);
-                    uint32_t head = 64u;
-                    if (count >= 128u) {
-                        tile_fragment<kLayers, kPix>(queue[64u + lane], layer, s_frag, s_msx, s_msy, s_o, s_ca, s_cb, s_cc,
-                                               ox, oy, G, fs_group);
-                        head = 128u;
-                    }
+                    uint32_t head = 0;
+                    do {
+                        tile_fragment<kLayers, kPix>(queue[head + lane], layer, s_frag, s_msx, s_msy, s_o, s_ca, s_cb,
+                                                     s_cc, ox, oy, G, fs_group);
+                        head += 64u;
+                    } while (count - head >= 64u);
                     count -= head;
                     __builtin_amdgcn_wave_barrier();
                     const uint32_t rest = lane < count ? queue[head + lane] : 0u;
