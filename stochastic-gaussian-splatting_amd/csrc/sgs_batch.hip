@@ -345,8 +345,11 @@ constexpr uint32_t kTile = kRasterPixels;
 constexpr uint32_t kWindow = 512;    // tiles of a workgroup's LDS histogram window
 // Each window cell has kHistCopies adjacent counters (distinct LDS banks); lane l counts into
 // copy l % kHistCopies, so the many lanes of a spatially compact chunk that hit one bin
-// spread over that many addresses instead of serialising on one.
-constexpr uint32_t kHistCopies = 2;
+// spread over that many addresses instead of serialising on one. The counting pass (a plain
+// add per instance) measured fastest with four copies, the fill pass (an add, then a
+// subtract with return, per instance) with two.
+template <bool kFill>
+constexpr uint32_t kHistCopiesOf = kFill ? 2u : 4u;
 
 struct TileRange {
     uint32_t tx0, tx1, ty0, ty1;  // inclusive
@@ -562,6 +565,15 @@ hipError_t launch_cov3d(const SceneArgs& sc, float4* cache, hipStream_t stream)
 // tiles is marked by all ones
 constexpr uint32_t kNoTiles = 0xffffffffu;
 
+// Gaussians per lane in the binning passes (counting, fill): a workgroup bins chunks of
+// kBinPer x 256 Gaussians per view step, so the per-step work shared by the workgroup (window
+// reductions, barriers, the histogram's flush with one global atomic per touched bin) is paid
+// once per kBinPer x 256 pairs.
+template <bool kFill>
+constexpr uint32_t kBinPer = kFill ? 1u : 2u;
+template <bool kFill>
+constexpr uint32_t kBinChunk = kBinPer<kFill> * kBatchBlock;
+
 template <bool kFill>
 __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, const float4* __restrict__ cov3d,
                                                                 const ViewDesc* __restrict__ views,
@@ -574,6 +586,7 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
                                                                 uint2* __restrict__ pair_rect,
                                                                 BatchCounters* __restrict__ ctr)
 {
+    constexpr uint32_t kHistCopies = kHistCopiesOf<kFill>;
     // window of the small Gaussians (x0, x1, y0, y1): one buffer reset at each view start in
     // the counting pass; in the fill pass two, a view's buffer being reset while the next one
     // is in use, so no barrier precedes a view's window atomics (each layout measured the
@@ -585,12 +598,13 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
     // they are zero between views, as each bin's slots are counted back down to zero.
     __shared__ uint32_t hist[kWindow * kHistCopies];
     __shared__ uint32_t hbase[kWindow * kHistCopies];
+    constexpr uint32_t kPer = kBinPer<kFill>;
     if constexpr (kFill) {
         if (ctr->overflow) return;
     }
     const uint32_t tid = threadIdx.x, lane = lane_id();
     const uint32_t hc = lane & (kHistCopies - 1u);
-    const uint32_t n_chunks = (sc.n + kBatchBlock - 1u) / kBatchBlock;
+    const uint32_t n_chunks = (sc.n + kBinChunk<kFill> - 1u) / kBinChunk<kFill>;
     if (tid == 0) {
         s_total = 0ull;
         s_visible = 0ull;
@@ -603,35 +617,43 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
     uint32_t wb = 0;  // window buffer of the next binned view
     // Chunks are dealt round-robin (a shared claim counter would serialise one atomic per
     // chunk on one address; the per-chunk work is nearly uniform). A lane loads its
-    // Gaussian's parameters once and bins it for every view in turn, so the parameter
+    // Gaussians' parameters once and bins them for every view in turn, so the parameter
     // stream is read from memory once per pass instead of once per view. The counting pass
     // records each (view, Gaussian) pair's rectangle of 16x16 tiles (a superset of the exact
     // one, from gaussian_tiles_bound); the fill pass reads it back instead of projecting again,
     // with the depth key recomputed from the mean. Both bin the pair into the bins holding
-    // those tiles.
+    // those tiles. Lane l holds Gaussians chunk * kPer * 256 + k * 256 + l (coalesced loads).
     for (uint32_t chunk = blockIdx.x; chunk < n_chunks; chunk += gridDim.x) {
-        const uint32_t g = chunk * kBatchBlock + tid;
-        float mx = 0.0f, my = 0.0f, mz = 0.0f;
-        Cov3d cv = { 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, -1.0f, 0.0f };
-        if (g < sc.n) {
-            mx = sc.means[3u * g + 0u];
-            my = sc.means[3u * g + 1u];
-            mz = sc.means[3u * g + 2u];
+        uint32_t g[kPer];
+        float mx[kPer], my[kPer], mz[kPer];
+        Cov3d cv[kPer];
+        uint2 next_rect[kPer];
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; k++) {
+            g[k] = chunk * kBinChunk<kFill> + k * kBatchBlock + tid;
+            mx[k] = my[k] = mz[k] = 0.0f;
+            cv[k] = { 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, -1.0f, 0.0f };
+            if (g[k] < sc.n) {
+                mx[k] = sc.means[3u * g[k] + 0u];
+                my[k] = sc.means[3u * g[k] + 1u];
+                mz[k] = sc.means[3u * g[k] + 2u];
+            }
+            if (!kFill && g[k] < sc.n) {
+                const float4 a = cov3d[2u * g[k]], b = cov3d[2u * g[k] + 1u];
+                cv[k] = { a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w };
+            }
+            // the fill pass reads each view's pair record one view ahead, so the load of view
+            // v + 1 is in flight while view v is binned
+            next_rect[k] = make_uint2(kNoTiles, kNoTiles);
+            if (kFill && g[k] < sc.n) next_rect[k] = pair_rect[g[k]];
         }
-        if (!kFill && g < sc.n) {
-            const float4 a = cov3d[2u * g], b = cov3d[2u * g + 1u];
-            cv = { a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w };
-        }
-        const bool valid = g < sc.n && !(cv.opacity < 0.0f);
-        const float trace = cv.c11 + cv.c22 + cv.c33;
-        // the fill pass reads each view's pair record one view ahead, so the load of view v + 1
-        // is in flight while view v is binned
-        uint2 next_rect = make_uint2(kNoTiles, kNoTiles);
-        if (kFill && g < sc.n) next_rect = pair_rect[g];
         for (uint32_t v = 0; v < n_views; v++) {
             if (views[v].route) {
                 // a view without bins: only keep the fill pass's one-view-ahead record read going
-                if (kFill && g < sc.n && v + 1u < n_views) next_rect = pair_rect[(size_t)(v + 1u) * sc.n + g];
+                if (kFill && v + 1u < n_views)
+#pragma unroll
+                    for (uint32_t k = 0; k < kPer; k++)
+                        if (g[k] < sc.n) next_rect[k] = pair_rect[(size_t)(v + 1u) * sc.n + g[k]];
                 continue;
             }
             if constexpr (!kFill) {
@@ -646,37 +668,54 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
             const ViewDesc& vd = views[v];
             const uint32_t bin0 = vd.tile_begin, gw = vd.grid_w, bl = vd.bin_log;
             const uint32_t ms = subrect_shift(bl, sc.n);
-            TileRange t16 = { 0u, 0u, 0u, 0u };
-            uint32_t key = 0;
-            bool has = false;
-            const size_t pair = (size_t)v * sc.n + g;
-            if constexpr (kFill) {
-                const uint2 r = next_rect;
-                if (g < sc.n && v + 1u < n_views) next_rect = pair_rect[pair + sc.n];
-                has = r.x != kNoTiles;
-                t16 = { r.x & 0xffffu, r.y & 0xffffu, r.x >> 16, r.y >> 16 };
-                // the depth key again from the mean in registers (the operations of
-                // gaussian_tiles_reg, so the same bits), stored as its sort bucket
-                const float4 r3 = vd.r3;
-                key = sort_bucket(f2u(r3.x * mx + r3.y * my + r3.z * mz + r3.w), view_keys[v]);
-            } else {
-                bool exact = false;
-                has = gaussian_tiles_bound(mx, my, mz, cv, trace, valid, vd, kRasterLog2, t16, exact);
-                if (exact) has = gaussian_tiles_reg(mx, my, mz, cv, valid, vd, kRasterLog2, t16, key);
-                if (g < sc.n)
-                    pair_rect[pair] = has ? make_uint2(t16.tx0 | (t16.ty0 << 16), t16.tx1 | (t16.ty1 << 16))
-                                          : make_uint2(kNoTiles, kNoTiles);
+            TileRange t16[kPer], tr[kPer];
+            uint32_t key[kPer], tw[kPer], n_t[kPer];
+            bool has[kPer], small[kPer];
+            uint32_t wx0l = 0xffffffffu, wy0l = 0xffffffffu, wx1l = 0u, wy1l = 0u, sum = 0u, vis = 0u;
+#pragma unroll
+            for (uint32_t k = 0; k < kPer; k++) {
+                t16[k] = { 0u, 0u, 0u, 0u };
+                key[k] = 0u;
+                const size_t pair = (size_t)v * sc.n + g[k];
+                if constexpr (kFill) {
+                    const uint2 r = next_rect[k];
+                    if (g[k] < sc.n && v + 1u < n_views) next_rect[k] = pair_rect[pair + sc.n];
+                    has[k] = r.x != kNoTiles;
+                    t16[k] = { r.x & 0xffffu, r.y & 0xffffu, r.x >> 16, r.y >> 16 };
+                    // the depth key again from the mean in registers (the operations of
+                    // gaussian_tiles_reg, so the same bits), stored as its sort bucket
+                    const float4 r3 = vd.r3;
+                    key[k] = sort_bucket(f2u(r3.x * mx[k] + r3.y * my[k] + r3.z * mz[k] + r3.w), view_keys[v]);
+                } else {
+                    const bool valid = g[k] < sc.n && !(cv[k].opacity < 0.0f);
+                    const float trace = cv[k].c11 + cv[k].c22 + cv[k].c33;
+                    bool exact = false;
+                    has[k] = gaussian_tiles_bound(mx[k], my[k], mz[k], cv[k], trace, valid, vd, kRasterLog2, t16[k],
+                                                  exact);
+                    if (exact)
+                        has[k] = gaussian_tiles_reg(mx[k], my[k], mz[k], cv[k], valid, vd, kRasterLog2, t16[k], key[k]);
+                    if (g[k] < sc.n)
+                        pair_rect[pair] = has[k] ? make_uint2(t16[k].tx0 | (t16[k].ty0 << 16),
+                                                              t16[k].tx1 | (t16[k].ty1 << 16))
+                                                 : make_uint2(kNoTiles, kNoTiles);
+                }
+                tr[k] = { t16[k].tx0 >> bl, t16[k].tx1 >> bl, t16[k].ty0 >> bl, t16[k].ty1 >> bl };  // bins
+                tw[k] = has[k] ? tr[k].tx1 - tr[k].tx0 + 1u : 0u;
+                n_t[k] = has[k] ? tw[k] * (tr[k].ty1 - tr[k].ty0 + 1u) : 0u;
+                small[k] = has[k] && n_t[k] <= kSmallTiles;
+                if (small[k]) {
+                    wx0l = min(wx0l, tr[k].tx0);
+                    wy0l = min(wy0l, tr[k].ty0);
+                    wx1l = max(wx1l, tr[k].tx1);
+                    wy1l = max(wy1l, tr[k].ty1);
+                }
+                sum += n_t[k];
+                vis += has[k] ? 1u : 0u;
             }
-            const TileRange tr = { t16.tx0 >> bl, t16.tx1 >> bl, t16.ty0 >> bl, t16.ty1 >> bl };  // bins
-            const uint32_t tw = has ? tr.tx1 - tr.tx0 + 1u : 0u;
-            const uint32_t n_t = has ? tw * (tr.ty1 - tr.ty0 + 1u) : 0u;
-            const bool small = has && n_t <= kSmallTiles;
             {
                 // window of the small Gaussians: wave reductions, one LDS atomic per wave
-                const uint32_t mx0 = wave_min_u32(small ? tr.tx0 : 0xffffffffu);
-                const uint32_t my0 = wave_min_u32(small ? tr.ty0 : 0xffffffffu);
-                const uint32_t mx1 = wave_max_u32(small ? tr.tx1 : 0u);
-                const uint32_t my1 = wave_max_u32(small ? tr.ty1 : 0u);
+                const uint32_t mx0 = wave_min_u32(wx0l), my0 = wave_min_u32(wy0l);
+                const uint32_t mx1 = wave_max_u32(wx1l), my1 = wave_max_u32(wy1l);
                 if (lane == 0u && mx0 <= mx1) {
                     atomicMin(kFill ? &s_win[wb][0] : &s_wx0, mx0);
                     atomicMax(kFill ? &s_win[wb][1] : &s_wx1, mx1);
@@ -686,23 +725,26 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
             }
             if constexpr (!kFill) {
                 // a Gaussian touches < 2^24 tiles, so a wave's sum fits 32 bits
-                const uint32_t sum = wave_sum_u32(n_t);
-                if (lane == 0u && sum) atomicAdd(&s_total, (unsigned long long)sum);
-                const uint32_t vis = (uint32_t)__popcll(__ballot(has));
-                if (lane == 0u && vis) atomicAdd(&s_visible, (unsigned long long)vis);
+                const uint32_t wsum = wave_sum_u32(sum), wvis = wave_sum_u32(vis);
+                if (lane == 0u && wsum) atomicAdd(&s_total, (unsigned long long)wsum);
+                if (lane == 0u && wvis) atomicAdd(&s_visible, (unsigned long long)wvis);
             }
             // large Gaussians: the wave walks each one's tiles
-            uint64_t big = __ballot(has && !small);
-            while (big) {
-                const int l = __builtin_ctzll(big);
-                big &= big - 1ull;
-                const uint32_t bx0 = bcast(tr.tx0, l), by0 = bcast(tr.ty0, l), bw = bcast(tw, l), bn = bcast(n_t, l);
-                const uint32_t bg = bcast(g, l), bkey = bcast(key, l);
-                const TileRange b16 = { bcast(t16.tx0, l), bcast(t16.tx1, l), bcast(t16.ty0, l), bcast(t16.ty1, l) };
-                for (uint32_t k = lane; k < bn; k += 64u) {
-                    const uint32_t ty = by0 + k / bw, tx = bx0 + k % bw;
-                    bin_direct<kFill>(bin0 + ty * gw + tx, instance_word(bg, tx, ty, b16, bl, ms), bkey, tile_count,
-                                      tile_cursor, offsets, entries);
+#pragma unroll
+            for (uint32_t k = 0; k < kPer; k++) {
+                uint64_t big = __ballot(has[k] && !small[k]);
+                while (big) {
+                    const int l = __builtin_ctzll(big);
+                    big &= big - 1ull;
+                    const uint32_t bx0 = bcast(tr[k].tx0, l), by0 = bcast(tr[k].ty0, l), bw = bcast(tw[k], l);
+                    const uint32_t bn = bcast(n_t[k], l), bg = bcast(g[k], l), bkey = bcast(key[k], l);
+                    const TileRange b16 = { bcast(t16[k].tx0, l), bcast(t16[k].tx1, l), bcast(t16[k].ty0, l),
+                                            bcast(t16[k].ty1, l) };
+                    for (uint32_t kk = lane; kk < bn; kk += 64u) {
+                        const uint32_t ty = by0 + kk / bw, tx = bx0 + kk % bw;
+                        bin_direct<kFill>(bin0 + ty * gw + tx, instance_word(bg, tx, ty, b16, bl, ms), bkey,
+                                          tile_count, tile_cursor, offsets, entries);
+                    }
                 }
             }
             __syncthreads();
@@ -722,10 +764,12 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
                         for (uint32_t i = tid; i < nw * kHistCopies; i += kBatchBlock) hist[i] = 0u;
                         __syncthreads();
                     }
-                    if (small)
-                        for (uint32_t ty = tr.ty0; ty <= tr.ty1; ty++)
-                            for (uint32_t tx = tr.tx0; tx <= tr.tx1; tx++)
-                                atomicAdd(&hist[((ty - wy0) * wt + tx - wx0) * kHistCopies + hc], 1u);
+#pragma unroll
+                    for (uint32_t k = 0; k < kPer; k++)
+                        if (small[k])
+                            for (uint32_t ty = tr[k].ty0; ty <= tr[k].ty1; ty++)
+                                for (uint32_t tx = tr[k].tx0; tx <= tr[k].tx1; tx++)
+                                    atomicAdd(&hist[((ty - wy0) * wt + tx - wx0) * kHistCopies + hc], 1u);
                     __syncthreads();
                     for (uint32_t i = tid; i < nw; i += kBatchBlock) {
                         uint32_t cc[kHistCopies], c = 0;
@@ -750,21 +794,26 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
                     }
                     if constexpr (kFill) {
                         __syncthreads();
-                        if (small)
-                            for (uint32_t ty = tr.ty0; ty <= tr.ty1; ty++)
-                                for (uint32_t tx = tr.tx0; tx <= tr.tx1; tx++) {
-                                    // the copy's count down to zero: slots hbase .. hbase + count - 1
-                                    const uint32_t i = ((ty - wy0) * wt + tx - wx0) * kHistCopies + hc;
-                                    const uint32_t slot = hbase[i] + atomicSub(&hist[i], 1u) - 1u;
-                                    entries.store(slot, key, instance_word(g, tx, ty, t16, bl, ms));
-                                }
+#pragma unroll
+                        for (uint32_t k = 0; k < kPer; k++)
+                            if (small[k])
+                                for (uint32_t ty = tr[k].ty0; ty <= tr[k].ty1; ty++)
+                                    for (uint32_t tx = tr[k].tx0; tx <= tr[k].tx1; tx++) {
+                                        // the copy's count down to zero: slots hbase .. hbase + count - 1
+                                        const uint32_t i = ((ty - wy0) * wt + tx - wx0) * kHistCopies + hc;
+                                        const uint32_t slot = hbase[i] + atomicSub(&hist[i], 1u) - 1u;
+                                        entries.store(slot, key[k], instance_word(g[k], tx, ty, t16[k], bl, ms));
+                                    }
                     }
-                } else if (small) {
+                } else {
                     // spread-out chunk: one global atomic per instance
-                    for (uint32_t ty = tr.ty0; ty <= tr.ty1; ty++)
-                        for (uint32_t tx = tr.tx0; tx <= tr.tx1; tx++)
-                            bin_direct<kFill>(bin0 + ty * gw + tx, instance_word(g, tx, ty, t16, bl, ms), key,
-                                              tile_count, tile_cursor, offsets, entries);
+#pragma unroll
+                    for (uint32_t k = 0; k < kPer; k++)
+                        if (small[k])
+                            for (uint32_t ty = tr[k].ty0; ty <= tr[k].ty1; ty++)
+                                for (uint32_t tx = tr[k].tx0; tx <= tr[k].tx1; tx++)
+                                    bin_direct<kFill>(bin0 + ty * gw + tx, instance_word(g[k], tx, ty, t16[k], bl, ms),
+                                                      key[k], tile_count, tile_cursor, offsets, entries);
                 }
             }
             __syncthreads();
@@ -1456,12 +1505,21 @@ hipError_t launch_batch_tiles(const SceneArgs& sc, const ViewDesc* views, uint32
     if (e != hipSuccess) return e;
     e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
     if (e != hipSuccess) return e;
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(batch_bin_kernel<true>),
-                                                     kBatchBlock, 0);
-    if (e != hipSuccess) return e;
-    uint64_t grid = (uint64_t)cus * (per_cu < 1 ? 1 : per_cu);
-    if (grid > n_chunks) grid = n_chunks;
-    if (grid < 1) grid = 1;
+    // each binning pass: as many workgroups as fit the device at once (chunks are dealt
+    // round-robin, so a workgroup that waits for a slot would lengthen the tail)
+    uint64_t grids[2];
+    for (int f = 0; f < 2; f++) {
+        const uint32_t n_bin_chunks = (sc.n + (f ? kBinChunk<true> : kBinChunk<false>) - 1u) /
+                                      (f ? kBinChunk<true> : kBinChunk<false>);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, f ? reinterpret_cast<const void*>(batch_bin_kernel<true>)
+                       : reinterpret_cast<const void*>(batch_bin_kernel<false>),
+            kBatchBlock, 0);
+        if (e != hipSuccess) return e;
+        grids[f] = (uint64_t)cus * (per_cu < 1 ? 1 : per_cu);
+        if (grids[f] > n_bin_chunks) grids[f] = n_bin_chunks;
+        if (grids[f] < 1) grids[f] = 1;
+    }
     // the scene's own cache of the view-independent terms, or one computed here per render
     const float4* cov3d = sc.cov3d ? sc.cov3d : tb.cov3d;
     if (sc.n > 0) {
@@ -1469,7 +1527,7 @@ hipError_t launch_batch_tiles(const SceneArgs& sc, const ViewDesc* views, uint32
         hipLaunchKernelGGL(view_key_range_kernel, dim3(n_views), dim3(kRangeThreads), 0, stream, views, tb.centroids,
                            n_chunks, tb.view_keys);
         if (!sc.cov3d) hipLaunchKernelGGL(cov3d_kernel, dim3(n_chunks), dim3(kBatchBlock), 0, stream, sc, tb.cov3d);
-        hipLaunchKernelGGL((batch_bin_kernel<false>), dim3((unsigned)grid), dim3(kBatchBlock), 0, stream, sc, cov3d,
+        hipLaunchKernelGGL((batch_bin_kernel<false>), dim3((unsigned)grids[0]), dim3(kBatchBlock), 0, stream, sc, cov3d,
                            views, n_views, tb.tile_count, tb.tile_cursor, tb.offsets,
                            tb.entries, tb.view_keys, tb.pair_rect, counters);
     }
@@ -1477,7 +1535,7 @@ hipError_t launch_batch_tiles(const SceneArgs& sc, const ViewDesc* views, uint32
                        CountInput{ tb.tile_count }, tb.offsets, nullptr, tb.n_tiles, tb.n_tiles, &counters->n_instances,
                        &counters->overflow, tb.capacity, nullptr);
     if (sc.n > 0) {
-        hipLaunchKernelGGL((batch_bin_kernel<true>), dim3((unsigned)grid), dim3(kBatchBlock), 0, stream, sc, cov3d,
+        hipLaunchKernelGGL((batch_bin_kernel<true>), dim3((unsigned)grids[1]), dim3(kBatchBlock), 0, stream, sc, cov3d,
                            views, n_views, tb.tile_count, tb.tile_cursor, tb.offsets,
                            tb.entries, tb.view_keys, tb.pair_rect, counters);
     }
