@@ -42,7 +42,12 @@ HBM_PEAK_GBS = 8000.0      # MI355X HBM3E
 N_CU = 256                 # MI355X compute units (8 XCDs x 32)
 N_XCD = 8
 PEAK_CLOCK_GHZ = 2.4       # MI355X peak engine clock
-# one wave64 VALU instruction per CU per cycle (four 16-lane SIMDs, four cycles each)
+# VALU issue: each of a CU's four 16-lane SIMDs takes a wave64 VALU instruction every four
+# cycles (a quad-cycle): 256 CUs x 4 SIMDs x 2.4 GHz / 4 = 614.4 G SIMD quad-cycles/s. gfx950
+# issues a second VALU instruction in some quad-cycles: SQ_ACTIVE_INST_VALU counts the
+# quad-cycles of every wave's VALU instructions (both of a pair) and SQ_ACTIVE_INST_VALU2 the
+# quad-cycles with two, so ACTIVE - ACTIVE2 quad-cycles had at least one VALU instruction (the
+# busy fraction, at most one) while instructions per quad-cycle can exceed one
 VALU_PEAK_G_PER_S = N_CU * PEAK_CLOCK_GHZ
 PROFILE_JSON = REPO / "profiles" / "c4_batched_kernels.json"
 PROFILE_MD = REPO / "profiles" / "c4_batched_kernels.md"
@@ -232,6 +237,10 @@ def summarize_profile(trace_dir: str, fetch_dir: str, write_dir: str, sq_dir: st
             cycles = med["GRBM_GUI_ACTIVE"] / N_XCD
             entry["valu_per_cu_cycle"] = med["SQ_INSTS_VALU"] / (N_CU * cycles)
             entry["clock_ghz"] = cycles / (entry["median_ms"] * 1e6)
+            if "SQ_ACTIVE_INST_VALU" in med and "SQ_ACTIVE_INST_VALU2" in med:
+                # SIMD quad-cycles with a VALU issue over the N_CU x 4 SIMDs x cycles / 4 available
+                entry["valu_busy_quad_cycles"] = med["SQ_ACTIVE_INST_VALU"] - med["SQ_ACTIVE_INST_VALU2"]
+                entry["valu_busy"] = entry["valu_busy_quad_cycles"] / (N_CU * cycles)
         kernels[name] = entry
         if name == "sgs::cov3d_kernel":
             renders = len(durations)
@@ -246,8 +255,8 @@ def summarize_profile(trace_dir: str, fetch_dir: str, write_dir: str, sq_dir: st
                "write": "rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/prof_write -o run -- "
                         "python3 bench.py --profile",
                "sq": "rocprofv3 --pmc SQ_INSTS_VALU SQ_WAVES SQ_BUSY_CYCLES SQ_INSTS_LDS SQ_INSTS_SALU "
-                     "SQ_LDS_BANK_CONFLICT GRBM_GUI_ACTIVE --output-format csv -d gpurun_out/prof_sq -o run -- "
-                     "python3 bench.py --profile",
+                     "SQ_LDS_BANK_CONFLICT SQ_ACTIVE_INST_VALU SQ_ACTIVE_INST_VALU2 GRBM_GUI_ACTIVE "
+                     "--output-format csv -d gpurun_out/prof_sq -o run -- python3 bench.py --profile",
                "summary": "python3 bench.py --summarize-profile gpurun_out/prof_trace gpurun_out/prof_fetch "
                           "gpurun_out/prof_write gpurun_out/prof_sq"}}
     PROFILE_JSON.parent.mkdir(exist_ok=True)
@@ -260,8 +269,8 @@ def summarize_profile(trace_dir: str, fetch_dir: str, write_dir: str, sq_dir: st
         f"Calibration: the 1 GiB copy reads FETCH_SIZE {fetch_kb:,.0f} KB and WRITE_SIZE {write_kb:,.0f} KB: "
         f"FETCH_SIZE x{cal['fetch_scale']:.3f}, WRITE_SIZE x{cal['write_scale']:.3f}.", "",
         "| kernel | per render | avg ms | fetched GB | written GB | measured GB/s | of 8 TB/s | VALU wave-instr |"
-        " VALU per CU-cycle | clock GHz |",
-        "|---|---|---|---|---|---|---|---|---|---|"]
+        " VALU instr per SIMD quad-cycle | VALU busy | clock GHz |",
+        "|---|---|---|---|---|---|---|---|---|---|---|"]
     total_ms = total_b = 0.0
     for name, e in kernels.items():
         per = e["dispatches"] / max(renders or 1, 1)
@@ -273,8 +282,9 @@ def summarize_profile(trace_dir: str, fetch_dir: str, write_dir: str, sq_dir: st
             f"| `{name}` | {per:g} | {e['avg_ms']:.3f} | "
             f"{(e.get('fetch_bytes') or 0) / 1e9:.3f} | {(e.get('write_bytes') or 0) / 1e9:.3f} | "
             f"{gbs:.0f} | {gbs / HBM_PEAK_GBS:.3f} | {e['counters'].get('SQ_INSTS_VALU', float('nan')):.3e} | "
-            f"{e.get('valu_per_cu_cycle', float('nan')):.3f} | {e.get('clock_ghz', float('nan')):.2f} |"
-            if gbs is not None else f"| `{name}` | {per:g} | {e['avg_ms']:.3f} | | | | | | | |")
+            f"{e.get('valu_per_cu_cycle', float('nan')):.3f} | {e.get('valu_busy', float('nan')):.3f} | "
+            f"{e.get('clock_ghz', float('nan')):.2f} |"
+            if gbs is not None else f"| `{name}` | {per:g} | {e['avg_ms']:.3f} | | | | | | | | |")
     lines += ["", f"One render: {total_ms:.2f} ms of kernels, {total_b / 1e9:.2f} GB of HBM traffic "
                   f"({total_b / (total_ms * 1e-3) / 1e9:.0f} GB/s over the kernel time)."]
     PROFILE_MD.write_text("\n".join(lines) + "\n")
@@ -495,7 +505,8 @@ def kernel_table(profile, views, n_instances, visible_pairs, scene_n):
             row["measured_frac"] = round(gbs / HBM_PEAK_GBS, 4)
         if "SQ_INSTS_VALU" in e["counters"]:
             row["valu_wave_instructions"] = e["counters"]["SQ_INSTS_VALU"]
-            row["valu_frac"] = round(e["counters"]["SQ_INSTS_VALU"] / (VALU_PEAK_G_PER_S * 1e9 * e["avg_ms"] * 1e-3), 4)
+        if "valu_busy" in e:
+            row["valu_busy_frac"] = round(e["valu_busy"], 4)
         if name in algo:
             row["algorithmic_bytes"] = round(algo[name])
             row["algorithmic_frac"] = round(algo[name] / (e["avg_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
@@ -508,25 +519,29 @@ def roofline_raster(profile, views, raster_ms: list, n_instances: int, visible_p
     """The dominant kernel: the tile rasteriser (its one-layer launch for views of one sample,
     its 8-layer launch over whole tiles for views of up to 8 samples and its 16-layer launch
     over half tiles for views of more, timed together with HIP events). It is bound by
-    VALU issue: wave64 VALU instructions per launch (SQ_INSTS_VALU of the profile summary)
-    over the live launch time, against 256 CUs x one wave-instruction per cycle x 2.4 GHz.
-    Beside it the HBM view: algorithmic bytes (44 B per visible (view, Gaussian) pair, a 4 B
-    instance word per list entry, 8 B per framebuffer word written) and the measured bytes
-    (profile)."""
+    VALU issue: the SIMD quad-cycles with a VALU issue per launch (SQ_ACTIVE_INST_VALU less
+    SQ_ACTIVE_INST_VALU2 of the profile summary) over the live launch time, against 256 CUs x 4
+    SIMDs x 2.4 GHz / 4, with the wave64 VALU instructions (SQ_INSTS_VALU) and their rate beside
+    it (above the busy rate where two instructions issue in one quad-cycle). Beside those the HBM
+    view: algorithmic bytes (44 B per visible (view, Gaussian) pair, a 4 B instance word per
+    list entry, 8 B per framebuffer word written) and the measured bytes (profile)."""
     words = samples_per_step(views)
     algo = 44.0 * visible_pairs + 4.0 * n_instances + 8.0 * words
     t = sum(raster_ms) / len(raster_ms) * 1e-3
     out = {"kernel": "sgs::batch_tile_raster_kernel<1, 16> + <8, 16> + <16, 8> (one stage)",
            "avg_launch_ms": round(t * 1e3, 3)}
     ks = [profile["kernels"].get(k) for k in RASTER_KERNELS] if profile else [None]
-    if all(k is not None and "SQ_INSTS_VALU" in k["counters"] for k in ks):
+    if all(k is not None and "valu_busy_quad_cycles" in k for k in ks):
+        busy = sum(k["valu_busy_quad_cycles"] for k in ks)
         valu = sum(k["counters"]["SQ_INSTS_VALU"] for k in ks)
         traffic = sum(k.get("hbm_bytes", 0.0) for k in ks)
-        achieved = valu / t / 1e9
+        achieved = busy / t / 1e9
         out.update({"bound": "valu", "achieved": round(achieved, 1), "peak": VALU_PEAK_G_PER_S,
-                    "unit": "G wave64 VALU instructions/s", "frac": round(achieved / VALU_PEAK_G_PER_S, 4),
+                    "unit": "G VALU-busy SIMD quad-cycles/s", "frac": round(achieved / VALU_PEAK_G_PER_S, 4),
                     "traffic": round(traffic),
+                    "valu_busy_quad_cycles_per_launch": busy,
                     "valu_wave_instructions_per_launch": valu,
+                    "valu_instructions_g_per_s": round(valu / t / 1e9, 1),
                     "profile_avg_ms": round(sum(k["avg_ms"] for k in ks), 3),
                     "profile": str(PROFILE_JSON.relative_to(REPO)), "profile_stale": profile["stale"]})
     else:
