@@ -137,7 +137,7 @@ def tune_render(means, scales, rotations, opacities, sh_coefficients, settings: 
 # small_bin_max_pixels: views whose framebuffer has at most this many pixels get 16-pixel
 # bins, the others 32 (-1: each view's own bin_size, by default the library's choice);
 # streams: view groups rendered by separate batches on concurrent streams
-BATCH_SPACE = {"schedule": [1, 2], "tile_samples": [8, 16], "small_bin_max_pixels": [-1, 0, 1 << 18, 1 << 21],
+BATCH_SPACE = {"schedule": [1, 2], "tile_samples": [0, 8, 16], "small_bin_max_pixels": [-1, 0, 1 << 18, 1 << 21],
                "streams": [1, 2, 4]}
 
 
