@@ -1082,18 +1082,34 @@ __global__ __launch_bounds__(kBatchBlock) void tile_sort_kernel(uint32_t n_views
         entries_in.gaussian[begin + i];
 }
 
-// inclusive scan of one value per thread over the 256-thread workgroup
-__device__ __forceinline__ uint32_t block_inclusive_scan(uint32_t x, uint32_t* wave_sums)
+// Inclusive scan over the wave through DPP (VALU, no LDS round trip): row_shr 1, 2, 4, 8 scan
+// each row of 16 lanes (lanes without a source add 0), row_bcast 15 adds the end of rows 0 and
+// 2 to rows 1 and 3, row_bcast 31 the end of rows 0-1 to rows 2 and 3.
+__device__ __forceinline__ uint32_t wave_inclusive_scan_dpp(uint32_t x)
+{
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, false);  // row_shr:1
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, false);  // row_shr:2
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, false);  // row_shr:4
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, false);  // row_shr:8
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false);  // row_bcast:15
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false);  // row_bcast:31
+    return x;
+}
+
+// inclusive scan of one value per thread over the 256-thread workgroup; total: their sum
+__device__ __forceinline__ uint32_t block_inclusive_scan(uint32_t x, uint32_t* wave_sums, uint32_t& total)
 {
     const uint32_t lane = lane_id(), wave = threadIdx.x / 64u;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t y = (uint32_t)__shfl_up((int)x, off, 64);
-        if ((int)lane >= off) x += y;
-    }
+    x = wave_inclusive_scan_dpp(x);
     if (lane == 63u) wave_sums[wave] = x;
     __syncthreads();
-    for (uint32_t w = 0; w < wave; w++) x += wave_sums[w];
+    total = 0u;
+#pragma unroll
+    for (uint32_t w = 0; w < kBatchBlock / 64u; w++) {
+        const uint32_t sw = wave_sums[w];
+        if (w < wave) x += sw;
+        total += sw;
+    }
     return x;
 }
 
@@ -1185,6 +1201,7 @@ __global__ __launch_bounds__(kBatchBlock, kLayers == 1u ? 7 : 5) void batch_tile
     __shared__ float s_ca[kBatchBlock], s_cb[kBatchBlock], s_cc[kBatchBlock];
     __shared__ uint32_t s_rect[kBatchBlock], s_end[kBatchBlock];
     __shared__ uint32_t s_wave[kBatchBlock / 64u];
+    __shared__ uint16_t s_start[kBatchBlock];  // the instance holding each lane's first quad
     // the batch's instances that reach this tile, in s_end: the projection reads them before the
     // scan's barrier, and s_end is written after it
     uint32_t* const s_g = s_end;
@@ -1336,28 +1353,27 @@ __global__ __launch_bounds__(kBatchBlock, kLayers == 1u ? 7 : 5) void batch_tile
             }
             s_rect[tid] = rect;
             // pixel quads of the batch's live instances, flattened over the workgroup's lanes
-            const uint32_t incl = block_inclusive_scan(cnt, s_wave);
+            uint32_t total;
+            const uint32_t incl = block_inclusive_scan(cnt, s_wave, total);
+            // Each lane walks a contiguous range of per = ceil(total / 256) of the batch's 2x2
+            // pixel quads (an instance's rectangle advances without division; the quad's pixels
+            // beyond an odd width or height are skipped). Fragments that some sample word of
+            // their pixel still lies behind go to the wave's LDS queue; alpha and the samples are
+            // evaluated on full queues of 64, so no lane idles on a rejected fragment. Each
+            // instance names itself to the lanes whose range starts in its quads
+            // [incl - cnt, incl) (t per < incl <= total <= 256 per keeps t below 256).
+            const uint32_t per = (total + kBatchBlock - 1u) / kBatchBlock;
+            if (cnt)
+                for (uint32_t t = (incl - cnt + per - 1u) / per; t * per < incl; t++) s_start[t] = (uint16_t)tid;
             s_end[tid] = incl;
             __syncthreads();
-            const uint32_t total = s_end[kBatchBlock - 1u];
-            // Each lane walks a contiguous range of the batch's 2x2 pixel quads (an instance's
-            // rectangle advances without division; the quad's pixels beyond an odd width or
-            // height are skipped). Fragments that some sample word of their pixel still lies
-            // behind go to the wave's LDS queue; alpha and the samples are evaluated on full
-            // queues of 64, so no lane idles on a rejected fragment.
-            const uint32_t per = (total + kBatchBlock - 1u) / kBatchBlock;
             uint32_t k = tid * per;
             const uint32_t kend = min(k + per, total);
             uint32_t j = 0, jend = 0, rect_j = 0, qw = 1, q = 0, r = 0, fdepth = 0;
             float msx = 0.0f, msy = 0.0f, thr = 0.0f;
             ConicExp2 ce = { 0.0f, 0.0f, 0.0f };
             if (k < kend) {
-                uint32_t bb = kBatchBlock - 1u;
-                while (j < bb) {
-                    const uint32_t m = (j + bb) >> 1;
-                    if (s_end[m] > k) bb = m;
-                    else j = m + 1u;
-                }
+                j = s_start[tid];
                 jend = s_end[j];
                 const uint32_t f = k - (j ? s_end[j - 1u] : 0u);
                 rect_j = s_rect[j];
