@@ -1119,7 +1119,7 @@ __device__ __forceinline__ uint32_t block_inclusive_scan(uint32_t x, uint32_t* w
 constexpr uint32_t kSub = 4;
 // One queued fragment (instance j << 8 | pixel l): alpha once, then every sample word of the
 // pixel the fragment still lies in front of gets its hash test and, on acceptance, the LDS
-// atomicMin. The words of the G samples of the pass are read in groups of up to four.
+// atomicMin. The words of the G samples of the pass are read in groups of four or eight.
 template <uint32_t kLayers, uint32_t kPix>
 __device__ __forceinline__ void tile_fragment(uint32_t entry, unsigned long long (*layer)[kPix],
                                               const uint64_t* s_frag, const float* s_msx, const float* s_msy,
@@ -1145,7 +1145,8 @@ __device__ __forceinline__ void tile_fragment(uint32_t entry, unsigned long long
     // decrease, so a stale read never drops a winner. The comparison is the branch condition
     // itself (a lane mask), and a sample no lane tests is skipped as a whole; each sample's
     // frame term is a wave-uniform constant.
-    constexpr uint32_t kGroup = kLayers < 4u ? kLayers : 4u;
+    // (eight words in flight for 16 samples measured faster than four; four for 8)
+    constexpr uint32_t kGroup = kLayers < 4u ? kLayers : kLayers < 16u ? 4u : 8u;
 #pragma unroll
     for (uint32_t s0 = 0; s0 < kLayers; s0 += kGroup) {
         if (s0 >= G) break;
