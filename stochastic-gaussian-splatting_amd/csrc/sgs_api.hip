@@ -870,6 +870,7 @@ int accumulate_tiles_single(const sgs_gaussians* g, const sgs_settings* s, const
     tb.view_keys = reinterpret_cast<uint2*>(tile_ws + L.view_keys);
     tb.cov3d = reinterpret_cast<float4*>(tile_ws + L.cov3d);
     tb.pair_rect = reinterpret_cast<uint2*>(tile_ws + L.pair_rect);
+    tb.compact_records = (cam.width_i + 15u) / 16u <= 255u && (cam.height_i + 15u) / 16u <= 255u ? 1u : 0u;
     tb.n_tiles = L.n_tiles;
     tb.capacity = L.capacity;
     tb.n_raster = prefixes.w[5];
@@ -1022,6 +1023,12 @@ int sgs_render_batch(const sgs_gaussians* gaussians, const sgs_view* views, uint
         tb.view_keys = reinterpret_cast<uint2*>(ws + L.view_keys);
         tb.cov3d = reinterpret_cast<float4*>(ws + L.cov3d);
         tb.pair_rect = reinterpret_cast<uint2*>(ws + L.pair_rect);
+        tb.compact_records = 1u;
+        for (uint32_t v = 0; v < n_views; v++) {
+            const uint64_t ss = views[v].supersampling_factor > 1 ? (uint64_t)views[v].supersampling_factor : 1u;
+            if ((views[v].width * ss + 15u) / 16u > 255u || (views[v].height * ss + 15u) / 16u > 255u)
+                tb.compact_records = 0u;
+        }
         tb.n_tiles = (uint32_t)L.n_tiles;
         tb.capacity = L.capacity;
         tb.n_raster = (uint32_t)L.n_raster;

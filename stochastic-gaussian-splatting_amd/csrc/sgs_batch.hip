@@ -565,6 +565,35 @@ hipError_t launch_cov3d(const SceneArgs& sc, float4* cache, hipStream_t stream)
 // tiles is marked by all ones
 constexpr uint32_t kNoTiles = 0xffffffffu;
 
+// The counting pass's record of each (view, Gaussian) pair, read back by the fill pass: its
+// rectangle of 16x16 tiles, 8 bytes (16-bit corners), or 4 bytes (8-bit corners) when every
+// view's tile grid has at most 255 columns and rows (framebuffers up to 4080 pixels a side),
+// so that all ones, the mark of a pair without tiles, is no rectangle.
+struct PairRecords {
+    void* base;
+    uint32_t compact;
+    __device__ __forceinline__ uint2 load(size_t i) const
+    {
+        if (compact) return make_uint2(static_cast<const uint32_t*>(base)[i], 0u);
+        return static_cast<const uint2*>(base)[i];
+    }
+    __device__ __forceinline__ void store(size_t i, bool has, const TileRange& t) const
+    {
+        if (compact)
+            static_cast<uint32_t*>(base)[i] = has ? t.tx0 | (t.tx1 << 8) | (t.ty0 << 16) | (t.ty1 << 24) : kNoTiles;
+        else
+            static_cast<uint2*>(base)[i] = has ? make_uint2(t.tx0 | (t.ty0 << 16), t.tx1 | (t.ty1 << 16))
+                                               : make_uint2(kNoTiles, kNoTiles);
+    }
+    // the rectangle of a loaded record; false for a pair without tiles
+    __device__ __forceinline__ bool decode(uint2 r, TileRange& t) const
+    {
+        if (compact) t = { r.x & 0xffu, (r.x >> 8) & 0xffu, (r.x >> 16) & 0xffu, r.x >> 24 };
+        else t = { r.x & 0xffffu, r.y & 0xffffu, r.x >> 16, r.y >> 16 };
+        return r.x != kNoTiles;
+    }
+};
+
 // Gaussians per lane in the binning passes (counting, fill): a workgroup bins chunks of
 // kBinPer x 256 Gaussians per view step, so the per-step work shared by the workgroup (window
 // reductions, barriers, the histogram's flush with one global atomic per touched bin) is paid
@@ -583,7 +612,7 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
                                                                 const uint32_t* __restrict__ offsets,
                                                                 EntryArrays entries,
                                                                 const uint2* __restrict__ view_keys,
-                                                                uint2* __restrict__ pair_rect,
+                                                                PairRecords pair_rect,
                                                                 BatchCounters* __restrict__ ctr)
 {
     constexpr uint32_t kHistCopies = kHistCopiesOf<kFill>;
@@ -645,7 +674,7 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
             // the fill pass reads each view's pair record one view ahead, so the load of view
             // v + 1 is in flight while view v is binned
             next_rect[k] = make_uint2(kNoTiles, kNoTiles);
-            if (kFill && g[k] < sc.n) next_rect[k] = pair_rect[g[k]];
+            if (kFill && g[k] < sc.n) next_rect[k] = pair_rect.load(g[k]);
         }
         for (uint32_t v = 0; v < n_views; v++) {
             if (views[v].route) {
@@ -653,7 +682,7 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
                 if (kFill && v + 1u < n_views)
 #pragma unroll
                     for (uint32_t k = 0; k < kPer; k++)
-                        if (g[k] < sc.n) next_rect[k] = pair_rect[(size_t)(v + 1u) * sc.n + g[k]];
+                        if (g[k] < sc.n) next_rect[k] = pair_rect.load((size_t)(v + 1u) * sc.n + g[k]);
                 continue;
             }
             if constexpr (!kFill) {
@@ -679,9 +708,8 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
                 const size_t pair = (size_t)v * sc.n + g[k];
                 if constexpr (kFill) {
                     const uint2 r = next_rect[k];
-                    if (g[k] < sc.n && v + 1u < n_views) next_rect[k] = pair_rect[pair + sc.n];
-                    has[k] = r.x != kNoTiles;
-                    t16[k] = { r.x & 0xffffu, r.y & 0xffffu, r.x >> 16, r.y >> 16 };
+                    if (g[k] < sc.n && v + 1u < n_views) next_rect[k] = pair_rect.load(pair + sc.n);
+                    has[k] = pair_rect.decode(r, t16[k]);
                     // the depth key again from the mean in registers (the operations of
                     // gaussian_tiles_reg, so the same bits), stored as its sort bucket
                     const float4 r3 = vd.r3;
@@ -694,10 +722,7 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
                                                   exact);
                     if (exact)
                         has[k] = gaussian_tiles_reg(mx[k], my[k], mz[k], cv[k], valid, vd, kRasterLog2, t16[k], key[k]);
-                    if (g[k] < sc.n)
-                        pair_rect[pair] = has[k] ? make_uint2(t16[k].tx0 | (t16[k].ty0 << 16),
-                                                              t16[k].tx1 | (t16[k].ty1 << 16))
-                                                 : make_uint2(kNoTiles, kNoTiles);
+                    if (g[k] < sc.n) pair_rect.store(pair, has[k], t16[k]);
                 }
                 tr[k] = { t16[k].tx0 >> bl, t16[k].tx1 >> bl, t16[k].ty0 >> bl, t16[k].ty1 >> bl };  // bins
                 tw[k] = has[k] ? tr[k].tx1 - tr[k].tx0 + 1u : 0u;
@@ -1546,7 +1571,7 @@ hipError_t launch_batch_tiles(const SceneArgs& sc, const ViewDesc* views, uint32
         if (!sc.cov3d) hipLaunchKernelGGL(cov3d_kernel, dim3(n_chunks), dim3(kBatchBlock), 0, stream, sc, tb.cov3d);
         hipLaunchKernelGGL((batch_bin_kernel<false>), dim3((unsigned)grids[0]), dim3(kBatchBlock), 0, stream, sc, cov3d,
                            views, n_views, tb.tile_count, tb.tile_cursor, tb.offsets,
-                           tb.entries, tb.view_keys, tb.pair_rect, counters);
+                           tb.entries, tb.view_keys, PairRecords{ tb.pair_rect, tb.compact_records }, counters);
     }
     hipLaunchKernelGGL((exclusive_scan_kernel<CountInput>), dim3(1), dim3(kScanThreads), 0, stream,
                        CountInput{ tb.tile_count }, tb.offsets, nullptr, tb.n_tiles, tb.n_tiles, &counters->n_instances,
@@ -1554,7 +1579,7 @@ hipError_t launch_batch_tiles(const SceneArgs& sc, const ViewDesc* views, uint32
     if (sc.n > 0) {
         hipLaunchKernelGGL((batch_bin_kernel<true>), dim3((unsigned)grids[1]), dim3(kBatchBlock), 0, stream, sc, cov3d,
                            views, n_views, tb.tile_count, tb.tile_cursor, tb.offsets,
-                           tb.entries, tb.view_keys, tb.pair_rect, counters);
+                           tb.entries, tb.view_keys, PairRecords{ tb.pair_rect, tb.compact_records }, counters);
     }
     const dim3 order_grid((tb.n_tiles + kOrderThreads - 1u) / kOrderThreads);
     hipLaunchKernelGGL(bin_order_kernel<false>, order_grid, dim3(kOrderThreads), 0, stream, tb.offsets, tb.n_tiles,

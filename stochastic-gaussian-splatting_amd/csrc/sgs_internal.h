@@ -234,7 +234,8 @@ struct TileBuffers {
     uint32_t* sorted_keys;      // [capacity] depth bucket floor at each tile batch start (begin + 256 m)
     float4* centroids;          // [n_chunks] mean position of each chunk of 256 Gaussians
     float4* cov3d;              // [2 * n] view-independent projection terms of each Gaussian
-    uint2* pair_rect;           // [n_views * n] bin rectangle of each (view, Gaussian) pair
+    uint2* pair_rect;           // [n_views * n] tile rectangle of each (view, Gaussian) pair
+    uint32_t compact_records;   // 1: 4-byte records (every view's 16x16-tile grid within 255 x 255)
     uint2* view_keys;           // [n_views] depth-key bucketing of each view (kmin, shift)
     uint32_t n_tiles, capacity;  // bins of the batch; instance capacity
     uint32_t n_raster;           // rasteriser workgroups: 16x16 tiles of every view's bins

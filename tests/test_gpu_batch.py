@@ -106,13 +106,14 @@ def test_batch_equals_plain_spp_loop(gpu, tuning):
 
 
 @pytest.mark.parametrize("case", ["many_samples", "many_samples_8", "wide_views", "wide_views_16",
-                                  "shade_table_overflow"])
+                                  "very_wide", "shade_table_overflow"])
 def test_batch_tile_schedule_edges(gpu, case, oracle_mod):
     """More than 16 samples (several LDS passes per tile), views wider than a workgroup's
-    1024-tile binning window with large Gaussians (direct binning), and a resolve tile with
-    more distinct winners than the batched resolve's 2048-slot shading table (the winners it
-    cannot take are shaded by their pixel), each against the scatter schedule and the plain
-    SPP loop."""
+    1024-tile binning window with large Gaussians (direct binning), a framebuffer wider than
+    4080 pixels (the 8-byte pair records of the binning passes, in the batch and in the plain
+    calls), and a resolve tile with more distinct winners than the batched resolve's
+    2048-slot shading table (the winners it cannot take are shaded by their pixel), each
+    against the scatter schedule and the plain SPP loop."""
     import torch
     from StochasticGSCudaBackend.batch import BatchRenderer, BatchTuning, BatchView
     if case.startswith("many_samples"):
@@ -127,6 +128,11 @@ def test_batch_tile_schedule_edges(gpu, case, oracle_mod):
                        for k in range(40)])
         won = fb[(fb >> np.uint64(32)) != np.uint64(0xFFFFFFFF)] & np.uint64(0xFFFFFFFF)
         assert len(np.unique(won)) > 2 * 2048  # the oracle's winners: 4198 distinct
+    elif case == "very_wide":
+        # a 4200 x 120 framebuffer: 263 columns of 16x16 tiles
+        arrays = small_scene(3000, seed=48, log_scale_mean=-3.3, log_scale_std=0.8)
+        cams = [small_camera(1400, 40, angle=0.2, supersampling_factor=3), small_camera(64, 48, angle=1.1)]
+        spps, frames = [2, 3], [70, 80]
     else:
         arrays = small_scene(1500, seed=46, log_scale_mean=-2.6, log_scale_std=0.6)
         cams = [small_camera(640, 520, angle=0.2), small_camera(300, 200, angle=2.5, supersampling_factor=3)]
