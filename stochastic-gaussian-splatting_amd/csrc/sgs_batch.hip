@@ -599,7 +599,7 @@ struct PairRecords {
 // reductions, barriers, the histogram's flush with one global atomic per touched bin) is paid
 // once per kBinPer x 256 pairs.
 template <bool kFill>
-constexpr uint32_t kBinPer = kFill ? 1u : 2u;
+constexpr uint32_t kBinPer = kFill ? 1u : 4u;  // measured: count 2.85 ms with 2, 2.96 with 3, 2.73 with 4
 template <bool kFill>
 constexpr uint32_t kBinChunk = kBinPer<kFill> * kBatchBlock;
 
