@@ -30,6 +30,16 @@ def test_coupled_cost_needs_second_pass(search):
     assert ms1 <= 4
 
 
+def test_min_gain_keeps_defaults_within_noise(search):
+    # 0.5 % faster stays below a 1 % threshold, 5 % faster passes it
+    space = {"a": [0, 1], "b": [0, 1]}
+    cost = lambda c: 100.0 - 0.5 * c["a"] - 5.0 * c["b"]
+    best, ms, _ = search(cost, space, {"a": 0, "b": 0}, min_gain=0.01)
+    assert best == {"a": 0, "b": 1} and ms == 95.0
+    best0, _, _ = search(cost, space, {"a": 0, "b": 0})
+    assert best0 == {"a": 1, "b": 1}
+
+
 def test_defaults_are_candidates(search):
     best, ms, trials = search(lambda c: 1.0, {"x": [5, 6]}, {"x": 0})
     assert best == {"x": 0} and trials[0][0] == {"x": 0}
