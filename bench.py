@@ -482,14 +482,19 @@ def kernel_table(profile, views, n_instances, visible_pairs, scene_n):
         return None
     words = samples_per_step(views)
     pairs_all = scene_n * len(views)
+    # pair records are 4 B (8-bit tile corners) while every view's framebuffer has at most
+    # 255 16-pixel tiles a side, else 8 B
+    fb_side = max(max(v["width"], v["height"]) * max(1, int(v.get("supersampling_factor", 1)))
+                  for v in views)
+    record = 4.0 if (fb_side + 15) // 16 <= 255 else 8.0
     algo = {
         # 32 B read (scales, rotation) + opacity, 32 B written per Gaussian
         "sgs::cov3d_kernel": 64.0 * scene_n,
-        # 44 B of parameters per Gaussian, 8 B pair record written per (view, Gaussian)
-        "sgs::batch_bin_kernel<false>": 44.0 * scene_n + 8.0 * pairs_all,
+        # 44 B of parameters per Gaussian, a pair record written per (view, Gaussian)
+        "sgs::batch_bin_kernel<false>": 44.0 * scene_n + record * pairs_all,
         # the pair record read back, 12 B of means per Gaussian, a 2 B depth bucket and a 4 B
         # Gaussian index written per instance
-        "sgs::batch_bin_kernel<true>": 8.0 * pairs_all + 12.0 * scene_n + 6.0 * n_instances,
+        "sgs::batch_bin_kernel<true>": record * pairs_all + 12.0 * scene_n + 6.0 * n_instances,
         # every bucket and index read once, a 4 B sorted index written per instance
         "sgs::tile_sort_kernel": 10.0 * n_instances,
     }
