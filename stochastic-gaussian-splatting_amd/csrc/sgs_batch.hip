@@ -396,23 +396,47 @@ __device__ __forceinline__ bool gaussian_tiles_reg(float mx, float my, float mz,
 // Row norms are bounded by their L1 norms. A Gaussian whose bounds are not finite takes the
 // exact path. The rasteriser projects exactly and clips every instance to its exact AABB, so
 // an extra tile changes no result.
-__device__ __forceinline__ bool gaussian_tiles_bound(float mx, float my, float mz, const Cov3d& cv, float trace,
-                                                     bool valid, const ViewDesc& vd, uint32_t shift, TileRange& t,
-                                                     bool& exact)
+// the per-view terms of gaussian_tiles_bound, computed once per view and wave and kept in
+// scalar registers
+struct BoundView {
+    float n1, n2, n3;  // L1 bounds of the rotation rows' Euclidean norms, rounded up
+    float kernel;      // the antialiasing kernel added to the covariance diagonal
+};
+
+__device__ __forceinline__ float uniform_f32(float x)
 {
-    exact = false;
-    if (!valid) return false;
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
+}
+
+__device__ __forceinline__ BoundView bound_view(const ViewDesc& vd)
+{
+    const float4 r1 = vd.r1, r2 = vd.r2, r3 = vd.r3;
+    return { uniform_f32((fabsf(r1.x) + fabsf(r1.y) + fabsf(r1.z)) * (1.0f + 0x1p-20f)),
+             uniform_f32((fabsf(r2.x) + fabsf(r2.y) + fabsf(r2.z)) * (1.0f + 0x1p-20f)),
+             uniform_f32((fabsf(r3.x) + fabsf(r3.y) + fabsf(r3.z)) * (1.0f + 0x1p-20f)),
+             uniform_f32((vd.cam.proper_antialiasing ? 0.1f : 0.3f) * vd.cam.ss * vd.cam.ss) };
+}
+
+// Every lane evaluates the whole bound (nearly every pair is in front of the camera), and the
+// tests combine into one condition, so the conversions to tile indices run under a single
+// branch.
+__device__ __forceinline__ bool gaussian_tiles_bound(float mx, float my, float mz, const Cov3d& cv, float trace,
+                                                     bool valid, const ViewDesc& vd, const BoundView& bv,
+                                                     uint32_t shift, TileRange& t, bool& exact)
+{
     const float4 r1 = vd.r1, r2 = vd.r2, r3 = vd.r3;
     const FbCamera& cam = vd.cam;
     const float depth = r3.x * mx + r3.y * my + r3.z * mz + r3.w;
-    if (!(depth >= cam.near_plane && depth <= cam.far_plane)) return false;
+    const bool in_range = valid && depth >= cam.near_plane && depth <= cam.far_plane;
     const float X = r1.x * mx + r1.y * my + r1.z * mz + r1.w;
     const float Y = r2.x * mx + r2.y * my + r2.z * mz + r2.w;
     const float rz = __builtin_amdgcn_rcpf(depth);
     const float x = X * rz, y = Y * rz;
     const float dx = fabsf(x) * 0x1p-21f, dy = fabsf(y) * 0x1p-21f;
-    const float xc = fmaxf(cam.clip_left, fminf(x, cam.clip_right));
-    const float yc = fmaxf(cam.clip_top, fminf(y, cam.clip_bottom));
+    // the projection's clamp (clip_left <= clip_right): the median of three for any x that is
+    // not NaN, and a NaN x makes the mean below not finite
+    const float xc = __builtin_amdgcn_fmed3f(x, cam.clip_left, cam.clip_right);
+    const float yc = __builtin_amdgcn_fmed3f(y, cam.clip_top, cam.clip_bottom);
     const float j11 = cam.focal_x * rz, j22 = cam.focal_y * rz;
     const float j13 = -j11 * xc, j23 = -j22 * yc;
     const float a1 = fmaf(j13, r3.x, j11 * r1.x), a2 = fmaf(j13, r3.y, j11 * r1.y), a3 = fmaf(j13, r3.z, j11 * r1.z);
@@ -425,15 +449,12 @@ __device__ __forceinline__ bool gaussian_tiles_bound(float mx, float my, float m
     const float q3 = fmaf(b3, cv.c33, fmaf(b2, cv.c23, b1 * cv.c13));
     const float cxx = fmaf(p3, a3, fmaf(p2, a2, p1 * a1));
     const float cyy = fmaf(q3, b3, fmaf(q2, b2, q1 * b1));
-    // L1 bounds of the rotation rows' Euclidean norms, rounded up
-    const float n1 = (fabsf(r1.x) + fabsf(r1.y) + fabsf(r1.z)) * (1.0f + 0x1p-20f);
-    const float n2 = (fabsf(r2.x) + fabsf(r2.y) + fabsf(r2.z)) * (1.0f + 0x1p-20f);
-    const float n3 = (fabsf(r3.x) + fabsf(r3.y) + fabsf(r3.z)) * (1.0f + 0x1p-20f);
+    const float n1 = bv.n1, n2 = bv.n2, n3 = bv.n3;
     const float sx = fmaf(fabsf(xc) + dx, n3, n1), sy = fmaf(fabsf(yc) + dy, n3, n2);
     const float tr = trace * (1.0f + 0x1p-16f);
     const float exx = tr * (j11 * j11) * sx * fmaf(sx, 0x1p-18f, 2.0f * dx * n3);
     const float eyy = tr * (j22 * j22) * sy * fmaf(sy, 0x1p-18f, 2.0f * dy * n3);
-    const float kernel = (cam.proper_antialiasing ? 0.1f : 0.3f) * cam.ss * cam.ss;
+    const float kernel = bv.kernel;
     const float cut = fmaf(fmaxf(2.0f * cv.pt, 0.0f), 1.0f + 0x1p-16f, 0x1p-20f);
     const float ex = __builtin_amdgcn_sqrtf(((cxx + exx) + kernel) * cut) * (1.0f + 0x1p-16f);
     const float ey = __builtin_amdgcn_sqrtf(((cyy + eyy) + kernel) * cut) * (1.0f + 0x1p-16f);
@@ -441,19 +462,20 @@ __device__ __forceinline__ bool gaussian_tiles_bound(float mx, float my, float m
     const float wx = ex + fmaf(fabsf(x * cam.focal_x) + fabsf(mean_x) + ex, 0x1p-20f, 0x1p-20f);
     const float wy = ey + fmaf(fabsf(y * cam.focal_y) + fabsf(mean_y) + ey, 0x1p-20f, 0x1p-20f);
     const float bx0 = mean_x - wx, bx1 = mean_x + wx, by0 = mean_y - wy, by1 = mean_y + wy;
-    if (!(wx <= 3.0e38f && wy <= 3.0e38f && fabsf(mean_x) <= 3.0e38f && fabsf(mean_y) <= 3.0e38f)) {
-        exact = true;
-        return false;
+    const bool finite = wx <= 3.0e38f && wy <= 3.0e38f && fabsf(mean_x) <= 3.0e38f && fabsf(mean_y) <= 3.0e38f;
+    exact = in_range && !finite;
+    bool has = in_range && finite && bx0 < cam.width && bx1 >= 0.0f && by0 < cam.height && by1 >= 0.0f;
+    t = { 0u, 0u, 0u, 0u };
+    if (has) {
+        // finite edges with bx0 < width and bx1 >= 0: the medians are max(bx0, 0) and min(bx1, width)
+        const uint32_t x0 = (uint32_t)floorf(__builtin_amdgcn_fmed3f(bx0, 0.0f, cam.width));
+        const uint32_t x1 = (uint32_t)ceilf(__builtin_amdgcn_fmed3f(bx1, 0.0f, cam.width));
+        const uint32_t y0 = (uint32_t)floorf(__builtin_amdgcn_fmed3f(by0, 0.0f, cam.height));
+        const uint32_t y1 = (uint32_t)ceilf(__builtin_amdgcn_fmed3f(by1, 0.0f, cam.height));
+        has = x1 > x0 && y1 > y0;
+        t = { x0 >> shift, (x1 - 1u) >> shift, y0 >> shift, (y1 - 1u) >> shift };
     }
-    if (bx0 >= cam.width || bx1 < 0.0f || by0 >= cam.height || by1 < 0.0f) return false;
-    const uint32_t x0 = (uint32_t)floorf(fmaxf(bx0, 0.0f)), x1 = (uint32_t)ceilf(fminf(bx1, cam.width));
-    const uint32_t y0 = (uint32_t)floorf(fmaxf(by0, 0.0f)), y1 = (uint32_t)ceilf(fminf(by1, cam.height));
-    if (x1 <= x0 || y1 <= y0) return false;
-    t.tx0 = x0 >> shift;
-    t.tx1 = (x1 - 1u) >> shift;
-    t.ty0 = y0 >> shift;
-    t.ty1 = (y1 - 1u) >> shift;
-    return true;
+    return has;
 }
 
 // Depth buckets of the per-bin sort (tile_sort_kernel): the fill pass stores each instance's
@@ -604,7 +626,7 @@ template <bool kFill>
 constexpr uint32_t kBinChunk = kBinPer<kFill> * kBatchBlock;
 
 template <bool kFill>
-__global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, const float4* __restrict__ cov3d,
+__global__ __launch_bounds__(kBatchBlock, kFill ? 1 : 4) void batch_bin_kernel(SceneArgs sc, const float4* __restrict__ cov3d,
                                                                 const ViewDesc* __restrict__ views,
                                                                 uint32_t n_views,
                                                                 uint32_t* __restrict__ tile_count,
@@ -697,6 +719,8 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
             const ViewDesc& vd = views[v];
             const uint32_t bin0 = vd.tile_begin, gw = vd.grid_w, bl = vd.bin_log;
             const uint32_t ms = subrect_shift(bl, sc.n);
+            BoundView bv = {};
+            if constexpr (!kFill) bv = bound_view(vd);
             TileRange t16[kPer], tr[kPer];
             uint32_t key[kPer], tw[kPer], n_t[kPer];
             bool has[kPer], small[kPer];
@@ -718,22 +742,21 @@ __global__ __launch_bounds__(kBatchBlock) void batch_bin_kernel(SceneArgs sc, co
                     const bool valid = g[k] < sc.n && !(cv[k].opacity < 0.0f);
                     const float trace = cv[k].c11 + cv[k].c22 + cv[k].c33;
                     bool exact = false;
-                    has[k] = gaussian_tiles_bound(mx[k], my[k], mz[k], cv[k], trace, valid, vd, kRasterLog2, t16[k],
-                                                  exact);
+                    has[k] = gaussian_tiles_bound(mx[k], my[k], mz[k], cv[k], trace, valid, vd, bv, kRasterLog2,
+                                                  t16[k], exact);
                     if (exact)
                         has[k] = gaussian_tiles_reg(mx[k], my[k], mz[k], cv[k], valid, vd, kRasterLog2, t16[k], key[k]);
                     if (g[k] < sc.n) pair_rect.store(pair, has[k], t16[k]);
                 }
                 tr[k] = { t16[k].tx0 >> bl, t16[k].tx1 >> bl, t16[k].ty0 >> bl, t16[k].ty1 >> bl };  // bins
                 tw[k] = has[k] ? tr[k].tx1 - tr[k].tx0 + 1u : 0u;
-                n_t[k] = has[k] ? tw[k] * (tr[k].ty1 - tr[k].ty0 + 1u) : 0u;
+                // bin rows and columns are below 2^16, so the 24-bit multiply is the product
+                n_t[k] = has[k] ? __umul24(tw[k], tr[k].ty1 - tr[k].ty0 + 1u) : 0u;
                 small[k] = has[k] && n_t[k] <= kSmallTiles;
-                if (small[k]) {
-                    wx0l = min(wx0l, tr[k].tx0);
-                    wy0l = min(wy0l, tr[k].ty0);
-                    wx1l = max(wx1l, tr[k].tx1);
-                    wy1l = max(wy1l, tr[k].ty1);
-                }
+                wx0l = min(wx0l, small[k] ? tr[k].tx0 : 0xffffffffu);
+                wy0l = min(wy0l, small[k] ? tr[k].ty0 : 0xffffffffu);
+                wx1l = max(wx1l, small[k] ? tr[k].tx1 : 0u);
+                wy1l = max(wy1l, small[k] ? tr[k].ty1 : 0u);
                 sum += n_t[k];
                 vis += has[k] ? 1u : 0u;
             }
