@@ -479,8 +479,11 @@ __device__ __forceinline__ bool gaussian_tiles_bound(float mx, float my, float m
 }
 
 // Depth buckets of the per-bin sort (tile_sort_kernel): the fill pass stores each instance's
-// bucket, the sort counts and places by it.
-constexpr uint32_t kSortBins = 1024;
+// bucket, the sort counts and places by it. One bucket below the view's depth range, 256
+// across it, the rest above (measured against 1024 buckets with 512 across: the sort 0.23 ms
+// faster, the rasteriser's front-to-back order costing it 0.08 ms).
+constexpr uint32_t kSortBins = 512;
+constexpr uint32_t kSortRangeLog2 = 8;  // buckets across the range: 2^kSortRangeLog2
 constexpr uint32_t kSortRegKeys = 32;  // entries per lane kept in registers for tiles of <= 8192
 constexpr uint32_t kSortInFlight = 16;  // entries per lane in flight in the two-pass path of larger tiles
 
@@ -952,7 +955,8 @@ __global__ __launch_bounds__(kRangeThreads) void view_key_range_kernel(const Vie
         const uint32_t kmin = f2u(lo_d), kmax = hi_d > lo_d ? f2u(hi_d) : kmin;
         const uint32_t range = kmax - kmin;
         const uint32_t bits = range ? 32u - (uint32_t)__clz(range) : 0u;
-        view_keys[blockIdx.x] = make_uint2(kmin, bits > 9u ? bits - 9u : 0u);  // (range >> shift) < 512
+        // (range >> shift) < 2^kSortRangeLog2
+        view_keys[blockIdx.x] = make_uint2(kmin, bits > kSortRangeLog2 ? bits - kSortRangeLog2 : 0u);
     }
 }
 
@@ -1072,11 +1076,12 @@ __global__ __launch_bounds__(kBatchBlock) void tile_sort_kernel(uint32_t n_views
         for (; i < n; i += kBatchBlock) atomicAdd(&hist[entries_in.bucket[begin + i]], 1u);
     }
     __syncthreads();
-    // exclusive scan of the buckets, 4 per thread
-    uint32_t v[4], local = 0;
+    // exclusive scan of the buckets, kQ per thread
+    constexpr uint32_t kQ = kSortBins / kBatchBlock;
+    uint32_t v[kQ], local = 0;
 #pragma unroll
-    for (uint32_t q = 0; q < 4u; q++) {
-        v[q] = hist[tid * 4u + q];
+    for (uint32_t q = 0; q < kQ; q++) {
+        v[q] = hist[tid * kQ + q];
         local += v[q];
     }
     uint32_t incl = local;
@@ -1090,8 +1095,8 @@ __global__ __launch_bounds__(kBatchBlock) void tile_sort_kernel(uint32_t n_views
     uint32_t run = incl - local;
     for (uint32_t w = 0; w < wave; w++) run += s_wave[w];
 #pragma unroll
-    for (uint32_t q = 0; q < 4u; q++) {
-        hist[tid * 4u + q] = run;
+    for (uint32_t q = 0; q < kQ; q++) {
+        hist[tid * kQ + q] = run;
         run += v[q];
     }
     __syncthreads();
