@@ -485,7 +485,11 @@ __device__ __forceinline__ bool gaussian_tiles_bound(float mx, float my, float m
 constexpr uint32_t kSortBins = 512;
 constexpr uint32_t kSortRangeLog2 = 8;  // buckets across the range: 2^kSortRangeLog2
 constexpr uint32_t kSortRegKeys = 32;  // entries per lane kept in registers for tiles of <= 8192
-constexpr uint32_t kSortInFlight = 16;  // entries per lane in flight in the two-pass path of larger tiles
+constexpr uint32_t kSortInFlight = 16;  // entries per lane in flight beyond the registers' entries
+constexpr uint32_t kSortCountInFlight = 8;  // the same while counting, beside the registers' entries
+constexpr uint32_t kSortStage = 12288;  // buckets held in LDS beyond the registers' entries (24 KB)
+static_assert(kSortStage % (kSortInFlight * 256u) == 0u && kSortStage % (kSortCountInFlight * 256u) == 0u,
+              "staged groups are whole");
 
 __device__ __forceinline__ uint32_t sort_bucket(uint32_t key, uint2 vk)
 {
@@ -1033,6 +1037,7 @@ __global__ __launch_bounds__(kBatchBlock) void tile_sort_kernel(uint32_t n_views
 {
     __shared__ uint32_t hist[kSortBins];
     __shared__ uint32_t s_wave[kBatchBlock / 64u];
+    __shared__ uint16_t s_stage[kSortStage];
     if (ctr->overflow) return;
     const uint32_t gt = order[blockIdx.x];
     const uint32_t begin = offsets[gt], end = offsets[gt + 1u];
@@ -1047,33 +1052,45 @@ __global__ __launch_bounds__(kBatchBlock) void tile_sort_kernel(uint32_t n_views
     const uint32_t n = end - begin, tid = threadIdx.x, lane = lane_id(), wave = tid / 64u;
     for (uint32_t i = tid; i < kSortBins; i += kBatchBlock) hist[i] = 0u;
     __syncthreads();
-    // Tiles of up to kSortRegKeys * 256 instances (most of them) keep each lane's buckets and
-    // instances in registers between the histogram and the scatter: every entry is read once.
-    // (two 16-bit buckets per register, so more workgroups fit a CU)
-    const bool in_regs = n <= kSortRegKeys * kBatchBlock;
+    // The first kSortRegKeys * 256 entries of a bin (all of them in most bins) stay in registers
+    // between the histogram and the scatter (two 16-bit buckets per register, so more
+    // workgroups fit a CU); the buckets of the next kSortStage entries stay in LDS. Only the
+    // buckets of entries beyond both are read twice.
+    constexpr uint32_t kRegEntries = kSortRegKeys * kBatchBlock;
+    const uint32_t n_regs = min(n, kRegEntries);
     uint32_t rb[kSortRegKeys / 2u], rg[kSortRegKeys];
-    if (in_regs) {
 #pragma unroll
-        for (uint32_t u = 0; u < kSortRegKeys; u++) {
-            const uint32_t idx = tid + u * kBatchBlock;
-            const uint32_t b = idx < n ? entries_in.bucket[begin + idx] : 0u;
-            rb[u / 2u] = (u & 1u) ? rb[u / 2u] | (b << 16) : b;
-            rg[u] = idx < n ? entries_in.gaussian[begin + idx] : 0u;
-        }
+    for (uint32_t u = 0; u < kSortRegKeys; u++) {
+        const uint32_t idx = tid + u * kBatchBlock;
+        const uint32_t b = idx < n_regs ? entries_in.bucket[begin + idx] : 0u;
+        rb[u / 2u] = (u & 1u) ? rb[u / 2u] | (b << 16) : b;
+        rg[u] = idx < n_regs ? entries_in.gaussian[begin + idx] : 0u;
+    }
 #pragma unroll
-        for (uint32_t u = 0; u < kSortRegKeys; u++)
-            if (tid + u * kBatchBlock < n) atomicAdd(&hist[(rb[u / 2u] >> (16u * (u & 1u))) & 0xffffu], 1u);
-    } else {
-        // kSortInFlight buckets in flight per lane; the first pass reads the 2-byte buckets only
+    for (uint32_t u = 0; u < kSortRegKeys; u++)
+        if (tid + u * kBatchBlock < n_regs) atomicAdd(&hist[(rb[u / 2u] >> (16u * (u & 1u))) & 0xffffu], 1u);
+    // entries kRegEntries + i, kSortCountInFlight buckets in flight per lane (the registers'
+    // entries are live here); i < kSortStage is uniform over each group of kSortCountInFlight
+    // * 256 (kSortStage is a multiple of it)
+    const uint32_t n_rest = n - n_regs;
+    const uint32_t rbegin = begin + n_regs;
+    {
         uint32_t i = tid;
-        for (; i + (kSortInFlight - 1u) * kBatchBlock < n; i += kSortInFlight * kBatchBlock) {
-            uint32_t k[kSortInFlight];
+        for (; i + (kSortCountInFlight - 1u) * kBatchBlock < n_rest; i += kSortCountInFlight * kBatchBlock) {
+            uint32_t k[kSortCountInFlight];
 #pragma unroll
-            for (uint32_t u = 0; u < kSortInFlight; u++) k[u] = entries_in.bucket[begin + i + u * kBatchBlock];
+            for (uint32_t u = 0; u < kSortCountInFlight; u++) k[u] = entries_in.bucket[rbegin + i + u * kBatchBlock];
+            if (i < kSortStage)
 #pragma unroll
-            for (uint32_t u = 0; u < kSortInFlight; u++) atomicAdd(&hist[k[u]], 1u);
+                for (uint32_t u = 0; u < kSortCountInFlight; u++) s_stage[i + u * kBatchBlock] = (uint16_t)k[u];
+#pragma unroll
+            for (uint32_t u = 0; u < kSortCountInFlight; u++) atomicAdd(&hist[k[u]], 1u);
         }
-        for (; i < n; i += kBatchBlock) atomicAdd(&hist[entries_in.bucket[begin + i]], 1u);
+        for (; i < n_rest; i += kBatchBlock) {
+            const uint32_t k = entries_in.bucket[rbegin + i];
+            if (i < kSortStage) s_stage[i] = (uint16_t)k;
+            atomicAdd(&hist[k], 1u);
+        }
     }
     __syncthreads();
     // exclusive scan of the buckets, kQ per thread
@@ -1113,26 +1130,33 @@ __global__ __launch_bounds__(kBatchBlock) void tile_sort_kernel(uint32_t n_views
         keys_out[begin + p] = bucket_floor(b, vk);
     }
     __syncthreads();
-    if (in_regs) {
 #pragma unroll
-        for (uint32_t u = 0; u < kSortRegKeys; u++)
-            if (tid + u * kBatchBlock < n)
-                inst_out[begin + atomicAdd(&hist[(rb[u / 2u] >> (16u * (u & 1u))) & 0xffffu], 1u)] = rg[u];
-        return;
-    }
+    for (uint32_t u = 0; u < kSortRegKeys; u++)
+        if (tid + u * kBatchBlock < n_regs)
+            inst_out[begin + atomicAdd(&hist[(rb[u / 2u] >> (16u * (u & 1u))) & 0xffffu], 1u)] = rg[u];
     uint32_t i = tid;
-    for (; i + (kSortInFlight - 1u) * kBatchBlock < n; i += kSortInFlight * kBatchBlock) {
+    for (; i + (kSortInFlight - 1u) * kBatchBlock < n_rest; i += kSortInFlight * kBatchBlock) {
         uint32_t k[kSortInFlight], g[kSortInFlight];
+        if (i < kSortStage) {
 #pragma unroll
-        for (uint32_t u = 0; u < kSortInFlight; u++) {
-            k[u] = entries_in.bucket[begin + i + u * kBatchBlock];
-            g[u] = entries_in.gaussian[begin + i + u * kBatchBlock];
+            for (uint32_t u = 0; u < kSortInFlight; u++) {
+                k[u] = s_stage[i + u * kBatchBlock];
+                g[u] = entries_in.gaussian[rbegin + i + u * kBatchBlock];
+            }
+        } else {
+#pragma unroll
+            for (uint32_t u = 0; u < kSortInFlight; u++) {
+                k[u] = entries_in.bucket[rbegin + i + u * kBatchBlock];
+                g[u] = entries_in.gaussian[rbegin + i + u * kBatchBlock];
+            }
         }
 #pragma unroll
         for (uint32_t u = 0; u < kSortInFlight; u++) inst_out[begin + atomicAdd(&hist[k[u]], 1u)] = g[u];
     }
-    for (; i < n; i += kBatchBlock) inst_out[begin + atomicAdd(&hist[entries_in.bucket[begin + i]], 1u)] =
-        entries_in.gaussian[begin + i];
+    for (; i < n_rest; i += kBatchBlock) {
+        const uint32_t k = i < kSortStage ? (uint32_t)s_stage[i] : (uint32_t)entries_in.bucket[rbegin + i];
+        inst_out[begin + atomicAdd(&hist[k], 1u)] = entries_in.gaussian[rbegin + i];
+    }
 }
 
 // Inclusive scan over the wave through DPP (VALU, no LDS round trip): row_shr 1, 2, 4, 8 scan
