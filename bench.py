@@ -247,18 +247,18 @@ def summarize_profile(trace_dir: str, fetch_dir: str, write_dir: str, sq_dir: st
     out = {"workload": "6M Gaussians (8-component mixture, SH3, Morton order), 64 views on a ring, 256x256.."
                        "2048x1152, spp in {1,4,16}; one batch, tile schedule, default tuning",
            "sources_sha256": sources_sha256(), "renders": renders, "calibration": cal, "kernels": kernels,
+           # the commands that wrote the four directories given (their paths as given)
            "commands": {
-               "trace": "rocprofv3 --kernel-trace --stats --output-format csv -d gpurun_out/prof_trace -o run -- "
+               "trace": f"rocprofv3 --kernel-trace --stats --output-format csv -d {trace_dir} -o run -- "
                         "python3 bench.py --profile",
-               "fetch": "rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/prof_fetch -o run -- "
+               "fetch": f"rocprofv3 --pmc FETCH_SIZE --output-format csv -d {fetch_dir} -o run -- "
                         "python3 bench.py --profile",
-               "write": "rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/prof_write -o run -- "
+               "write": f"rocprofv3 --pmc WRITE_SIZE --output-format csv -d {write_dir} -o run -- "
                         "python3 bench.py --profile",
                "sq": "rocprofv3 --pmc SQ_INSTS_VALU SQ_WAVES SQ_BUSY_CYCLES SQ_INSTS_LDS SQ_INSTS_SALU "
                      "SQ_LDS_BANK_CONFLICT SQ_ACTIVE_INST_VALU SQ_ACTIVE_INST_VALU2 GRBM_GUI_ACTIVE "
-                     "--output-format csv -d gpurun_out/prof_sq -o run -- python3 bench.py --profile",
-               "summary": "python3 bench.py --summarize-profile gpurun_out/prof_trace gpurun_out/prof_fetch "
-                          "gpurun_out/prof_write gpurun_out/prof_sq"}}
+                     f"--output-format csv -d {sq_dir} -o run -- python3 bench.py --profile",
+               "summary": f"python3 bench.py --summarize-profile {trace_dir} {fetch_dir} {write_dir} {sq_dir}"}}
     PROFILE_JSON.parent.mkdir(exist_ok=True)
     PROFILE_JSON.write_text(json.dumps(out, indent=1) + "\n")
     lines = [
