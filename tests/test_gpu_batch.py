@@ -335,6 +335,35 @@ def test_batch_rejects_incomplete_feature_views(gpu):
                                        blue_noise=torch.zeros(100, dtype=torch.uint8, device="cuda"))])
 
 
+@pytest.mark.parametrize("n", [6000, 15000, 45000])
+def test_batch_sort_long_bins(gpu, n):
+    """Views of one 16-pixel bin, so that the bin's list holds about every Gaussian: the
+    per-bin sort keeps up to 8192 entries in registers, the buckets of the next 12288 in LDS and
+    reads those of any further entries twice; lists within each range (one- and 16-sample
+    views, whole and half raster tiles) equal the scatter schedule and the plain SPP loop."""
+    import torch
+    from StochasticGSCudaBackend.batch import BatchRenderer, BatchTuning, BatchView
+    arrays = small_scene(n, seed=49, log_scale_mean=-4.5, log_scale_std=0.4)
+    t = to_device(arrays)
+    cams, spps, frames = [small_camera(16, 16, angle=0.5), small_camera(16, 16, angle=2.0)], [16, 1], [21, 77]
+    views = [BatchView(settings_of(c), spp=k, frame_idx=f, bin_size=16) for c, k, f in zip(cams, spps, frames)]
+    print_case(test="batch_sort_long_bins", n=n)
+    tiles = BatchRenderer(t["means"], t["scales"], t["rotations"], t["opacities"], t["sh"], views)
+    scatter = BatchRenderer(t["means"], t["scales"], t["rotations"], t["opacities"], t["sh"], views,
+                            BatchTuning(schedule=2))
+    tiles.render()
+    scatter.render()
+    torch.cuda.synchronize()
+    n_inst, pairs, overflow = tiles.tile_stats()
+    assert not overflow and n_inst == pairs
+    # per view about n entries: below 8192, within the LDS stage, and beyond it
+    lo, hi = {6000: (1, 8192), 15000: (8193, 20480), 45000: (20481, 1 << 30)}[n]
+    assert 2 * lo <= n_inst <= 2 * hi
+    for v, (c, k, f) in enumerate(zip(cams, spps, frames)):
+        assert_close(tiles.outputs[v], scatter.outputs[v])
+        assert_close(tiles.outputs[v], plain_spp(gpu, t, c, f, k))
+
+
 def test_batch_bins_without_subrectangles(gpu):
     """More Gaussians than a 64-pixel bin's instance word leaves index bits for (2^24): the
     fill stores no tile sub-rectangles and every tile of a bin takes every instance; the
