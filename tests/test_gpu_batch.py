@@ -335,6 +335,44 @@ def test_batch_rejects_incomplete_feature_views(gpu):
                                        blue_noise=torch.zeros(100, dtype=torch.uint8, device="cuda"))])
 
 
+@pytest.mark.parametrize("case", ["no_gaussians", "all_culled", "ragged_views"])
+def test_batch_degenerate_inputs(gpu, case):
+    """A scene without Gaussians, views in which every Gaussian is culled by the near plane (no
+    instances at all), and views smaller than one raster tile or one pixel wide or high: the tile
+    schedule equals the scatter schedule and the plain SPP loop."""
+    import torch
+    from StochasticGSCudaBackend.batch import BatchRenderer, BatchTuning, BatchView
+    arrays = small_scene(2500, seed=50, log_scale_mean=-3.3, log_scale_std=0.8)
+    if case == "no_gaussians":
+        arrays = {k: v[:0] for k, v in arrays.items()}
+    t = to_device(arrays)
+    if case == "ragged_views":
+        cams = [small_camera(1, 1, angle=0.3), small_camera(17, 1, angle=1.0), small_camera(1, 33, angle=1.7),
+                small_camera(15, 15, angle=2.4, supersampling_factor=2), small_camera(47, 5, angle=3.1)]
+        spps, frames = [1, 3, 16, 2, 9], [5, 6, 7, 8, 9]
+    else:
+        cams = [small_camera(40, 30, angle=0.6), small_camera(33, 47, angle=2.0)]
+        spps, frames = [4, 16], [31, 32]
+        if case == "all_culled":
+            for c in cams:
+                c["near_plane"], c["far_plane"] = 50.0, 100.0  # the scene lies within a few units of the origin
+    views = [BatchView(settings_of(c), spp=k, frame_idx=f) for c, k, f in zip(cams, spps, frames)]
+    print_case(test="batch_degenerate", case=case)
+    tiles = BatchRenderer(t["means"], t["scales"], t["rotations"], t["opacities"], t["sh"], views)
+    scatter = BatchRenderer(t["means"], t["scales"], t["rotations"], t["opacities"], t["sh"], views,
+                            BatchTuning(schedule=2))
+    tiles.render()
+    scatter.render()
+    torch.cuda.synchronize()
+    n_inst, pairs, overflow = tiles.tile_stats()
+    assert not overflow
+    if case != "ragged_views":
+        assert n_inst == 0 and pairs == 0
+    for v, (c, k, f) in enumerate(zip(cams, spps, frames)):
+        assert_close(tiles.outputs[v], scatter.outputs[v])
+        assert_close(tiles.outputs[v], plain_spp(gpu, t, c, f, k))
+
+
 @pytest.mark.parametrize("n", [6000, 15000, 45000])
 def test_batch_sort_long_bins(gpu, n):
     """Views of one 16-pixel bin, so that the bin's list holds about every Gaussian: the
